@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""SU(3) checkerboard Metropolis throughput on one MI355X.
+
+One step = one full checkerboard sweep (mu = 0..3, parity 0/1: 8 sub-pass
+launches) of single-generator Metropolis over an SU(3) Wilson-action
+lattice, 8 proposals per link (the reference sweep's proposal set,
+su3_ensemble.rs:342-391), links stored FP32, accept test and
+reunitarisation FP64.  Default workload: 32^4, beta = 6.0, proposal angle
+0.35, Haar-random hot start; the links stay resident in HBM for the whole
+timed region.
+
+Prints one JSON line with the throughput (link updates/s), the roofline of
+the dominant kernel (k_su3_metropolis: algorithmic bytes per launch / mean
+launch duration from HIP events on its stream) and the CPU baseline (the
+reference-exact restatement of metropolis_sweep timed on the host cores).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import threading
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "yang-mills_amd"))
+
+METRIC = "SU(3) link updates/s (% HBM roofline) + plaquette ESS/s learned vs plain"
+HBM_PEAK_GBS = 8000.0
+# Compulsory traffic per link update of one checkerboard sub-pass: the updated
+# link is read and written, the opposite-parity mu links and the three other
+# directions (both parities) are read once: 4V reads + V/2 writes per V/2
+# updates = 9 link transfers of 18 FP32 = 72 bytes.
+BYTES_PER_UPDATE_FP32 = 9 * 72
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds_target):
+    """Reference-exact metropolis_sweep (oracle restatement) on host threads."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import numpy as np
+
+    import oracle_lib as O
+
+    L = O.lib()
+    cores = max(1, min(16, os.cpu_count() or 1))
+    n = (6, 6, 6, 6)
+    links_per_sweep = 4 * 6 ** 4
+    # one sweep of 6^4 takes ~1.1 s per core; size the sample to the target
+    sweeps = max(1, int(round(seconds_target / 1.2)))
+    done = [0] * cores
+
+    def work(i):
+        links = O.su3_identity(n)
+        rng = C.c_uint64(O.lib().ymo_derive_su3_chain_seed(0x2C02198105010002, i))
+        p, a = C.c_uint64(), C.c_uint64()
+        for _ in range(sweeps):
+            st = L.ymo_su3_metropolis_sweep(O.ext(n), O.dptr(links), 6.0, 0.35, C.byref(rng), C.byref(p), C.byref(a))
+            assert st == 0
+            done[i] += links_per_sweep
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    total = sum(done)
+    return {
+        "value": total / dt,
+        "unit": "link updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"{cores} threads x {sweeps} reference-order sweeps of SU(3) 6^4 at beta=6.0 "
+                   f"(8 checked-product Metropolis proposals per link, SplitMix64), {total} link updates in {dt:.1f} s"),
+    }
+
+
+def load_traffic(extent):
+    path = os.path.join(REPO, "profiles", f"su3_metropolis_{extent}x4_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--extent", type=int, default=32)
+    ap.add_argument("--hits", type=int, default=8)
+    ap.add_argument("--beta", type=float, default=6.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              "(independent chains shard across GPUs with no communication; run one process per GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ym_amd
+
+    L = args.extent
+    n = (L, L, L, L)
+    V = L ** 4
+    angle = 0.35
+    key = ym_amd.derive_su3_chain_seed(0x2C02198105010002, 0)
+    log(f"SU(3) {L}^4 FP32 hot start, beta={args.beta}, hits={args.hits}")
+    lat = ym_amd.Su3Lattice4(n, 32)
+    lat.hot_start(key)
+    log(f"warmup {args.warmup} sweeps")
+    lat.metropolis_sweeps(args.beta, angle, key, 0, args.warmup, args.hits)
+    log(f"timed {args.steps} sweeps")
+    t0 = time.perf_counter()
+    prop, acc, total_ms, ksum_ms, launches = lat.metropolis_sweeps_profiled(
+        args.beta, angle, key, args.warmup, args.steps, args.hits)
+    wall = time.perf_counter() - t0
+    plaq = lat.average_plaquette()
+    unit_res, det_res = lat.link_residuals()
+    updates = 4 * V * args.steps
+    value = updates / wall
+    avg_launch_s = ksum_ms / launches / 1e3
+    bytes_per_launch = (V // 2) * BYTES_PER_UPDATE_FP32
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    log(f"{value:.4e} link updates/s, kernel avg {avg_launch_s * 1e6:.1f} us, {achieved:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "link updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Haar-random hot start from Philox4x32-10, seed derive_su3_chain_seed(0x2c02198105010002, 0)",
+        "config": {
+            "workload": f"SU(3) Wilson action {L}^4, beta={args.beta}, single-generator Metropolis, "
+                        f"{args.hits} proposals/link/sweep, checkerboard sweeps",
+            "extents": list(n),
+            "beta": args.beta,
+            "proposal_angle": angle,
+            "hits_per_link": args.hits,
+            "link_storage": "fp32 (18 reals)",
+            "accept_test": "fp64",
+            "reunitarisation": "fp64 per update",
+            "acceptance": acc / prop,
+            "final_plaquette": plaq,
+            "unitarity_residual_fp32_storage": unit_res,
+            "determinant_residual_fp32_storage": det_res,
+            "device_time_ms_per_step": total_ms / args.steps,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(L),
+            "kernel": "k_su3_metropolis",
+            "bytes_per_launch": bytes_per_launch,
+            "mean_launch_us": avg_launch_s * 1e6,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,233 @@
+/*
+ * ym_amd.h -- C ABI of the MI355X (gfx950) lattice link-update and
+ * gauge-observable library (libym_amd.so).
+ *
+ * Drop-in boundary for the hot path of QSOLKCB/YANG-MILLS.  The reference
+ * exposes Rust module functions compiled into each binary; each entry point
+ * below names the reference function it replaces (file:line under
+ * reference/rust/src/).  Conventions:
+ *   - every function returns an int32_t status: YM_OK (0) or a code whose
+ *     value enumerates the reference's error variants (lib.rs:53-73,
+ *     su2_ensemble.rs:110-121, su3.rs:292-300, su3_lattice.rs:27-39,
+ *     su3_ensemble.rs:124-137, glueball_*.rs); no C++ exception crosses
+ *     the ABI and no call aborts the process;
+ *   - outputs are written through caller-owned pointers; the library owns
+ *     device memory, the caller owns host buffers;
+ *   - host lattice buffers use the reference AoS FP64 layout: link index
+ *     site*4 + mu, site lexicographic with x3 fastest (lib.rs:284-289);
+ *     SU(3) links are 18 doubles (row-major 3x3, re/im interleaved,
+ *     su3.rs:82-85), SU(2) links 4 doubles (a0..a3, U = a0 I - i a.sigma,
+ *     lib.rs:1-12);
+ *   - each lattice handle owns one HIP stream; calls are synchronous and a
+ *     handle must not be used from two host threads at once.
+ */
+#ifndef YM_AMD_H
+#define YM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+enum {
+    YM_OK = 0,
+    YM_INVALID_EXTENT = 1,            /* ReferenceError::InvalidLatticeExtent */
+    YM_LATTICE_TOO_LARGE = 2,         /* ReferenceError::LatticeTooLarge (device memory) */
+    YM_ALLOCATION_FAILED = 3,         /* ReferenceError::AllocationFailed */
+    YM_DIRECTION_OUT_OF_RANGE = 4,    /* ReferenceError::DirectionOutOfRange */
+    YM_DIRECTIONS_MUST_DIFFER = 5,
+    YM_NON_FINITE_BETA = 6,
+    YM_NEGATIVE_BETA = 7,
+    YM_NON_FINITE_ACTION = 8,
+    YM_ZERO_RECTANGLE_EXTENT = 12,
+    YM_COORDINATE_OUT_OF_RANGE = 17,  /* ReferenceError::CoordinateOutOfRange */
+    YM_BETA_OUT_OF_RANGE = 30,        /* Su3EnsembleError / EnsembleError::BetaOutsideReferenceRange */
+    YM_ANGLE_OUT_OF_RANGE = 31,       /* Su3EnsembleError::ProposalAngleOutsideReferenceRange */
+    YM_MEASUREMENT_COUNT = 32,
+    YM_ZERO_STRIDE = 33,
+    YM_LAG_OUT_OF_RANGE = 34,
+    YM_SCHEDULE_OUT_OF_RANGE = 35,
+    YM_NON_FINITE_STATISTIC = 37,
+    YM_TOO_MANY_OR = 40,
+    YM_ZERO_ATTEMPT_LIMIT = 41,       /* EnsembleError::ZeroHeatbathAttemptLimit */
+    YM_HEATBATH_ATTEMPT_LIMIT = 42,   /* EnsembleError::HeatbathAttemptLimitExceeded */
+    YM_TIME_SLICE_OUT_OF_RANGE = 50,
+    YM_SPATIAL_EXTENT_TOO_SMALL = 51,
+    YM_SPATIAL_NOT_CUBIC = 52,
+    YM_NON_FINITE_RHO = 60,
+    YM_RHO_OUT_OF_RANGE = 61,
+    YM_TOO_MANY_LEVELS = 62,
+    YM_EMPTY_CONFIGURATION_SET = 70,
+    YM_TOO_MANY_CONFIGURATIONS = 71,
+    YM_TIME_EXTENT_TOO_LARGE = 72,
+    YM_CORRELATOR_SHAPE = 75,
+    YM_EFF_INVALID_LENGTH = 80,
+    YM_EFF_INVALID_LAG = 81,
+    YM_EFF_INVALID_WEIGHT = 82,
+    YM_EFF_INVALID_COST = 83,
+    YM_EFF_INVALID_DIAGNOSTIC = 84,
+    YM_EFF_DEGENERATE = 85,
+    YM_EFF_NON_FINITE = 86,
+    YM_INVALID_ARGUMENT = 96,         /* null pointer, wrong group/precision, size mismatch */
+    YM_NO_DEVICE = 97,
+    YM_HIP_ERROR = 98,
+    YM_UNSUPPORTED = 99               /* e.g. a checkerboard sweep on an odd extent */
+};
+
+typedef struct ym_lattice ym_lattice;
+
+/* ------------------------------------------------------- lattice handles */
+/* Lattice4::new_identity (lib.rs:237-263) / Su3Lattice4::new_identity
+ * (su3_lattice.rs:85-106): group 2 or 3, precision 32 (FP32 link storage)
+ * or 64 (FP64 link storage), every extent >= 2.  Links start at identity.
+ * The reference's 2^20 / 2^18 link caps do not apply (device memory only). */
+int32_t ym_lattice_create(int32_t group, const int64_t extents[4], int32_t precision, ym_lattice **out);
+int32_t ym_lattice_destroy(ym_lattice *lat);
+/* Device bytes held by the link field. */
+int32_t ym_lattice_bytes(const ym_lattice *lat, int64_t *bytes);
+int32_t ym_lattice_set_identity(ym_lattice *lat);
+/* Hot start (no reference counterpart; the reference is cold-start only,
+ * su2_ensemble.rs:490, su3_ensemble.rs:412): Haar-random links from
+ * Philox4x32-10 keyed by `seed`, counter (site, mu, draw). */
+int32_t ym_lattice_hot_start(ym_lattice *lat, uint64_t seed);
+/* Lattice4::set_link / Su3Lattice4::set_link for every link at once
+ * (lib.rs:323-331, su3_lattice.rs:154-162): `count` = links * (4 or 18). */
+int32_t ym_lattice_upload(ym_lattice *lat, const double *aos, int64_t count);
+/* Lattice4::link / Su3Lattice4::link for every link (lib.rs:319-321). */
+int32_t ym_lattice_download(const ym_lattice *lat, double *aos, int64_t count);
+/* Clone (same group, precision and extents). */
+int32_t ym_lattice_copy(ym_lattice *dst, const ym_lattice *src);
+
+/* ------------------------------------------------------------- samplers */
+/* su3_ensemble.rs:342-391 metropolis_sweep, as `n_sweeps` even/odd
+ * checkerboard sweeps (mu = 0..3, parity 0 then 1).  Per link: `hits`
+ * single-generator proposals, generator h mod 8 (hits = 8 is one reference
+ * sweep's proposal set), angle from the reference's paired grid
+ * (su3_ensemble.rs:199-213), random numbers from Philox4x32-10 keyed by
+ * `key` (derive_su3_chain_seed, su3_ensemble.rs:189) with counter
+ * (site, mu, hit, first_sweep + k).  Accept iff dS <= 0 or u < exp(-dS) in
+ * FP64.  reunitarize != 0 applies FP64 Gram-Schmidt to every updated link
+ * before it is stored (always on for FP32 storage).  Bounds as the
+ * reference: 0 <= beta <= 12, 1e-8 <= angle <= 1.  Counts are accumulated
+ * into *proposals / *accepts (not reset).  Requires even extents. */
+int32_t ym_su3_metropolis_sweeps(ym_lattice *lat, double beta, double proposal_angle, int32_t hits, uint64_t key,
+                                 uint64_t first_sweep, int32_t n_sweeps, int32_t reunitarize, uint64_t *proposals,
+                                 uint64_t *accepts);
+/* Profiling variant of ym_su3_metropolis_sweeps (same arithmetic and
+ * results): HIP events on the handle's stream around every sub-pass launch.
+ * *total_ms = first start to last stop; *kernel_ms_sum = sum of the
+ * per-launch durations; *launches = 8 * n_sweeps.  No reference
+ * counterpart (the reference has no timers). */
+int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *lat, double beta, double proposal_angle, int32_t hits,
+                                          uint64_t key, uint64_t first_sweep, int32_t n_sweeps, int32_t reunitarize,
+                                          uint64_t *proposals, uint64_t *accepts, float *total_ms,
+                                          float *kernel_ms_sum, int32_t *launches);
+/* su2_ensemble.rs:321-342 heatbath_sweep, checkerboard order; attempts of a
+ * link draw Philox(site, mu, attempt, sweep).  0 <= beta <= 8,
+ * max_attempts > 0.  A link that exhausts its attempts keeps its value and
+ * the call returns YM_HEATBATH_ATTEMPT_LIMIT after the sweeps finish.
+ * *proposals (attempts) and *updates are accumulated. */
+int32_t ym_su2_heatbath_sweeps(ym_lattice *lat, double beta, uint64_t key, uint64_t first_sweep, int32_t n_sweeps,
+                               int32_t max_attempts, uint64_t *proposals, uint64_t *updates);
+/* su2_ensemble.rs:283-299, 344-346 overrelaxation_sweep, checkerboard order. */
+int32_t ym_su2_overrelax_sweeps(ym_lattice *lat, int32_t n_sweeps);
+
+/* ----------------------------------------------------------- observables */
+/* su2_ensemble.rs:348-373 / su3_ensemble.rs:228-259 average_plaquette:
+ * (1/6V) sum_x sum_{mu<nu} (1/N) Re Tr P_{mu nu}(x), FP64 accumulation. */
+int32_t ym_average_plaquette(const ym_lattice *lat, double *out);
+/* lib.rs:363-382 / su3_lattice.rs:214-242 wilson_action: beta * sum (1 - P). */
+int32_t ym_wilson_action(const ym_lattice *lat, double beta, double *out);
+/* lib.rs:507-546 staple (SU(2)); for SU(3) the same six-term sum A with
+ * Re Tr(U A) = sum of the six plaquettes touching U (su3_ensemble.rs:261-307).
+ * out: 4 (SU(2)) or 18 (SU(3)) doubles, computed as in the sweep kernels. */
+int32_t ym_staple(const ym_lattice *lat, const int64_t x[4], int32_t mu, double *out);
+/* Averaged rectangular Wilson loops: for 1 <= R <= rmax, 1 <= T <= tmax,
+ * the mean over all sites and the planes of `plane_set` of
+ * rectangular_wilson_loop(x, mu, nu, R, T) (lib.rs:471-481,
+ * path_holonomy.rs:219-260 + wilson_loop :172-181): R steps along mu, T along
+ * nu.  plane_set 0: all 12 ordered (mu, nu); 1: mu spatial, nu = 3 (time);
+ * 2: the 6 ordered spatial pairs.  re/im: rmax*tmax doubles, index
+ * (R-1)*tmax + (T-1); im is 0 for SU(2). */
+int32_t ym_wilson_loops(const ym_lattice *lat, int32_t rmax, int32_t tmax, int32_t plane_set, double *re, double *im);
+/* Polyakov loop averaged over all starting sites of the hyperplane
+ * x_dir = 0 (lib.rs:483-505 SU(2) (1/2)Re Tr, su3_lattice.rs:185-212 SU(3)
+ * (1/3)Tr): *re, *im of the mean, *abs = mean of |P(x)|. */
+int32_t ym_polyakov_loop(const ym_lattice *lat, int32_t dir, double *re, double *im, double *abs_mean);
+/* glueball_operators.rs:143-160 scalar_a1pp_operator_basis for every time
+ * slice t = 0..N3-1 at once: plaq_t[t] (1x1, 3 orientations) and rect_t[t]
+ * (2x1, 6 orientations), each the mean of (1/3) Re Tr over L^3 sites.
+ * SU(3) only; spatial extents equal and >= 3. */
+int32_t ym_su3_timeslice_a1pp(const ym_lattice *lat, double *plaq_t, double *rect_t);
+/* glueball_smearing.rs:257-268 spatial_stout_smear: `levels` synchronous
+ * spatial stout steps with parameter rho (0 < rho <= 0.25) from src into
+ * dst (same shape; src is not modified; temporal links copied).  The
+ * reference's 8-level cap is lifted to 64. */
+int32_t ym_su3_stout_spatial(ym_lattice *dst, const ym_lattice *src, double rho, int32_t levels);
+/* su3_lattice.rs:285-297 max_link_{unitarity,determinant}_residual (SU(3));
+ * lib.rs:696-712 max_link_unit_residual in *unit for SU(2) (*det = 0). */
+int32_t ym_link_residuals(const ym_lattice *lat, double *unit, double *det);
+/* FP64 Gram-Schmidt (SU(3)) / normalisation (SU(2)) of every link; no
+ * reference counterpart (the reference fails closed on residuals). */
+int32_t ym_reunitarize(ym_lattice *lat);
+
+/* ------------------------------------------------------ chain drivers */
+typedef struct {
+    int64_t extents[4];
+    double beta;
+    uint64_t master_seed, chain_id;
+    int64_t thermalization_sweeps, measurements, sweeps_between_measurements;
+    double proposal_angle;
+    int64_t autocorrelation_max_lag;
+    int32_t hits;        /* proposals per link per sweep (8 = reference) */
+    int32_t precision;   /* 32 or 64 */
+    int32_t hot_start;   /* 0 = cold (reference), 1 = Haar-random from the chain key */
+    int32_t reunitarize; /* FP64 reunitarisation per update (forced for FP32) */
+} ym_su3_chain_config;
+
+typedef struct {
+    double mean, variance;
+    int64_t positive_window_lag;
+    double tau_int, ess, standard_error;
+} ym_chain_diagnostics;
+
+typedef struct {
+    uint64_t derived_seed;
+    int64_t total_sweeps;
+    uint64_t proposals, accepts;
+    double mean_acceptance;
+    double final_unitarity_residual, final_determinant_residual;
+    ym_chain_diagnostics diagnostics;
+} ym_su3_chain_report;
+
+/* su3_ensemble.rs:408-480 run_chain on the device: validate (beta, angle,
+ * schedule; no size cap), key = derive_su3_chain_seed(master, chain)
+ * (su3_ensemble.rs:189-192), thermalisation, then `measurements` x
+ * (`sweeps_between_measurements` sweeps + average plaquette), diagnostics
+ * (su2_ensemble.rs:375-463), final residuals.  measurements_out holds
+ * config->measurements doubles. */
+int32_t ym_su3_run_chain(const ym_su3_chain_config *config, double *measurements_out, ym_su3_chain_report *report);
+
+/* ------------------------------------------------- host FP64 statistics */
+/* su2_ensemble.rs:375-463 diagnostics.  autocorr (optional) gets max_lag+1. */
+int32_t ym_diagnostics(const double *values, int64_t n, int64_t max_lag, ym_chain_diagnostics *out, double *autocorr);
+/* su2_ensemble.rs:46-55 mix64 / derive_chain_seed and su3_ensemble.rs:189-192. */
+uint64_t ym_derive_chain_seed(uint64_t master_seed, uint64_t chain_id);
+uint64_t ym_derive_su3_chain_seed(uint64_t master_seed, uint64_t chain_id);
+
+/* --------------------------------------------------------------- timing */
+/* HIP events on the handle's stream around the next calls: begin records
+ * the start event; end records the stop event, synchronises and returns the
+ * elapsed milliseconds. */
+int32_t ym_timer_begin(ym_lattice *lat);
+int32_t ym_timer_end(ym_lattice *lat, float *ms);
+
+/* Human-readable name of a status code (static storage). */
+const char *ym_status_string(int32_t status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,580 @@
+/*
+ * ORACLE -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Per-precision body of the device-order CPU replay, instantiated by
+ * dev_exact.c with REAL = float (RFMA = fmaf) and REAL = double (RFMA = fma).
+ * Storage: links[((mu*2 + parity)*NC + c)*Vh + slot].
+ */
+#define YM_CAT2(a, b) a##_##b
+#define YM_CAT(a, b) YM_CAT2(a, b)
+#define F(name) YM_CAT(name, RSUF)
+
+static inline REAL *F(lk)(REAL *L, const geom *g, int mu, int p, int nc, int c, int64_t s) {
+    return L + (((int64_t)(mu * 2 + p) * nc + c) * g->vh + s);
+}
+static void F(load_link)(const REAL *L, const geom *g, const int64_t x[4], int mu, int nc, REAL *out) {
+    int p = parity_of(x);
+    int64_t s = slot_of(g, x);
+    for (int c = 0; c < nc; ++c) out[c] = L[((int64_t)(mu * 2 + p) * nc + c) * g->vh + s];
+}
+static void F(store_link)(REAL *L, const geom *g, const int64_t x[4], int mu, int nc, const REAL *v) {
+    int p = parity_of(x);
+    int64_t s = slot_of(g, x);
+    for (int c = 0; c < nc; ++c) L[((int64_t)(mu * 2 + p) * nc + c) * g->vh + s] = v[c];
+}
+
+static int F(to_device)(const int64_t n[4], const double *aos, void *dev, int nc) {
+    geom g = make_geom(n);
+    REAL *L = (REAL *)dev;
+    /* padding slots hold the identity */
+    for (int mu = 0; mu < 4; ++mu) for (int p = 0; p < 2; ++p) for (int c = 0; c < nc; ++c) for (int64_t s = 0; s < g.vh; ++s) {
+        REAL v = 0;
+        if (nc == 18 && (c == 0 || c == 8 || c == 16)) v = 1;
+        if (nc == 4 && c == 0) v = 1;
+        *F(lk)(L, &g, mu, p, nc, c, s) = v;
+    }
+    int64_t x[4];
+    for (x[0] = 0; x[0] < n[0]; ++x[0]) for (x[1] = 0; x[1] < n[1]; ++x[1]) for (x[2] = 0; x[2] < n[2]; ++x[2]) for (x[3] = 0; x[3] < n[3]; ++x[3]) {
+        int64_t site = ((x[0] * n[1] + x[1]) * n[2] + x[2]) * n[3] + x[3];
+        for (int mu = 0; mu < 4; ++mu) {
+            REAL v[18];
+            for (int c = 0; c < nc; ++c) v[c] = (REAL)aos[(site * 4 + mu) * nc + c];
+            F(store_link)(L, &g, x, mu, nc, v);
+        }
+    }
+    return YMO_OK;
+}
+static int F(from_device)(const int64_t n[4], const void *dev, double *aos, int nc) {
+    geom g = make_geom(n);
+    const REAL *L = (const REAL *)dev;
+    int64_t x[4];
+    for (x[0] = 0; x[0] < n[0]; ++x[0]) for (x[1] = 0; x[1] < n[1]; ++x[1]) for (x[2] = 0; x[2] < n[2]; ++x[2]) for (x[3] = 0; x[3] < n[3]; ++x[3]) {
+        int64_t site = ((x[0] * n[1] + x[1]) * n[2] + x[2]) * n[3] + x[3];
+        for (int mu = 0; mu < 4; ++mu) {
+            REAL v[18];
+            F(load_link)(L, &g, x, mu, nc, v);
+            for (int c = 0; c < nc; ++c) aos[(site * 4 + mu) * nc + c] = (double)v[c];
+        }
+    }
+    return YMO_OK;
+}
+
+/* ---------------------------------------------------- SU(3) staple (REAL) */
+/* element (i,j) of a 3x3 complex matrix stored as 18 reals */
+#define RE(m, i, j) (m)[2 * (3 * (i) + (j))]
+#define IM(m, i, j) (m)[2 * (3 * (i) + (j)) + 1]
+
+/* T = X Y^dagger */
+static void F(mul_nd)(const REAL *X, const REAL *Y, REAL *T) {
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+        REAL re = RE(X, i, 0) * RE(Y, j, 0);
+        re = RFMA(IM(X, i, 0), IM(Y, j, 0), re);
+        re = RFMA(RE(X, i, 1), RE(Y, j, 1), re);
+        re = RFMA(IM(X, i, 1), IM(Y, j, 1), re);
+        re = RFMA(RE(X, i, 2), RE(Y, j, 2), re);
+        re = RFMA(IM(X, i, 2), IM(Y, j, 2), re);
+        REAL im = IM(X, i, 0) * RE(Y, j, 0);
+        im = RFMA(-RE(X, i, 0), IM(Y, j, 0), im);
+        im = RFMA(IM(X, i, 1), RE(Y, j, 1), im);
+        im = RFMA(-RE(X, i, 1), IM(Y, j, 1), im);
+        im = RFMA(IM(X, i, 2), RE(Y, j, 2), im);
+        im = RFMA(-RE(X, i, 2), IM(Y, j, 2), im);
+        RE(T, i, j) = re;
+        IM(T, i, j) = im;
+    }
+}
+/* T = X^dagger Y^dagger */
+static void F(mul_dd)(const REAL *X, const REAL *Y, REAL *T) {
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+        REAL re = RE(X, 0, i) * RE(Y, j, 0);
+        re = RFMA(-IM(X, 0, i), IM(Y, j, 0), re);
+        re = RFMA(RE(X, 1, i), RE(Y, j, 1), re);
+        re = RFMA(-IM(X, 1, i), IM(Y, j, 1), re);
+        re = RFMA(RE(X, 2, i), RE(Y, j, 2), re);
+        re = RFMA(-IM(X, 2, i), IM(Y, j, 2), re);
+        REAL im = (-RE(X, 0, i)) * IM(Y, j, 0);
+        im = RFMA(-IM(X, 0, i), RE(Y, j, 0), im);
+        im = RFMA(-RE(X, 1, i), IM(Y, j, 1), im);
+        im = RFMA(-IM(X, 1, i), RE(Y, j, 1), im);
+        im = RFMA(-RE(X, 2, i), IM(Y, j, 2), im);
+        im = RFMA(-IM(X, 2, i), RE(Y, j, 2), im);
+        RE(T, i, j) = re;
+        IM(T, i, j) = im;
+    }
+}
+/* A += T Z^dagger */
+static void F(acc_nd)(REAL *A, const REAL *T, const REAL *Z) {
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+        REAL re = RE(A, i, j), im = IM(A, i, j);
+        for (int k = 0; k < 3; ++k) {
+            re = RFMA(RE(T, i, k), RE(Z, j, k), re);
+            re = RFMA(IM(T, i, k), IM(Z, j, k), re);
+        }
+        for (int k = 0; k < 3; ++k) {
+            im = RFMA(IM(T, i, k), RE(Z, j, k), im);
+            im = RFMA(-RE(T, i, k), IM(Z, j, k), im);
+        }
+        RE(A, i, j) = re;
+        IM(A, i, j) = im;
+    }
+}
+/* A += T Z */
+static void F(acc_nn)(REAL *A, const REAL *T, const REAL *Z) {
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+        REAL re = RE(A, i, j), im = IM(A, i, j);
+        for (int k = 0; k < 3; ++k) {
+            re = RFMA(RE(T, i, k), RE(Z, k, j), re);
+            re = RFMA(-IM(T, i, k), IM(Z, k, j), re);
+        }
+        for (int k = 0; k < 3; ++k) {
+            im = RFMA(RE(T, i, k), IM(Z, k, j), im);
+            im = RFMA(IM(T, i, k), RE(Z, k, j), im);
+        }
+        RE(A, i, j) = re;
+        IM(A, i, j) = im;
+    }
+}
+
+/* A = sum_{nu != mu} [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)] */
+static void F(su3_staple)(const REAL *L, const geom *g, const int64_t x[4], int mu, REAL *A) {
+    for (int c = 0; c < 18; ++c) A[c] = 0;
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        int64_t xpm[4], xpn[4], xmn[4], xpmmn[4];
+        shift(g, x, mu, +1, xpm);
+        shift(g, x, nu, +1, xpn);
+        shift(g, x, nu, -1, xmn);
+        shift(g, xpm, nu, -1, xpmmn);
+        REAL X[18], Y[18], Z[18], T[18];
+        F(load_link)(L, g, xpm, nu, 18, X);
+        F(load_link)(L, g, xpn, mu, 18, Y);
+        F(mul_nd)(X, Y, T);
+        F(load_link)(L, g, x, nu, 18, Z);
+        F(acc_nd)(A, T, Z);
+        F(load_link)(L, g, xpmmn, nu, 18, X);
+        F(load_link)(L, g, xmn, mu, 18, Y);
+        F(mul_dd)(X, Y, T);
+        F(load_link)(L, g, xmn, nu, 18, Z);
+        F(acc_nn)(A, T, Z);
+    }
+}
+static int F(su3_staple_out)(const int64_t n[4], const void *dev, const int64_t x[4], int mu, double *out18) {
+    geom g = make_geom(n);
+    REAL A[18];
+    F(su3_staple)((const REAL *)dev, &g, x, mu, A);
+    for (int c = 0; c < 18; ++c) out18[c] = (double)A[c];
+    return YMO_OK;
+}
+
+/* ---------------------------------------------------- FP64 link algebra */
+#ifndef YM_ROT_DEFINED
+#define YM_ROT_DEFINED
+/* rows i,j <- [[c, i s],[i s, c]] rows (lambda_1-type block) */
+static inline void rot_sym(double *m, int i, int j, double c, double s) {
+    for (int k = 0; k < 3; ++k) {
+        double xr = RE(m, i, k), xi = IM(m, i, k), yr = RE(m, j, k), yi = IM(m, j, k);
+        RE(m, i, k) = fma(c, xr, -(s * yi));
+        IM(m, i, k) = fma(c, xi, s * yr);
+        RE(m, j, k) = fma(c, yr, -(s * xi));
+        IM(m, j, k) = fma(c, yi, s * xr);
+    }
+}
+/* rows i,j <- [[c, s],[-s, c]] rows (lambda_2-type block) */
+static inline void rot_asym(double *m, int i, int j, double c, double s) {
+    for (int k = 0; k < 3; ++k) {
+        double xr = RE(m, i, k), xi = IM(m, i, k), yr = RE(m, j, k), yi = IM(m, j, k);
+        RE(m, i, k) = fma(c, xr, s * yr);
+        IM(m, i, k) = fma(c, xi, s * yi);
+        RE(m, j, k) = fma(c, yr, -(s * xr));
+        IM(m, j, k) = fma(c, yi, -(s * xi));
+    }
+}
+/* row i <- (c + i s) row i */
+static inline void rot_phase(double *m, int i, double c, double s) {
+    for (int k = 0; k < 3; ++k) {
+        double xr = RE(m, i, k), xi = IM(m, i, k);
+        RE(m, i, k) = fma(c, xr, -(s * xi));
+        IM(m, i, k) = fma(c, xi, s * xr);
+    }
+}
+#endif
+
+#ifndef YM_REUNIT_DEFINED
+#define YM_REUNIT_DEFINED
+/* Gram-Schmidt on rows 0,1 and row2 = conj(row0 x row1), all FP64. */
+static void reunitarize3(double *u) {
+    double n = RE(u, 0, 0) * RE(u, 0, 0);
+    n = fma(IM(u, 0, 0), IM(u, 0, 0), n);
+    n = fma(RE(u, 0, 1), RE(u, 0, 1), n);
+    n = fma(IM(u, 0, 1), IM(u, 0, 1), n);
+    n = fma(RE(u, 0, 2), RE(u, 0, 2), n);
+    n = fma(IM(u, 0, 2), IM(u, 0, 2), n);
+    double inv = 1.0 / sqrt(n);
+    for (int k = 0; k < 3; ++k) { RE(u, 0, k) *= inv; IM(u, 0, k) *= inv; }
+    double pr = RE(u, 0, 0) * RE(u, 1, 0);
+    pr = fma(IM(u, 0, 0), IM(u, 1, 0), pr);
+    pr = fma(RE(u, 0, 1), RE(u, 1, 1), pr);
+    pr = fma(IM(u, 0, 1), IM(u, 1, 1), pr);
+    pr = fma(RE(u, 0, 2), RE(u, 1, 2), pr);
+    pr = fma(IM(u, 0, 2), IM(u, 1, 2), pr);
+    double pi = RE(u, 0, 0) * IM(u, 1, 0);
+    pi = fma(-IM(u, 0, 0), RE(u, 1, 0), pi);
+    pi = fma(RE(u, 0, 1), IM(u, 1, 1), pi);
+    pi = fma(-IM(u, 0, 1), RE(u, 1, 1), pi);
+    pi = fma(RE(u, 0, 2), IM(u, 1, 2), pi);
+    pi = fma(-IM(u, 0, 2), RE(u, 1, 2), pi);
+    for (int k = 0; k < 3; ++k) {
+        double ar = RE(u, 0, k), ai = IM(u, 0, k);
+        RE(u, 1, k) = fma(-pr, ar, fma(pi, ai, RE(u, 1, k)));
+        IM(u, 1, k) = fma(-pr, ai, fma(-pi, ar, IM(u, 1, k)));
+    }
+    n = RE(u, 1, 0) * RE(u, 1, 0);
+    n = fma(IM(u, 1, 0), IM(u, 1, 0), n);
+    n = fma(RE(u, 1, 1), RE(u, 1, 1), n);
+    n = fma(IM(u, 1, 1), IM(u, 1, 1), n);
+    n = fma(RE(u, 1, 2), RE(u, 1, 2), n);
+    n = fma(IM(u, 1, 2), IM(u, 1, 2), n);
+    inv = 1.0 / sqrt(n);
+    for (int k = 0; k < 3; ++k) { RE(u, 1, k) *= inv; IM(u, 1, k) *= inv; }
+    /* row2_k = conj(a_p b_q - a_q b_p) for (k,p,q) cyclic */
+    static const int cyc[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+    for (int k = 0; k < 3; ++k) {
+        int p = cyc[k][0], q = cyc[k][1];
+        double ar = RE(u, 0, p), ai = IM(u, 0, p), br = RE(u, 1, q), bi = IM(u, 1, q);
+        double cr = RE(u, 0, q), ci = IM(u, 0, q), dr = RE(u, 1, p), di = IM(u, 1, p);
+        double re = ar * br;
+        re = fma(-ai, bi, re);
+        re = fma(-cr, dr, re);
+        re = fma(ci, di, re);
+        double im = ar * bi;
+        im = fma(ai, br, im);
+        im = fma(-cr, di, im);
+        im = fma(-ci, dr, im);
+        RE(u, 2, k) = re;
+        IM(u, 2, k) = -im;
+    }
+}
+#endif
+
+/* ------------------------------------------------ SU(3) Metropolis sweep */
+static void F(su3_update_link)(REAL *L, const geom *g, const int64_t x[4], int mu, const ymd_su3_params *P,
+                               uint64_t *acc) {
+    REAL A[18], Uf[18];
+    F(su3_staple)(L, g, x, mu, A);
+    F(load_link)(L, g, x, mu, 18, Uf);
+    double u[18], a[18], M[18];
+    for (int c = 0; c < 18; ++c) { u[c] = (double)Uf[c]; a[c] = (double)A[c]; }
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+        double re = RE(u, i, 0) * RE(a, 0, j);
+        re = fma(-IM(u, i, 0), IM(a, 0, j), re);
+        re = fma(RE(u, i, 1), RE(a, 1, j), re);
+        re = fma(-IM(u, i, 1), IM(a, 1, j), re);
+        re = fma(RE(u, i, 2), RE(a, 2, j), re);
+        re = fma(-IM(u, i, 2), IM(a, 2, j), re);
+        double im = RE(u, i, 0) * IM(a, 0, j);
+        im = fma(IM(u, i, 0), RE(a, 0, j), im);
+        im = fma(RE(u, i, 1), IM(a, 1, j), im);
+        im = fma(IM(u, i, 1), RE(a, 1, j), im);
+        im = fma(RE(u, i, 2), IM(a, 2, j), im);
+        im = fma(IM(u, i, 2), RE(a, 2, j), im);
+        RE(M, i, j) = re;
+        IM(M, i, j) = im;
+    }
+    double nbt = -(P->beta / 3.0);
+    uint32_t site = lex_of(g, x);
+    static const int blk[8][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 2}, {0, 2}, {1, 2}, {1, 2}, {0, 1}};
+    for (int h = 0; h < P->hits; ++h) {
+        uint32_t o[4];
+        philox_draw(P->key, site, word1(mu, KIND_SU3_HIT, (uint32_t)h), P->sweep, o);
+        double theta;
+        ymo_proposal_angle_from_u64(raw64(o[0], o[1]), P->angle, &theta);
+        double uacc = unit53(o[2], o[3]);
+        int gen = h & 7;
+        double s, cm1, s2 = 0.0, cm2 = 0.0, dre;
+        if (gen == 7) {
+            double psi = theta * 0x1.279a74590331cp-2;
+            ymd_sincos_small(psi, &s, &cm1);
+            ymd_sincos_small(-2.0 * psi, &s2, &cm2);
+            double t1 = RE(M, 0, 0) + RE(M, 1, 1), t2 = IM(M, 0, 0) + IM(M, 1, 1);
+            double d1 = fma(cm1, t1, -(s * t2));
+            double d2 = fma(cm2, RE(M, 2, 2), -(s2 * IM(M, 2, 2)));
+            dre = d1 + d2;
+        } else {
+            ymd_sincos_small(theta * 0.5, &s, &cm1);
+            if (gen == 2) {
+                double t1 = RE(M, 0, 0) + RE(M, 1, 1), t2 = IM(M, 0, 0) - IM(M, 1, 1);
+                dre = fma(cm1, t1, -(s * t2));
+            } else {
+                int i = blk[gen][0], j = blk[gen][1];
+                double t1 = RE(M, i, i) + RE(M, j, j);
+                if (gen == 0 || gen == 3 || gen == 5) {
+                    double t2 = IM(M, i, j) + IM(M, j, i);
+                    dre = fma(cm1, t1, -(s * t2));
+                } else {
+                    double t2 = RE(M, j, i) - RE(M, i, j);
+                    dre = fma(cm1, t1, s * t2);
+                }
+            }
+        }
+        double dS = nbt * dre;
+        int accept = (dS <= 0.0) || (uacc < ymd_exp_neg(dS));
+        if (!accept) continue;
+        *acc += 1;
+        double c = 1.0 + cm1;
+        if (gen == 7) {
+            double c2 = 1.0 + cm2;
+            rot_phase(u, 0, c, s); rot_phase(u, 1, c, s); rot_phase(u, 2, c2, s2);
+            rot_phase(M, 0, c, s); rot_phase(M, 1, c, s); rot_phase(M, 2, c2, s2);
+        } else if (gen == 2) {
+            rot_phase(u, 0, c, s); rot_phase(u, 1, c, -s);
+            rot_phase(M, 0, c, s); rot_phase(M, 1, c, -s);
+        } else {
+            int i = blk[gen][0], j = blk[gen][1];
+            if (gen == 0 || gen == 3 || gen == 5) { rot_sym(u, i, j, c, s); rot_sym(M, i, j, c, s); }
+            else { rot_asym(u, i, j, c, s); rot_asym(M, i, j, c, s); }
+        }
+    }
+    if (P->reunitarize) reunitarize3(u);
+    for (int c = 0; c < 18; ++c) Uf[c] = (REAL)u[c];
+    F(store_link)(L, g, x, mu, 18, Uf);
+}
+
+static int F(su3_sweep)(const int64_t n[4], void *dev, const ymd_su3_params *P, uint64_t *proposals, uint64_t *accepts) {
+    geom g = make_geom(n);
+    if (g.n[0] % 2 || g.n[1] % 2 || g.n[2] % 2 || g.n[3] % 2) return YMO_UNSUPPORTED;
+    REAL *L = (REAL *)dev;
+    uint64_t acc = 0, prop = 0;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int p = 0; p < 2; ++p)
+            for (int64_t s = 0; s < g.vh; ++s) {
+                int64_t x[4];
+                if (!site_of(&g, p, s, x)) continue;
+                F(su3_update_link)(L, &g, x, mu, P, &acc);
+                prop += (uint64_t)P->hits;
+            }
+    *proposals = prop;
+    *accepts = acc;
+    return YMO_OK;
+}
+
+/* ---------------------------------------------------------- hot starts */
+/* Box-Muller from two 53-bit uniforms: r = sqrt(-2 log(1-u1)), angle 2 pi u2 */
+static inline void F(gauss2)(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1, double *g1, double *g2) {
+    double u1 = 1.0 - unit53(a0, a1);
+    double u2 = unit53(b0, b1);
+    double r = sqrt(-2.0 * ymd_log(u1));
+    double s, c;
+    ymd_sincos_turns(u2, &s, &c);
+    *g1 = r * c;
+    *g2 = r * s;
+}
+static int F(su3_hot)(const int64_t n[4], void *dev, uint64_t key) {
+    geom g = make_geom(n);
+    REAL *L = (REAL *)dev;
+    int64_t x[4];
+    for (x[0] = 0; x[0] < n[0]; ++x[0]) for (x[1] = 0; x[1] < n[1]; ++x[1]) for (x[2] = 0; x[2] < n[2]; ++x[2]) for (x[3] = 0; x[3] < n[3]; ++x[3]) {
+        uint32_t site = lex_of(&g, x);
+        for (int mu = 0; mu < 4; ++mu) {
+            double gs[12];
+            for (int k = 0; k < 6; ++k) {
+                uint32_t o[4];
+                philox_draw(key, site, word1(mu, KIND_HOT_START, (uint32_t)k), 0, o);
+                F(gauss2)(o[0], o[1], o[2], o[3], &gs[2 * k], &gs[2 * k + 1]);
+            }
+            double u[18];
+            for (int c = 0; c < 12; ++c) u[c] = gs[c];
+            for (int c = 12; c < 18; ++c) u[c] = 0.0;
+            reunitarize3(u);
+            REAL v[18];
+            for (int c = 0; c < 18; ++c) v[c] = (REAL)u[c];
+            F(store_link)(L, &g, x, mu, 18, v);
+        }
+    }
+    return YMO_OK;
+}
+
+/* ---------------------------------------------------------- SU(2) */
+/* quaternion product with the reference convention (lib.rs:212-224) as an
+ * explicit FMA chain */
+static inline void F(qmul)(const REAL *a, const REAL *b, REAL *r) {
+    REAL r0 = a[0] * b[0];
+    r0 = RFMA(-a[1], b[1], r0); r0 = RFMA(-a[2], b[2], r0); r0 = RFMA(-a[3], b[3], r0);
+    REAL r1 = a[0] * b[1];
+    r1 = RFMA(a[1], b[0], r1); r1 = RFMA(a[2], b[3], r1); r1 = RFMA(-a[3], b[2], r1);
+    REAL r2 = a[0] * b[2];
+    r2 = RFMA(-a[1], b[3], r2); r2 = RFMA(a[2], b[0], r2); r2 = RFMA(a[3], b[1], r2);
+    REAL r3 = a[0] * b[3];
+    r3 = RFMA(a[1], b[2], r3); r3 = RFMA(-a[2], b[1], r3); r3 = RFMA(a[3], b[0], r3);
+    r[0] = r0; r[1] = r1; r[2] = r2; r[3] = r3;
+}
+static inline void F(qdag)(const REAL *a, REAL *r) { r[0] = a[0]; r[1] = -a[1]; r[2] = -a[2]; r[3] = -a[3]; }
+
+static void F(su2_staple)(const REAL *L, const geom *g, const int64_t x[4], int mu, REAL *A) {
+    for (int c = 0; c < 4; ++c) A[c] = 0;
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        int64_t xpm[4], xpn[4], xmn[4], xpmmn[4];
+        shift(g, x, mu, +1, xpm);
+        shift(g, x, nu, +1, xpn);
+        shift(g, x, nu, -1, xmn);
+        shift(g, xpm, nu, -1, xpmmn);
+        REAL X[4], Y[4], Z[4], Yd[4], Zd[4], Xd[4], T[4], R[4];
+        F(load_link)(L, g, xpm, nu, 4, X);
+        F(load_link)(L, g, xpn, mu, 4, Y);
+        F(load_link)(L, g, x, nu, 4, Z);
+        F(qdag)(Y, Yd); F(qdag)(Z, Zd);
+        F(qmul)(X, Yd, T);
+        F(qmul)(T, Zd, R);
+        for (int c = 0; c < 4; ++c) A[c] = A[c] + R[c];
+        F(load_link)(L, g, xpmmn, nu, 4, X);
+        F(load_link)(L, g, xmn, mu, 4, Y);
+        F(load_link)(L, g, xmn, nu, 4, Z);
+        F(qdag)(X, Xd); F(qdag)(Y, Yd);
+        F(qmul)(Xd, Yd, T);
+        F(qmul)(T, Z, R);
+        for (int c = 0; c < 4; ++c) A[c] = A[c] + R[c];
+    }
+}
+
+#ifndef YM_QNORM_DEFINED
+#define YM_QNORM_DEFINED
+static inline double qnorm2(const double *a) {
+    double n = a[0] * a[0];
+    n = fma(a[1], a[1], n); n = fma(a[2], a[2], n); n = fma(a[3], a[3], n);
+    return n;
+}
+static inline void qmul64(const double *a, const double *b, double *r) {
+    double r0 = a[0] * b[0];
+    r0 = fma(-a[1], b[1], r0); r0 = fma(-a[2], b[2], r0); r0 = fma(-a[3], b[3], r0);
+    double r1 = a[0] * b[1];
+    r1 = fma(a[1], b[0], r1); r1 = fma(a[2], b[3], r1); r1 = fma(-a[3], b[2], r1);
+    double r2 = a[0] * b[2];
+    r2 = fma(-a[1], b[3], r2); r2 = fma(a[2], b[0], r2); r2 = fma(a[3], b[1], r2);
+    double r3 = a[0] * b[3];
+    r3 = fma(a[1], b[2], r3); r3 = fma(-a[2], b[1], r3); r3 = fma(a[3], b[0], r3);
+    r[0] = r0; r[1] = r1; r[2] = r2; r[3] = r3;
+}
+static inline void qnormalize(double *a) {
+    double inv = 1.0 / sqrt(qnorm2(a));
+    for (int c = 0; c < 4; ++c) a[c] *= inv;
+}
+#endif
+
+static int F(su2_hb)(const int64_t n[4], void *dev, double beta, uint64_t key, uint64_t sweep, int64_t max_attempts,
+                     uint64_t *proposals, uint64_t *updates) {
+    geom g = make_geom(n);
+    if (g.n[0] % 2 || g.n[1] % 2 || g.n[2] % 2 || g.n[3] % 2) return YMO_UNSUPPORTED;
+    REAL *L = (REAL *)dev;
+    uint64_t prop = 0, upd = 0;
+    int status = YMO_OK;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int p = 0; p < 2; ++p)
+            for (int64_t s = 0; s < g.vh; ++s) {
+                int64_t x[4];
+                if (!site_of(&g, p, s, x)) continue;
+                uint32_t site = lex_of(&g, x);
+                REAL A[4];
+                F(su2_staple)(L, &g, x, mu, A);
+                double a[4] = {(double)A[0], (double)A[1], (double)A[2], (double)A[3]};
+                double norm = sqrt(qnorm2(a));
+                double alpha = beta * norm;
+                double r0 = 0.0, radial = 0.0;
+                int64_t att = 0;
+                int ok = 0;
+                while (att < max_attempts) {
+                    uint32_t o[4];
+                    philox_draw(key, site, word1(mu, KIND_HB_ATTEMPT, (uint32_t)att), sweep, o);
+                    att += 1;
+                    double u1 = unit53(o[0], o[1]), u2 = unit53(o[2], o[3]);
+                    if (alpha <= 1.0e-300) {
+                        r0 = 2.0 * u1 - 1.0;
+                    } else {
+                        double tm = -ymd_expm1(-2.0 * alpha);
+                        double y = -ymd_log1p(-u1 * tm) / alpha;
+                        r0 = 1.0 - y;
+                        if (r0 < -1.0) r0 = -1.0;
+                        if (r0 > 1.0) r0 = 1.0;
+                    }
+                    radial = sqrt(fmax(fma(-r0, r0, 1.0), 0.0));
+                    if (u2 <= radial) { ok = 1; break; }
+                }
+                prop += (uint64_t)att;
+                if (!ok) { status = YMO_HEATBATH_ATTEMPT_LIMIT; continue; }
+                upd += 1;
+                uint32_t o[4];
+                philox_draw(key, site, word1(mu, KIND_HB_DIRECTION, 0), sweep, o);
+                double z = 2.0 * unit53(o[0], o[1]) - 1.0;
+                double sn, cs;
+                ymd_sincos_turns(unit53(o[2], o[3]), &sn, &cs);
+                double planar = sqrt(fmax(fma(-z, z, 1.0), 0.0));
+                double rp = radial * planar;
+                double R[4] = {r0, rp * cs, rp * sn, radial * z};
+                double un[4];
+                if (norm == 0.0) {
+                    memcpy(un, R, sizeof un);
+                } else {
+                    double ad[4] = {a[0] / norm, -(a[1] / norm), -(a[2] / norm), -(a[3] / norm)};
+                    qmul64(R, ad, un);
+                }
+                qnormalize(un);
+                REAL v[4] = {(REAL)un[0], (REAL)un[1], (REAL)un[2], (REAL)un[3]};
+                F(store_link)(L, &g, x, mu, 4, v);
+            }
+    *proposals = prop;
+    *updates = upd;
+    return status;
+}
+
+static int F(su2_or)(const int64_t n[4], void *dev) {
+    geom g = make_geom(n);
+    if (g.n[0] % 2 || g.n[1] % 2 || g.n[2] % 2 || g.n[3] % 2) return YMO_UNSUPPORTED;
+    REAL *L = (REAL *)dev;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int p = 0; p < 2; ++p)
+            for (int64_t s = 0; s < g.vh; ++s) {
+                int64_t x[4];
+                if (!site_of(&g, p, s, x)) continue;
+                REAL A[4], Uf[4];
+                F(su2_staple)(L, &g, x, mu, A);
+                double a[4] = {(double)A[0], (double)A[1], (double)A[2], (double)A[3]};
+                double norm = sqrt(qnorm2(a));
+                if (norm == 0.0) continue;
+                double ad[4] = {a[0] / norm, -(a[1] / norm), -(a[2] / norm), -(a[3] / norm)};
+                F(load_link)(L, &g, x, mu, 4, Uf);
+                double ud[4] = {(double)Uf[0], -(double)Uf[1], -(double)Uf[2], -(double)Uf[3]};
+                double t[4], r[4];
+                qmul64(ad, ud, t);
+                qmul64(t, ad, r);
+                qnormalize(r);
+                REAL v[4] = {(REAL)r[0], (REAL)r[1], (REAL)r[2], (REAL)r[3]};
+                F(store_link)(L, &g, x, mu, 4, v);
+            }
+    return YMO_OK;
+}
+
+static int F(su2_hot)(const int64_t n[4], void *dev, uint64_t key) {
+    geom g = make_geom(n);
+    REAL *L = (REAL *)dev;
+    int64_t x[4];
+    for (x[0] = 0; x[0] < n[0]; ++x[0]) for (x[1] = 0; x[1] < n[1]; ++x[1]) for (x[2] = 0; x[2] < n[2]; ++x[2]) for (x[3] = 0; x[3] < n[3]; ++x[3]) {
+        uint32_t site = lex_of(&g, x);
+        for (int mu = 0; mu < 4; ++mu) {
+            double q[4];
+            for (int k = 0; k < 2; ++k) {
+                uint32_t o[4];
+                philox_draw(key, site, word1(mu, KIND_HOT_START, (uint32_t)k), 0, o);
+                F(gauss2)(o[0], o[1], o[2], o[3], &q[2 * k], &q[2 * k + 1]);
+            }
+            qnormalize(q);
+            REAL v[4] = {(REAL)q[0], (REAL)q[1], (REAL)q[2], (REAL)q[3]};
+            F(store_link)(L, &g, x, mu, 4, v);
+        }
+    }
+    return YMO_OK;
+}
+
+#undef RE
+#undef IM
+#undef F
+#undef YM_CAT
+#undef YM_CAT2

@@ -1,0 +1,34 @@
+import os
+import sys
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(REPO, "yang-mills_amd"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libym_amd.so")
+    config.addinivalue_line("markers", "slow: CPU test that runs for more than a few seconds")
+
+
+def gpu_available():
+    try:
+        import torch  # noqa: F401  (device counting only, no HIP init here)
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def ym():
+    import ym_amd
+    ym_amd.lib()
+    return ym_amd
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle_lib
+    return oracle_lib

@@ -1,0 +1,351 @@
+"""GPU parity tests: libym_amd.so (HIP, gfx950) against the CPU oracle.
+
+Two bars:
+  * bit-exact against the device-order replay (oracle/dev_exact.c): hot start,
+    SU(3) Metropolis sweeps (FP32 and FP64 storage) and SU(2) heat-bath /
+    over-relaxation sweeps reproduce every link bit and every accept count;
+  * FP64 storage against the reference-exact restatement (oracle/ref_*.c) on
+    identical configurations: plaquette, action, staples, Delta S, Wilson
+    loops, Polyakov loops, A1++ operators and stout smearing agree to
+    1e-13 relative (absolute where the value can vanish).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-13
+
+
+def dev_buffer(n, nc, prec):
+    vh = O.lib().ymd_slots_per_parity(O.ext(n))
+    return np.zeros(8 * nc * vh, dtype=np.float32 if prec == 32 else np.float64)
+
+
+def dev_to_aos(n, buf, prec, nc):
+    aos = np.zeros(int(np.prod(n)) * 4 * nc)
+    f = O.lib().ymd_su3_from_device if nc == 18 else O.lib().ymd_su2_from_device
+    assert f(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, O.dptr(aos)) == 0
+    return aos
+
+
+def aos_to_dev(n, aos, prec, nc):
+    buf = dev_buffer(n, nc, prec)
+    f = O.lib().ymd_su3_to_device if nc == 18 else O.lib().ymd_su2_to_device
+    assert f(O.ext(n), O.dptr(np.ascontiguousarray(aos)), buf.ctypes.data_as(C.c_void_p), prec) == 0
+    return buf
+
+
+def assert_rel(a, b, tol=REL, scale=None):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    s = scale if scale is not None else max(1.0, float(np.max(np.abs(b))))
+    err = float(np.max(np.abs(a - b))) / s
+    assert err <= tol, f"max error {err:.3e} > {tol:.1e}"
+
+
+# ------------------------------------------------------------- round trips
+@pytest.mark.parametrize("group,prec", [(3, 32), (3, 64), (2, 32), (2, 64)])
+def test_upload_download_round_trip(ym, group, prec):
+    n = (4, 3, 5, 6)
+    cls = ym.Su3Lattice4 if group == 3 else ym.Lattice4
+    lat = cls(n, prec)
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal(lat.link_count() * cls.NCOMP)
+    lat.upload(a)
+    b = lat.download()
+    expect = a if prec == 64 else a.astype(np.float32).astype(np.float64)
+    assert np.array_equal(b, expect)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_su3_hot_start_matches_replay(ym, prec):
+    n = (4, 4, 2, 6)
+    key = ym.derive_su3_chain_seed(0x1234, 5)
+    lat = ym.Su3Lattice4(n, prec)
+    lat.hot_start(key)
+    buf = dev_buffer(n, 18, prec)
+    assert O.lib().ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, key) == 0
+    assert np.array_equal(lat.download(), dev_to_aos(n, buf, prec, 18))
+    unit, det = lat.link_residuals()
+    assert unit < (1e-6 if prec == 32 else 1e-14) and det < (1e-6 if prec == 32 else 1e-14)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_su2_hot_start_matches_replay(ym, prec):
+    n = (4, 2, 4, 4)
+    lat = ym.Lattice4(n, prec)
+    lat.hot_start(99)
+    buf = dev_buffer(n, 4, prec)
+    assert O.lib().ymd_su2_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, 99) == 0
+    assert np.array_equal(lat.download(), dev_to_aos(n, buf, prec, 4))
+
+
+# ------------------------------------------------ decision-for-decision chains
+@pytest.mark.parametrize("prec,n,beta,hits", [
+    (32, (4, 4, 4, 4), 6.0, 8),
+    (32, (2, 4, 6, 4), 5.7, 10),
+    (64, (4, 4, 4, 4), 6.0, 8),
+    (64, (4, 2, 2, 8), 0.0, 3),
+])
+def test_su3_metropolis_matches_replay_bitwise(ym, prec, n, beta, hits):
+    key = ym.derive_su3_chain_seed(0x2c02198105010002, 1)
+    angle = 0.35
+    lat = ym.Su3Lattice4(n, prec)
+    lat.hot_start(key ^ 0x55)
+    buf = dev_buffer(n, 18, prec)
+    assert O.lib().ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, key ^ 0x55) == 0
+    params = O.DevSu3Params(beta=beta, angle=angle, hits=hits, reunitarize=1, key=key, sweep=0)
+    for sweep in range(3):
+        gp, ga = lat.metropolis_sweeps(beta, angle, key, sweep, 1, hits)
+        params.sweep = sweep
+        cp, ca = C.c_uint64(), C.c_uint64()
+        st = O.lib().ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, C.byref(params),
+                                             C.byref(cp), C.byref(ca))
+        assert st == 0
+        assert gp == cp.value and ga == ca.value, f"sweep {sweep}: gpu {ga}/{gp} replay {ca.value}/{cp.value}"
+        assert np.array_equal(lat.download(), dev_to_aos(n, buf, prec, 18)), f"links differ after sweep {sweep}"
+    if beta == 0.0:
+        assert ga == gp  # every proposal accepted at beta = 0
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_su2_heatbath_overrelax_match_replay_bitwise(ym, prec):
+    n = (4, 4, 2, 6)
+    key = ym.derive_chain_seed(0x2b00201211160101, 0)
+    lat = ym.Lattice4(n, prec)
+    buf = dev_buffer(n, 4, prec)
+    aos = O.su2_identity(n)
+    buf = aos_to_dev(n, aos, prec, 4)
+    for sweep in range(3):
+        gp, gu = lat.heatbath_sweeps(2.3, key, sweep, 1, 10_000)
+        cp, cu = C.c_uint64(), C.c_uint64()
+        st = O.lib().ymd_su2_heatbath_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, 2.3, key, sweep,
+                                           10_000, C.byref(cp), C.byref(cu))
+        assert st == 0
+        assert (gp, gu) == (cp.value, cu.value)
+        assert np.array_equal(lat.download(), dev_to_aos(n, buf, prec, 4))
+        lat.overrelax_sweeps(1)
+        assert O.lib().ymd_su2_overrelax_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec) == 0
+        assert np.array_equal(lat.download(), dev_to_aos(n, buf, prec, 4))
+
+
+def test_su2_heatbath_attempt_limit_reported(ym):
+    # A cap of one attempt at large alpha rejects most first attempts: the
+    # sweep must return HeatbathAttemptLimitExceeded (status 42), not hang.
+    lat = ym.Lattice4((4, 4, 4, 4), 32)
+    with pytest.raises(ym.YmError) as e:
+        lat.heatbath_sweeps(8.0, 1, 0, 1, 1)
+    assert e.value.status == 42
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_su3_staple_matches_replay_bitwise(ym, prec):
+    n = (4, 4, 4, 6)
+    lat = ym.Su3Lattice4(n, prec)
+    lat.hot_start(3)
+    buf = dev_buffer(n, 18, prec)
+    O.lib().ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, 3)
+    for x, mu in [((0, 0, 0, 0), 0), ((1, 2, 3, 5), 3), ((3, 3, 0, 1), 2)]:
+        g = lat.staple(x, mu)
+        c = np.zeros(18)
+        xs = (C.c_int64 * 4)(*x)
+        assert O.lib().ymd_su3_staple(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, xs, mu, O.dptr(c)) == 0
+        assert np.array_equal(g, c)
+
+
+def test_determinism_same_key_same_links(ym):
+    n = (4, 4, 4, 4)
+    a, b = ym.Su3Lattice4(n, 32), ym.Su3Lattice4(n, 32)
+    for lat in (a, b):
+        lat.hot_start(11)
+        lat.metropolis_sweeps(6.0, 0.35, 77, 0, 2, 8)
+    assert np.array_equal(a.download(), b.download())
+
+
+# -------------------------------------------- FP64 vs the reference restatement
+def hot_fp64(ym, n, seed, group=3):
+    cls = ym.Su3Lattice4 if group == 3 else ym.Lattice4
+    lat = cls(n, 64)
+    lat.hot_start(seed)
+    return lat, lat.download()
+
+
+def test_su3_plaquette_and_action_fp64(ym):
+    n = (4, 4, 4, 6)
+    lat, aos = hot_fp64(ym, n, 21)
+    lat.metropolis_sweeps(6.0, 0.35, 5, 0, 2, 8, True)
+    aos = lat.download()
+    ref = C.c_double()
+    assert O.lib().ymo_su3_average_plaquette(O.ext(n), O.dptr(aos), C.byref(ref)) == 0
+    assert_rel(lat.average_plaquette(), ref.value)
+    assert O.lib().ymo_su3_wilson_action(O.ext(n), O.dptr(aos), 6.0, C.byref(ref)) == 0
+    assert_rel(lat.wilson_action(6.0), ref.value, scale=abs(ref.value))
+
+
+def test_su2_plaquette_and_staple_fp64(ym):
+    n = (4, 4, 2, 6)
+    lat, aos = hot_fp64(ym, n, 4, group=2)
+    ref = C.c_double()
+    assert O.lib().ymo_su2_average_plaquette(O.ext(n), O.dptr(aos), C.byref(ref)) == 0
+    assert_rel(lat.average_plaquette(), ref.value)
+    for x, mu in [((0, 0, 0, 0), 0), ((3, 1, 1, 5), 2)]:
+        out = np.zeros(4)
+        assert O.lib().ymo_su2_staple(O.ext(n), O.dptr(aos), (C.c_int64 * 4)(*x), mu, O.dptr(out)) == 0
+        assert_rel(lat.staple(x, mu), out)
+
+
+def test_su3_staple_reproduces_reference_local_action_and_delta(ym):
+    """Re Tr(U A)/3 summed with the six plaquettes (su3_ensemble.rs:261-334)."""
+    n = (4, 4, 4, 4)
+    lat, aos = hot_fp64(ym, n, 8)
+    beta = 5.7
+    for x, mu in [((0, 0, 0, 0), 2), ((1, 3, 2, 0), 0), ((3, 3, 3, 3), 3)]:
+        A = O.mat_from18(lat.staple(x, mu))
+        off = O.link_offset(n, x, mu, 18)
+        U = O.mat_from18(aos[off:off + 18])
+        local = beta * (6.0 - np.trace(U @ A).real / 3.0)
+        ref = C.c_double()
+        assert O.lib().ymo_su3_local_link_action(O.ext(n), O.dptr(aos), (C.c_int64 * 4)(*x), mu, beta, C.byref(ref)) == 0
+        assert_rel(local, ref.value, scale=abs(ref.value))
+        coef = np.zeros(8)
+        coef[4] = 0.21
+        R = O.mat_from18(O.su3_from_algebra(coef))
+        cand = R @ U
+        dS = -(beta / 3.0) * np.trace((cand - U) @ A).real
+        assert O.lib().ymo_su3_local_action_difference(O.ext(n), O.dptr(aos), (C.c_int64 * 4)(*x), mu,
+                                                       O.dptr(O.mat_to18(cand)), beta, C.byref(ref)) == 0
+        assert abs(dS - ref.value) <= 1e-12 * max(1.0, abs(ref.value))
+
+
+@pytest.mark.parametrize("group", [3, 2])
+def test_wilson_loops_fp64_vs_reference(ym, group):
+    n = (4, 4, 4, 4)
+    lat, aos = hot_fp64(ym, n, 31, group)
+    rmax, tmax = 3, 2
+    for plane_set, planes in [(0, [(m, k) for m in range(4) for k in range(4) if m != k]),
+                              (1, [(0, 3), (1, 3), (2, 3)])]:
+        re, im = lat.wilson_loops(rmax, tmax, plane_set)
+        for r in range(1, rmax + 1):
+            for t in range(1, tmax + 1):
+                acc_re = acc_im = 0.0
+                for (m, k) in planes:
+                    for site in np.ndindex(*n):
+                        s = (C.c_int64 * 4)(*site)
+                        if group == 3:
+                            a, b = C.c_double(), C.c_double()
+                            assert O.lib().ymo_su3_rect_wilson_loop(O.ext(n), O.dptr(aos), s, m, k, r, t, C.byref(a), C.byref(b)) == 0
+                            acc_re += a.value
+                            acc_im += b.value
+                        else:
+                            a = C.c_double()
+                            assert O.lib().ymo_su2_rect_wilson_loop(O.ext(n), O.dptr(aos), s, m, k, r, t, C.byref(a)) == 0
+                            acc_re += a.value
+                cnt = len(planes) * int(np.prod(n))
+                assert abs(re[r - 1, t - 1] - acc_re / cnt) <= REL
+                assert abs(im[r - 1, t - 1] - acc_im / cnt) <= REL
+
+
+@pytest.mark.parametrize("group", [3, 2])
+def test_polyakov_fp64_vs_reference(ym, group):
+    n = (4, 2, 4, 6)
+    lat, aos = hot_fp64(ym, n, 12, group)
+    for d in range(4):
+        val, ab = lat.polyakov_loop(d)
+        acc = 0j
+        accabs = 0.0
+        cnt = 0
+        for site in np.ndindex(*n):
+            if site[d] != 0:
+                continue
+            s = (C.c_int64 * 4)(*site)
+            if group == 3:
+                a, b = C.c_double(), C.c_double()
+                assert O.lib().ymo_su3_polyakov_loop(O.ext(n), O.dptr(aos), s, d, C.byref(a), C.byref(b)) == 0
+                z = complex(a.value, b.value)
+            else:
+                a = C.c_double()
+                assert O.lib().ymo_su2_polyakov_loop(O.ext(n), O.dptr(aos), s, d, C.byref(a)) == 0
+                z = complex(a.value, 0.0)
+            acc += z
+            accabs += abs(z)
+            cnt += 1
+        assert abs(val - acc / cnt) <= REL
+        assert abs(ab - accabs / cnt) <= REL
+
+
+def test_a1pp_fp64_vs_reference(ym):
+    n = (4, 4, 4, 5)
+    lat, aos = hot_fp64(ym, n, 17)
+    pl, rc = lat.timeslice_a1pp()
+    for t in range(n[3]):
+        p, r = C.c_double(), C.c_double()
+        pc, rc_ = C.c_int64(), C.c_int64()
+        assert O.lib().ymo_su3_a1pp_basis(O.ext(n), O.dptr(aos), t, C.byref(p), C.byref(r), C.byref(pc), C.byref(rc_)) == 0
+        assert abs(pl[t] - p.value) <= REL and abs(rc[t] - r.value) <= REL
+
+
+def test_stout_frozen_cycle_fixture(ym):
+    """glueball-spatial-stout-phase3b1-v1.json: one level at rho = 0.1."""
+    n = (3, 3, 3, 3)
+    aos = O.su3_identity(n)
+    cyc = np.zeros(18)
+    O.lib().ymo_su3_traceless_cycle_link(O.dptr(cyc))
+    aos[0:18] = cyc
+    src = ym.Su3Lattice4(n, 64)
+    src.upload(aos)
+    out = ym.spatial_stout_smear(src, 0.1, 1)
+    sm = out.download()
+    d = (sm - aos).reshape(-1, 9, 2)
+    change = float(np.max(np.hypot(d[..., 0], d[..., 1])))  # max_link_residual, glueball_smearing.rs:270
+    assert abs(change - 0.2158247298307305) <= 1e-12
+    pl, rc = out.timeslice_a1pp()
+    assert abs(pl[0] - 0.9693015447000209) <= 1e-12
+    assert abs(rc[0] - 0.9511047606560571) <= 1e-12
+    # temporal links unchanged, bit for bit
+    assert np.array_equal(sm.reshape(-1, 4, 18)[:, 3], aos.reshape(-1, 4, 18)[:, 3])
+
+
+@pytest.mark.parametrize("levels", [1, 3])
+def test_stout_fp64_vs_reference_hot(ym, levels):
+    n = (4, 4, 4, 4)
+    lat, aos = hot_fp64(ym, n, 23)
+    lat.metropolis_sweeps(6.0, 0.35, 3, 0, 2, 8, True)
+    aos = lat.download()
+    out = ym.spatial_stout_smear(lat, 0.1, levels).download()
+    ref = np.zeros_like(aos)
+    assert O.lib().ymo_su3_stout_smear(O.ext(n), O.dptr(aos), O.dptr(ref), 0.1, levels, 0) == 0
+    assert float(np.max(np.abs(out - ref))) <= 1e-13
+
+
+def test_residuals_fp64_vs_reference(ym):
+    n = (4, 4, 2, 4)
+    lat, aos = hot_fp64(ym, n, 2)
+    u, d = lat.link_residuals()
+    ru, rd = C.c_double(), C.c_double()
+    O.lib().ymo_su3_max_residuals(O.ext(n), O.dptr(aos), C.byref(ru), C.byref(rd))
+    assert abs(u - ru.value) <= 1e-15 and abs(d - rd.value) <= 2e-15
+    assert u <= 1e-14
+
+
+def test_fixtures_closed_forms(ym):
+    """su3-lattice-v1.json: identity action 0; single cycle link action 12 at beta 2."""
+    n = (3, 3, 3, 3)
+    lat = ym.Su3Lattice4(n, 64)
+    assert lat.wilson_action(2.0) == 0.0
+    assert lat.average_plaquette() == 1.0
+    aos = O.su3_identity(n)
+    O.lib().ymo_su3_traceless_cycle_link(O.dptr(aos[0:18]))
+    lat.upload(aos)
+    assert abs(lat.wilson_action(2.0) - 12.0) <= 1e-12
+    re, im = lat.wilson_loops(1, 1, 0)
+    assert abs(re[0, 0] - lat.average_plaquette()) <= 1e-15
+    su2 = ym.Lattice4(n, 64)
+    a2 = O.su2_identity(n)
+    a2[0:4] = [0.0, 1.0, 0.0, 0.0]
+    su2.upload(a2)
+    assert abs(su2.wilson_action(2.0) - 12.0) <= 1e-12

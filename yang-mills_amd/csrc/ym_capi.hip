@@ -1,0 +1,695 @@
+// C ABI of libym_amd.so: lattice handles, sampler and observable entry
+// points, the device chain driver and host FP64 statistics.  See
+// include/ym_amd.h for the contract and the reference function each entry
+// point replaces.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/ym_amd.h"
+#include "ym_device.hpp"
+#include "ym_internal.hpp"
+
+namespace ym {
+
+__global__ void k_fill_zero(double *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+__global__ void k_fill_zero_u64(unsigned long long *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+__global__ void k_fill_zero_i32(int *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+void launch_fill_zero(double *p, size_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+void launch_fill_zero_u64(unsigned long long *p, size_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+void launch_fill_zero_i32(int *p, size_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+
+}  // namespace ym
+
+struct ym_lattice {
+    int group = 0, prec = 0;
+    int64_t n[4] = {0, 0, 0, 0};
+    ym::Geom g{};
+    int ncomp = 0;           // reals per link
+    size_t link_bytes = 0;   // bytes of the link field
+    void *links = nullptr;
+    hipStream_t stream = nullptr;
+    double *part = nullptr;  // per-block partial results
+    size_t part_cap = 0;     // doubles
+    double *dres = nullptr;  // small result buffer (64 doubles)
+    unsigned long long *dctr = nullptr;  // 4 counters
+    int *dstatus = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+constexpr int64_t kMaxSites = (int64_t)1 << 30;
+
+int32_t hip_status(hipError_t e) { return e == hipSuccess ? YM_OK : YM_HIP_ERROR; }
+
+#define YM_HIP(call)                              \
+    do {                                          \
+        hipError_t e__ = (call);                  \
+        if (e__ != hipSuccess) return YM_HIP_ERROR; \
+    } while (0)
+
+int32_t ensure_part(ym_lattice *l, size_t n) {
+    if (n <= l->part_cap) return YM_OK;
+    if (l->part) (void)hipFree(l->part);
+    l->part = nullptr;
+    l->part_cap = 0;
+    if (hipMalloc((void **)&l->part, n * sizeof(double)) != hipSuccess) return YM_ALLOCATION_FAILED;
+    l->part_cap = n;
+    return YM_OK;
+}
+
+int32_t finish(ym_lattice *l) {
+    YM_HIP(hipGetLastError());
+    YM_HIP(hipStreamSynchronize(l->stream));
+    return YM_OK;
+}
+
+bool all_even(const ym_lattice *l) {
+    for (int d = 0; d < 4; ++d)
+        if (l->n[d] % 2) return false;
+    return true;
+}
+
+int32_t sum_partials(ym_lattice *l, int nblk, int ncols, double *host_out) {
+    ym::launch_sum_partials(l->part, nblk, ncols, nblk, l->dres, l->stream);
+    YM_HIP(hipGetLastError());
+    YM_HIP(hipMemcpyAsync(host_out, l->dres, sizeof(double) * ncols, hipMemcpyDeviceToHost, l->stream));
+    return finish(l);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ym_status_string(int32_t s) {
+    switch (s) {
+    case YM_OK: return "ok";
+    case YM_INVALID_EXTENT: return "every lattice extent must be >= 2";
+    case YM_LATTICE_TOO_LARGE: return "lattice exceeds the device limit";
+    case YM_ALLOCATION_FAILED: return "device allocation failed";
+    case YM_DIRECTION_OUT_OF_RANGE: return "lattice direction must be in 0..4";
+    case YM_DIRECTIONS_MUST_DIFFER: return "directions must differ";
+    case YM_NON_FINITE_BETA: return "beta must be finite";
+    case YM_NEGATIVE_BETA: return "beta must be non-negative";
+    case YM_NON_FINITE_ACTION: return "Wilson action evaluation produced a non-finite result";
+    case YM_ZERO_RECTANGLE_EXTENT: return "rectangular Wilson-loop extents must be positive";
+    case YM_COORDINATE_OUT_OF_RANGE: return "lattice coordinate is outside its periodic fundamental domain";
+    case YM_BETA_OUT_OF_RANGE: return "beta is outside the bounded reference range";
+    case YM_ANGLE_OUT_OF_RANGE: return "Metropolis proposal angle is outside the bounded reference range";
+    case YM_MEASUREMENT_COUNT: return "measurement count is outside the accepted range";
+    case YM_ZERO_STRIDE: return "measurement stride must be positive";
+    case YM_LAG_OUT_OF_RANGE: return "autocorrelation lag must be smaller than the measurement count";
+    case YM_SCHEDULE_OUT_OF_RANGE: return "schedule exceeds the accepted range";
+    case YM_NON_FINITE_STATISTIC: return "calculation produced a non-finite value";
+    case YM_TOO_MANY_OR: return "over-relaxation sweep count exceeds the bound";
+    case YM_ZERO_ATTEMPT_LIMIT: return "heat-bath rejection attempt limit must be positive";
+    case YM_HEATBATH_ATTEMPT_LIMIT: return "SU(2) heat-bath rejection sampler exhausted its bounded attempt limit";
+    case YM_TIME_SLICE_OUT_OF_RANGE: return "Euclidean time slice is outside the lattice";
+    case YM_SPATIAL_EXTENT_TOO_SMALL: return "every spatial extent must be at least 3";
+    case YM_SPATIAL_NOT_CUBIC: return "spatial extents must be equal";
+    case YM_NON_FINITE_RHO: return "stout rho must be finite";
+    case YM_RHO_OUT_OF_RANGE: return "stout rho must satisfy 0 < rho <= 0.25";
+    case YM_TOO_MANY_LEVELS: return "stout level count exceeds the limit";
+    case YM_INVALID_ARGUMENT: return "invalid argument";
+    case YM_NO_DEVICE: return "no HIP device";
+    case YM_HIP_ERROR: return "HIP runtime error";
+    case YM_UNSUPPORTED: return "operation not supported for this lattice";
+    default: return "unknown status";
+    }
+}
+
+uint64_t ym_derive_chain_seed(uint64_t master, uint64_t chain) {
+    uint64_t v = master ^ ((chain + 1ull) * 0x9E3779B97F4A7C15ull);
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    return v ^ (v >> 31);
+}
+uint64_t ym_derive_su3_chain_seed(uint64_t master, uint64_t chain) {
+    return ym_derive_chain_seed(master ^ 0x535533454e535631ull, chain);
+}
+
+int32_t ym_lattice_create(int32_t group, const int64_t extents[4], int32_t precision, ym_lattice **out) {
+    if (!extents || !out) return YM_INVALID_ARGUMENT;
+    *out = nullptr;
+    if ((group != 2 && group != 3) || (precision != 32 && precision != 64)) return YM_INVALID_ARGUMENT;
+    for (int d = 0; d < 4; ++d)
+        if (extents[d] < 2) return YM_INVALID_EXTENT;
+    int64_t vol = 1;
+    for (int d = 0; d < 4; ++d) {
+        if (extents[d] > kMaxSites) return YM_LATTICE_TOO_LARGE;
+        vol *= extents[d];
+        if (vol > kMaxSites) return YM_LATTICE_TOO_LARGE;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return YM_NO_DEVICE;
+    ym_lattice *l = new (std::nothrow) ym_lattice();
+    if (!l) return YM_ALLOCATION_FAILED;
+    l->group = group;
+    l->prec = precision;
+    for (int d = 0; d < 4; ++d) { l->n[d] = extents[d]; l->g.n[d] = (int)extents[d]; }
+    l->g.n3p = (int)(extents[3] + (extents[3] & 1));
+    l->g.half = l->g.n3p / 2;
+    l->g.vh = (int)(extents[0] * extents[1] * extents[2] * l->g.half);
+    l->g.volume = (int)vol;
+    l->ncomp = group == 3 ? 18 : 4;
+    l->link_bytes = (size_t)8 * l->ncomp * l->g.vh * (precision / 8);
+    int32_t st = YM_OK;
+    if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess) st = YM_HIP_ERROR;
+    if (!st && hipMalloc(&l->links, l->link_bytes) != hipSuccess) st = YM_LATTICE_TOO_LARGE;
+    if (!st && hipMalloc((void **)&l->dres, 64 * sizeof(double)) != hipSuccess) st = YM_ALLOCATION_FAILED;
+    if (!st && hipMalloc((void **)&l->dctr, 4 * sizeof(unsigned long long)) != hipSuccess) st = YM_ALLOCATION_FAILED;
+    if (!st && hipMalloc((void **)&l->dstatus, sizeof(int)) != hipSuccess) st = YM_ALLOCATION_FAILED;
+    if (!st && (hipEventCreate(&l->ev0) != hipSuccess || hipEventCreate(&l->ev1) != hipSuccess)) st = YM_HIP_ERROR;
+    if (!st) st = ym_lattice_set_identity(l);
+    if (st) {
+        ym_lattice_destroy(l);
+        return st;
+    }
+    *out = l;
+    return YM_OK;
+}
+
+int32_t ym_lattice_destroy(ym_lattice *l) {
+    if (!l) return YM_INVALID_ARGUMENT;
+    if (l->stream) (void)hipStreamSynchronize(l->stream);
+    if (l->links) (void)hipFree(l->links);
+    if (l->part) (void)hipFree(l->part);
+    if (l->dres) (void)hipFree(l->dres);
+    if (l->dctr) (void)hipFree(l->dctr);
+    if (l->dstatus) (void)hipFree(l->dstatus);
+    if (l->ev0) (void)hipEventDestroy(l->ev0);
+    if (l->ev1) (void)hipEventDestroy(l->ev1);
+    if (l->stream) (void)hipStreamDestroy(l->stream);
+    delete l;
+    return YM_OK;
+}
+
+int32_t ym_lattice_bytes(const ym_lattice *l, int64_t *bytes) {
+    if (!l || !bytes) return YM_INVALID_ARGUMENT;
+    *bytes = (int64_t)l->link_bytes;
+    return YM_OK;
+}
+
+int32_t ym_lattice_set_identity(ym_lattice *l) {
+    if (!l) return YM_INVALID_ARGUMENT;
+    // An identity AoS image is uploaded through the layout kernel.
+    const size_t count = (size_t)l->g.volume * 4 * l->ncomp;
+    std::vector<double> h(count, 0.0);
+    for (size_t k = 0; k < (size_t)l->g.volume * 4; ++k) {
+        if (l->ncomp == 18) { h[k * 18 + 0] = 1.0; h[k * 18 + 8] = 1.0; h[k * 18 + 16] = 1.0; }
+        else h[k * 4] = 1.0;
+    }
+    return ym_lattice_upload(l, h.data(), (int64_t)count);
+}
+
+int32_t ym_lattice_hot_start(ym_lattice *l, uint64_t seed) {
+    if (!l) return YM_INVALID_ARGUMENT;
+    if (l->group == 3) ym::launch_su3_hot(l->prec, l->links, l->g, seed, l->stream);
+    else ym::launch_su2_hot(l->prec, l->links, l->g, seed, l->stream);
+    return finish(l);
+}
+
+int32_t ym_lattice_upload(ym_lattice *l, const double *aos, int64_t count) {
+    if (!l || !aos) return YM_INVALID_ARGUMENT;
+    if (count != (int64_t)l->g.volume * 4 * l->ncomp) return YM_INVALID_ARGUMENT;
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(double) * count) != hipSuccess) return YM_ALLOCATION_FAILED;
+    int32_t st = hip_status(hipMemcpyAsync(d, aos, sizeof(double) * count, hipMemcpyHostToDevice, l->stream));
+    if (!st) {
+        if (l->group == 3) ym::launch_su3_from_aos(l->prec, l->links, l->g, d, l->stream);
+        else ym::launch_su2_from_aos(l->prec, l->links, l->g, d, l->stream);
+        st = finish(l);
+    }
+    (void)hipFree(d);
+    return st;
+}
+
+int32_t ym_lattice_download(const ym_lattice *lc, double *aos, int64_t count) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !aos) return YM_INVALID_ARGUMENT;
+    if (count != (int64_t)l->g.volume * 4 * l->ncomp) return YM_INVALID_ARGUMENT;
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(double) * count) != hipSuccess) return YM_ALLOCATION_FAILED;
+    if (l->group == 3) ym::launch_su3_to_aos(l->prec, l->links, l->g, d, l->stream);
+    else ym::launch_su2_to_aos(l->prec, l->links, l->g, d, l->stream);
+    int32_t st = hip_status(hipGetLastError());
+    if (!st) st = hip_status(hipMemcpyAsync(aos, d, sizeof(double) * count, hipMemcpyDeviceToHost, l->stream));
+    if (!st) st = finish(l);
+    (void)hipFree(d);
+    return st;
+}
+
+int32_t ym_lattice_copy(ym_lattice *dst, const ym_lattice *src) {
+    if (!dst || !src) return YM_INVALID_ARGUMENT;
+    if (dst->group != src->group || dst->prec != src->prec) return YM_INVALID_ARGUMENT;
+    for (int d = 0; d < 4; ++d)
+        if (dst->n[d] != src->n[d]) return YM_INVALID_ARGUMENT;
+    YM_HIP(hipStreamSynchronize(src->stream));
+    YM_HIP(hipMemcpyAsync(dst->links, src->links, src->link_bytes, hipMemcpyDeviceToDevice, dst->stream));
+    return finish(dst);
+}
+
+int32_t ym_su3_metropolis_sweeps(ym_lattice *l, double beta, double angle, int32_t hits, uint64_t key,
+                                 uint64_t first_sweep, int32_t n_sweeps, int32_t reunitarize, uint64_t *proposals,
+                                 uint64_t *accepts) {
+    if (!l || l->group != 3 || hits < 1 || n_sweeps < 0) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(beta) || beta < 0.0 || beta > 12.0) return YM_BETA_OUT_OF_RANGE;
+    if (!std::isfinite(angle) || angle < 1.0e-8 || angle > 1.0) return YM_ANGLE_OUT_OF_RANGE;
+    if (!all_even(l)) return YM_UNSUPPORTED;
+    const int reunit = (l->prec == 32) ? 1 : (reunitarize != 0);
+    ym::launch_fill_zero_u64(l->dctr, 1, l->stream);
+    for (int k = 0; k < n_sweeps; ++k)
+        for (int mu = 0; mu < 4; ++mu)
+            for (int p = 0; p < 2; ++p)
+                ym::launch_su3_metropolis(l->prec, l->links, l->g, mu, p, beta, angle, hits, key,
+                                          first_sweep + (uint64_t)k, reunit, l->dctr, l->stream);
+    YM_HIP(hipGetLastError());
+    unsigned long long acc = 0;
+    YM_HIP(hipMemcpyAsync(&acc, l->dctr, sizeof(acc), hipMemcpyDeviceToHost, l->stream));
+    int32_t st = finish(l);
+    if (st) return st;
+    if (proposals) *proposals += (uint64_t)n_sweeps * (uint64_t)hits * 4ull * (uint64_t)l->g.volume;
+    if (accepts) *accepts += acc;
+    return YM_OK;
+}
+
+int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *l, double beta, double angle, int32_t hits, uint64_t key,
+                                          uint64_t first_sweep, int32_t n_sweeps, int32_t reunitarize,
+                                          uint64_t *proposals, uint64_t *accepts, float *total_ms,
+                                          float *kernel_ms_sum, int32_t *launches) {
+    if (!l || l->group != 3 || hits < 1 || n_sweeps < 1 || !total_ms || !kernel_ms_sum || !launches)
+        return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(beta) || beta < 0.0 || beta > 12.0) return YM_BETA_OUT_OF_RANGE;
+    if (!std::isfinite(angle) || angle < 1.0e-8 || angle > 1.0) return YM_ANGLE_OUT_OF_RANGE;
+    if (!all_even(l)) return YM_UNSUPPORTED;
+    const int reunit = (l->prec == 32) ? 1 : (reunitarize != 0);
+    const int nl = 8 * n_sweeps;
+    std::vector<hipEvent_t> ev(2 * (size_t)nl, nullptr);
+    int32_t st = YM_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) st = YM_HIP_ERROR;
+    if (!st) {
+        ym::launch_fill_zero_u64(l->dctr, 1, l->stream);
+        int k = 0;
+        for (int s = 0; s < n_sweeps; ++s)
+            for (int mu = 0; mu < 4; ++mu)
+                for (int p = 0; p < 2; ++p, ++k) {
+                    (void)hipEventRecord(ev[2 * k], l->stream);
+                    ym::launch_su3_metropolis(l->prec, l->links, l->g, mu, p, beta, angle, hits, key,
+                                              first_sweep + (uint64_t)s, reunit, l->dctr, l->stream);
+                    (void)hipEventRecord(ev[2 * k + 1], l->stream);
+                }
+        unsigned long long acc = 0;
+        st = hip_status(hipGetLastError());
+        if (!st) st = hip_status(hipMemcpyAsync(&acc, l->dctr, sizeof(acc), hipMemcpyDeviceToHost, l->stream));
+        if (!st) st = finish(l);
+        if (!st) {
+            float sum = 0.f, ms = 0.f;
+            for (int i = 0; i < nl && !st; ++i) {
+                st = hip_status(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+                sum += ms;
+            }
+            if (!st) st = hip_status(hipEventElapsedTime(total_ms, ev[0], ev[2 * nl - 1]));
+            *kernel_ms_sum = sum;
+            *launches = nl;
+            if (proposals) *proposals += (uint64_t)n_sweeps * (uint64_t)hits * 4ull * (uint64_t)l->g.volume;
+            if (accepts) *accepts += acc;
+        }
+    }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return st;
+}
+
+int32_t ym_su2_heatbath_sweeps(ym_lattice *l, double beta, uint64_t key, uint64_t first_sweep, int32_t n_sweeps,
+                               int32_t max_attempts, uint64_t *proposals, uint64_t *updates) {
+    if (!l || l->group != 2 || n_sweeps < 0) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(beta) || beta < 0.0 || beta > 8.0) return YM_BETA_OUT_OF_RANGE;
+    if (max_attempts <= 0) return YM_ZERO_ATTEMPT_LIMIT;
+    if (!all_even(l)) return YM_UNSUPPORTED;
+    ym::launch_fill_zero_u64(l->dctr, 2, l->stream);
+    ym::launch_fill_zero_i32(l->dstatus, 1, l->stream);
+    for (int k = 0; k < n_sweeps; ++k)
+        for (int mu = 0; mu < 4; ++mu)
+            for (int p = 0; p < 2; ++p)
+                ym::launch_su2_heatbath(l->prec, l->links, l->g, mu, p, beta, key, first_sweep + (uint64_t)k,
+                                        max_attempts, l->dctr, l->dstatus, l->stream);
+    YM_HIP(hipGetLastError());
+    unsigned long long ctr[2] = {0, 0};
+    int status = 0;
+    YM_HIP(hipMemcpyAsync(ctr, l->dctr, sizeof(ctr), hipMemcpyDeviceToHost, l->stream));
+    YM_HIP(hipMemcpyAsync(&status, l->dstatus, sizeof(int), hipMemcpyDeviceToHost, l->stream));
+    int32_t st = finish(l);
+    if (st) return st;
+    if (proposals) *proposals += ctr[0];
+    if (updates) *updates += ctr[1];
+    return status ? YM_HEATBATH_ATTEMPT_LIMIT : YM_OK;
+}
+
+int32_t ym_su2_overrelax_sweeps(ym_lattice *l, int32_t n_sweeps) {
+    if (!l || l->group != 2 || n_sweeps < 0) return YM_INVALID_ARGUMENT;
+    if (!all_even(l)) return YM_UNSUPPORTED;
+    for (int k = 0; k < n_sweeps; ++k)
+        for (int mu = 0; mu < 4; ++mu)
+            for (int p = 0; p < 2; ++p) ym::launch_su2_overrelax(l->prec, l->links, l->g, mu, p, l->stream);
+    return finish(l);
+}
+
+int32_t ym_average_plaquette(const ym_lattice *lc, double *out) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !out) return YM_INVALID_ARGUMENT;
+    const int nblk = 2 * ((l->g.vh + 255) / 256);
+    int32_t st = ensure_part(l, nblk);
+    if (st) return st;
+    ym::launch_plaquette(l->group, l->prec, l->links, l->g, l->part, l->stream);
+    double sum = 0.0;
+    st = sum_partials(l, nblk, 1, &sum);
+    if (st) return st;
+    const double mean = sum / (6.0 * (double)l->g.volume);
+    if (!std::isfinite(mean)) return YM_NON_FINITE_STATISTIC;
+    *out = mean;
+    return YM_OK;
+}
+
+int32_t ym_wilson_action(const ym_lattice *l, double beta, double *out) {
+    if (!l || !out) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(beta)) return YM_NON_FINITE_BETA;
+    if (beta < 0.0) return YM_NEGATIVE_BETA;
+    double p = 0.0;
+    int32_t st = ym_average_plaquette(l, &p);
+    if (st) return st;
+    const double a = beta * (6.0 * (double)l->g.volume) * (1.0 - p);
+    if (!std::isfinite(a)) return YM_NON_FINITE_ACTION;
+    *out = a;
+    return YM_OK;
+}
+
+int32_t ym_staple(const ym_lattice *lc, const int64_t x[4], int32_t mu, double *out) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !x || !out) return YM_INVALID_ARGUMENT;
+    if (mu < 0 || mu > 3) return YM_DIRECTION_OUT_OF_RANGE;
+    int xi[4];
+    for (int d = 0; d < 4; ++d) {
+        if (x[d] < 0 || x[d] >= l->n[d]) return YM_COORDINATE_OUT_OF_RANGE;
+        xi[d] = (int)x[d];
+    }
+    if (l->group == 3) ym::launch_su3_staple_probe(l->prec, l->links, l->g, xi, mu, l->dres, l->stream);
+    else ym::launch_su2_staple_probe(l->prec, l->links, l->g, xi, mu, l->dres, l->stream);
+    YM_HIP(hipGetLastError());
+    YM_HIP(hipMemcpyAsync(out, l->dres, sizeof(double) * l->ncomp, hipMemcpyDeviceToHost, l->stream));
+    return finish(l);
+}
+
+int32_t ym_wilson_loops(const ym_lattice *lc, int32_t rmax, int32_t tmax, int32_t plane_set, double *re, double *im) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !re || !im) return YM_INVALID_ARGUMENT;
+    if (rmax < 1 || tmax < 1 || rmax > 64 || tmax > 64) return YM_ZERO_RECTANGLE_EXTENT;
+    int planes[12][2], np = 0;
+    for (int m = 0; m < 4; ++m)
+        for (int n = 0; n < 4; ++n) {
+            if (m == n) continue;
+            const bool temporal = (m < 3 && n == 3), spatial = (m < 3 && n < 3);
+            if (plane_set == 0 || (plane_set == 1 && temporal) || (plane_set == 2 && spatial)) {
+                planes[np][0] = m;
+                planes[np][1] = n;
+                ++np;
+            }
+        }
+    if (plane_set < 0 || plane_set > 2) return YM_INVALID_ARGUMENT;
+    const int nblk = 2 * ((l->g.vh + 127) / 128);
+    const int ncols = rmax * tmax * 2;
+    int32_t st = ensure_part(l, (size_t)nblk * ncols);
+    if (st) return st;
+    ym::launch_fill_zero(l->part, (size_t)nblk * ncols, l->stream);
+    for (int k = 0; k < np; ++k)
+        ym::launch_wilson_loops(l->group, l->prec, l->links, l->g, planes[k][0], planes[k][1], rmax, tmax, l->part, l->stream);
+    std::vector<double> sums(ncols);
+    if (ncols > 64) {
+        double *dout = nullptr;
+        if (hipMalloc((void **)&dout, sizeof(double) * ncols) != hipSuccess) return YM_ALLOCATION_FAILED;
+        ym::launch_sum_partials(l->part, nblk, ncols, nblk, dout, l->stream);
+        st = hip_status(hipMemcpyAsync(sums.data(), dout, sizeof(double) * ncols, hipMemcpyDeviceToHost, l->stream));
+        if (!st) st = finish(l);
+        (void)hipFree(dout);
+    } else {
+        st = sum_partials(l, nblk, ncols, sums.data());
+    }
+    if (st) return st;
+    const double denom = (double)np * (double)l->g.volume;
+    for (int r = 0; r < rmax; ++r)
+        for (int t = 0; t < tmax; ++t) {
+            re[r * tmax + t] = sums[(r * tmax + t) * 2] / denom;
+            im[r * tmax + t] = sums[(r * tmax + t) * 2 + 1] / denom;
+        }
+    return YM_OK;
+}
+
+int32_t ym_polyakov_loop(const ym_lattice *lc, int32_t dir, double *re, double *im, double *abs_mean) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !re || !im || !abs_mean) return YM_INVALID_ARGUMENT;
+    if (dir < 0 || dir > 3) return YM_DIRECTION_OUT_OF_RANGE;
+    const int nbase = l->g.volume / l->g.n[dir];
+    const int nblk = (nbase + 255) / 256;
+    int32_t st = ensure_part(l, (size_t)nblk * 3);
+    if (st) return st;
+    ym::launch_polyakov(l->group, l->prec, l->links, l->g, dir, l->part, l->stream);
+    double s[3];
+    st = sum_partials(l, nblk, 3, s);
+    if (st) return st;
+    *re = s[0] / nbase;
+    *im = s[1] / nbase;
+    *abs_mean = s[2] / nbase;
+    return YM_OK;
+}
+
+int32_t ym_su3_timeslice_a1pp(const ym_lattice *lc, double *plaq_t, double *rect_t) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !plaq_t || !rect_t) return YM_INVALID_ARGUMENT;
+    if (l->group != 3) return YM_UNSUPPORTED;
+    for (int d = 0; d < 3; ++d)
+        if (l->n[d] < 3) return YM_SPATIAL_EXTENT_TOO_SMALL;
+    if (l->n[0] != l->n[1] || l->n[1] != l->n[2]) return YM_SPATIAL_NOT_CUBIC;
+    const int nsp = l->g.n[0] * l->g.n[1] * l->g.n[2];
+    const int nblk = (nsp + 255) / 256;
+    const int T = l->g.n[3];
+    int32_t st = ensure_part(l, (size_t)nblk * 2 * T);
+    if (st) return st;
+    ym::launch_a1pp(l->prec, l->links, l->g, l->part, l->stream);
+    double *dout = nullptr;
+    if (hipMalloc((void **)&dout, sizeof(double) * 2 * T) != hipSuccess) return YM_ALLOCATION_FAILED;
+    ym::launch_sum_partials(l->part, nblk, 2 * T, nblk, dout, l->stream);
+    std::vector<double> s(2 * T);
+    st = hip_status(hipMemcpyAsync(s.data(), dout, sizeof(double) * 2 * T, hipMemcpyDeviceToHost, l->stream));
+    if (!st) st = finish(l);
+    (void)hipFree(dout);
+    if (st) return st;
+    for (int t = 0; t < T; ++t) {
+        plaq_t[t] = s[2 * t] / (3.0 * nsp);
+        rect_t[t] = s[2 * t + 1] / (6.0 * nsp);
+    }
+    return YM_OK;
+}
+
+int32_t ym_su3_stout_spatial(ym_lattice *dst, const ym_lattice *src, double rho, int32_t levels) {
+    if (!dst || !src) return YM_INVALID_ARGUMENT;
+    if (dst->group != 3 || src->group != 3 || dst->prec != src->prec) return YM_INVALID_ARGUMENT;
+    for (int d = 0; d < 4; ++d)
+        if (dst->n[d] != src->n[d]) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(rho)) return YM_NON_FINITE_RHO;
+    if (rho <= 0.0 || rho > 0.25) return YM_RHO_OUT_OF_RANGE;
+    if (levels < 0 || levels > 64) return YM_TOO_MANY_LEVELS;
+    for (int d = 0; d < 3; ++d)
+        if (src->n[d] < 3) return YM_SPATIAL_EXTENT_TOO_SMALL;
+    if (src->n[0] != src->n[1] || src->n[1] != src->n[2]) return YM_SPATIAL_NOT_CUBIC;
+    if (dst == src) return YM_INVALID_ARGUMENT;
+    YM_HIP(hipStreamSynchronize(src->stream));
+    if (levels == 0) return ym_lattice_copy(dst, src);
+    void *tmp = nullptr;
+    if (levels > 1 && hipMalloc(&tmp, src->link_bytes) != hipSuccess) return YM_ALLOCATION_FAILED;
+    // Level k writes dst when (levels-1-k) is even, tmp otherwise, so the
+    // buffers alternate and the last level lands in dst.
+    const void *cur = src->links;
+    for (int k = 0; k < levels; ++k) {
+        void *target = ((levels - 1 - k) % 2 == 0) ? dst->links : tmp;
+        ym::launch_stout(src->prec, cur, target, src->g, rho, dst->stream);
+        cur = target;
+    }
+    int32_t st = finish(dst);
+    if (tmp) (void)hipFree(tmp);
+    return st;
+}
+
+int32_t ym_link_residuals(const ym_lattice *lc, double *unit, double *det) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !unit || !det) return YM_INVALID_ARGUMENT;
+    const int nblk = 8 * ((l->g.vh + 255) / 256);
+    int32_t st = ensure_part(l, (size_t)nblk * 2);
+    if (st) return st;
+    if (l->group == 3) ym::launch_su3_residuals(l->prec, l->links, l->g, l->part, l->stream);
+    else ym::launch_su2_residual(l->prec, l->links, l->g, l->part, l->stream);
+    ym::launch_max_partials(l->part, nblk, 2, 2, l->dres, l->stream);
+    double r[2];
+    YM_HIP(hipGetLastError());
+    YM_HIP(hipMemcpyAsync(r, l->dres, sizeof(r), hipMemcpyDeviceToHost, l->stream));
+    st = finish(l);
+    if (st) return st;
+    *unit = r[0];
+    *det = r[1];
+    return YM_OK;
+}
+
+int32_t ym_reunitarize(ym_lattice *l) {
+    if (!l) return YM_INVALID_ARGUMENT;
+    if (l->group == 3) {
+        ym::launch_su3_reunit(l->prec, l->links, l->g, l->stream);
+        return finish(l);
+    }
+    // SU(2): one over-relaxation-free normalisation pass via download/upload.
+    const size_t count = (size_t)l->g.volume * 16;
+    std::vector<double> h(count);
+    int32_t st = ym_lattice_download(l, h.data(), (int64_t)count);
+    if (st) return st;
+    for (size_t k = 0; k < count / 4; ++k) {
+        double *q = &h[4 * k];
+        double n = q[0] * q[0];
+        for (int c = 1; c < 4; ++c) n = std::fma(q[c], q[c], n);
+        const double inv = 1.0 / std::sqrt(n);
+        for (int c = 0; c < 4; ++c) q[c] *= inv;
+    }
+    return ym_lattice_upload(l, h.data(), (int64_t)count);
+}
+
+int32_t ym_timer_begin(ym_lattice *l) {
+    if (!l) return YM_INVALID_ARGUMENT;
+    YM_HIP(hipEventRecord(l->ev0, l->stream));
+    return YM_OK;
+}
+int32_t ym_timer_end(ym_lattice *l, float *ms) {
+    if (!l || !ms) return YM_INVALID_ARGUMENT;
+    YM_HIP(hipEventRecord(l->ev1, l->stream));
+    YM_HIP(hipEventSynchronize(l->ev1));
+    YM_HIP(hipEventElapsedTime(ms, l->ev0, l->ev1));
+    return YM_OK;
+}
+
+// su2_ensemble.rs:375-463: mean, variance (/N), autocorrelation with /(N-lag),
+// positive-window tau = 1/2 + sum of the leading positive rho, ESS =
+// min(N, N / 2 tau), SE = sqrt(var / ESS); sums in index order.
+int32_t ym_diagnostics(const double *v, int64_t n, int64_t max_lag, ym_chain_diagnostics *out, double *autocorr) {
+    if (!v || !out) return YM_INVALID_ARGUMENT;
+    if (n < 2) return YM_MEASUREMENT_COUNT;
+    if (max_lag < 0 || max_lag >= n) return YM_LAG_OUT_OF_RANGE;
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return YM_NON_FINITE_STATISTIC;
+    const double nd = (double)n;
+    double s = -0.0;
+    for (int64_t i = 0; i < n; ++i) s += v[i];
+    const double mean = s / nd;
+    double q = -0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double d = v[i] - mean;
+        q += d * d;
+    }
+    const double var = q / nd;
+    if (!std::isfinite(mean) || !std::isfinite(var)) return YM_NON_FINITE_STATISTIC;
+    out->mean = mean;
+    out->variance = var;
+    if (autocorr) autocorr[0] = 1.0;
+    if (var == 0.0) {
+        if (autocorr)
+            for (int64_t k = 1; k <= max_lag; ++k) autocorr[k] = 0.0;
+        out->positive_window_lag = 0;
+        out->tau_int = 0.5;
+        out->ess = nd;
+        out->standard_error = 0.0;
+        return YM_OK;
+    }
+    double tau = 0.5;
+    int64_t wlag = 0;
+    bool open = true;
+    for (int64_t lag = 1; lag <= max_lag; ++lag) {
+        const int64_t pairs = n - lag;
+        double c = -0.0;
+        for (int64_t i = 0; i < pairs; ++i) c += (v[i] - mean) * (v[i + lag] - mean);
+        const double rho = (c / (double)pairs) / var;
+        if (!std::isfinite(rho)) return YM_NON_FINITE_STATISTIC;
+        if (autocorr) autocorr[lag] = rho;
+        if (open && rho > 0.0) {
+            tau += rho;
+            wlag = lag;
+        } else {
+            open = false;
+        }
+    }
+    if (!std::isfinite(tau) || tau < 0.5) return YM_NON_FINITE_STATISTIC;
+    const double ess = std::fmin(nd / (2.0 * tau), nd);
+    const double se = std::sqrt(var / ess);
+    if (!std::isfinite(ess) || !std::isfinite(se)) return YM_NON_FINITE_STATISTIC;
+    out->positive_window_lag = wlag;
+    out->tau_int = tau;
+    out->ess = ess;
+    out->standard_error = se;
+    return YM_OK;
+}
+
+int32_t ym_su3_run_chain(const ym_su3_chain_config *c, double *meas, ym_su3_chain_report *rep) {
+    if (!c || !meas || !rep) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(c->beta) || c->beta < 0.0 || c->beta > 12.0) return YM_BETA_OUT_OF_RANGE;
+    if (!std::isfinite(c->proposal_angle) || c->proposal_angle < 1.0e-8 || c->proposal_angle > 1.0) return YM_ANGLE_OUT_OF_RANGE;
+    if (c->measurements < 2) return YM_MEASUREMENT_COUNT;
+    if (c->sweeps_between_measurements <= 0) return YM_ZERO_STRIDE;
+    if (c->autocorrelation_max_lag < 0 || c->autocorrelation_max_lag >= c->measurements) return YM_LAG_OUT_OF_RANGE;
+    if (c->thermalization_sweeps < 0 || c->hits < 1) return YM_SCHEDULE_OUT_OF_RANGE;
+    ym_lattice *l = nullptr;
+    int32_t st = ym_lattice_create(3, c->extents, c->precision, &l);
+    if (st) return st;
+    const uint64_t key = ym_derive_su3_chain_seed(c->master_seed, c->chain_id);
+    if (c->hot_start) st = ym_lattice_hot_start(l, key);
+    uint64_t prop = 0, acc = 0, sweep = 0;
+    if (!st && c->thermalization_sweeps > 0) {
+        st = ym_su3_metropolis_sweeps(l, c->beta, c->proposal_angle, c->hits, key, sweep, (int32_t)c->thermalization_sweeps,
+                                      c->reunitarize, &prop, &acc);
+        sweep += (uint64_t)c->thermalization_sweeps;
+    }
+    for (int64_t m = 0; !st && m < c->measurements; ++m) {
+        st = ym_su3_metropolis_sweeps(l, c->beta, c->proposal_angle, c->hits, key, sweep,
+                                      (int32_t)c->sweeps_between_measurements, c->reunitarize, &prop, &acc);
+        sweep += (uint64_t)c->sweeps_between_measurements;
+        if (!st) st = ym_average_plaquette(l, &meas[m]);
+        if (!st && (meas[m] < -0.5 - 1e-12 || meas[m] > 1.0 + 1e-12)) st = YM_NON_FINITE_STATISTIC;
+    }
+    if (!st) st = ym_diagnostics(meas, c->measurements, c->autocorrelation_max_lag, &rep->diagnostics, nullptr);
+    if (!st) st = ym_link_residuals(l, &rep->final_unitarity_residual, &rep->final_determinant_residual);
+    if (!st) {
+        rep->derived_seed = key;
+        rep->total_sweeps = (int64_t)sweep;
+        rep->proposals = prop;
+        rep->accepts = acc;
+        rep->mean_acceptance = prop ? (double)acc / (double)prop : 0.0;
+    }
+    ym_lattice_destroy(l);
+    return st;
+}
+
+}  // extern "C"

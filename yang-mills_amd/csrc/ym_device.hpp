@@ -1,0 +1,604 @@
+// Device-side arithmetic for the gfx950 lattice kernels.
+//
+// Everything here is written with explicit fused multiply-adds and compiled
+// with -ffp-contract=off, so the rounding of every operation is fixed by the
+// source: a CPU replay that performs the same operations in the same order
+// reproduces the GPU chain bit for bit.
+//
+// Contents
+//  * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) counter-based RNG.
+//  * FP64 exp(-d) (Cody-Waite reduction + degree-13 Taylor), sin/cos on
+//    |x| <= pi/4 (Taylor to x^19/x^20), sin/cos of 2*pi*t with an exact
+//    quarter-turn reduction, log (atanh series), log1p, expm1.
+//  * Checkerboard slot geometry: per (mu, parity) a plane of Vh slots; the
+//    slot of a site is (row*N3p + x3) >> 1, row = (x0*N1 + x1)*N2 + x2 and
+//    N3p = N3 rounded up to even (odd N3 leaves one padding slot per row).
+//  * 3x3 complex (SU(3)) and quaternion (SU(2)) products in Real = float or
+//    double with a fixed FMA chain.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ym {
+
+// ------------------------------------------------------------------ RNG
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Counter layout: (lexicographic site, mu | kind << 2 | index << 5, sweep lo, sweep hi).
+enum : uint32_t { KIND_SU3_HIT = 0, KIND_HB_ATTEMPT = 1, KIND_HB_DIRECTION = 2, KIND_HOT_START = 3, KIND_LEARNED = 4 };
+
+__device__ __forceinline__ void draw(uint64_t key, uint32_t site, int mu, uint32_t kind, uint32_t index,
+                                     uint64_t sweep, uint32_t out[4]) {
+    philox4x32_10(site, (uint32_t)mu | (kind << 2) | (index << 5), (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                  (uint32_t)key, (uint32_t)(key >> 32), out);
+}
+__device__ __forceinline__ double unit53(uint32_t lo, uint32_t hi) {
+    uint64_t v = ((uint64_t)hi << 32) | lo;
+    return (double)(v >> 11) * 0x1p-53;
+}
+
+// Reference angle grid (su3_ensemble.rs:199-213): the high bit selects the
+// sign, the low 52 bits an odd multiple of 2^-53 of the proposal angle.
+__device__ __forceinline__ double angle_from_raw(uint32_t lo, uint32_t hi, double proposal_angle) {
+    uint64_t raw = ((uint64_t)hi << 32) | lo;
+    uint64_t odd = (raw & ((1ull << 52) - 1ull)) * 2ull + 1ull;
+    double magnitude = proposal_angle * (double)odd * 0x1p-53;
+    return (hi >> 31) ? magnitude : -magnitude;
+}
+
+// ------------------------------------------------------------ FP64 math
+__device__ __forceinline__ double exp_neg(double d) {
+    if (!(d <= 745.0)) return 0.0;
+    double x = -d;
+    double k = __builtin_rint(x * 0x1.71547652b82fep+0);
+    double r = __builtin_fma(-k, 0x1.62e42fefa39efp-1, x);
+    r = __builtin_fma(-k, 0x1.abc9e3b39803fp-56, r);
+    double p = 0x1.6124613a86d09p-33;
+    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);
+    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0x1.0000000000000p-1);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)k);
+}
+
+// sin x and cos x - 1, |x| <= pi/4.
+__device__ __forceinline__ void sincos_small(double x, double &s, double &cm1) {
+    double x2 = x * x;
+    double ps = -0x1.2f49b46814157p-57;
+    ps = __builtin_fma(ps, x2, 0x1.952c77030ad4ap-49);
+    ps = __builtin_fma(ps, x2, -0x1.ae7f3e733b81fp-41);
+    ps = __builtin_fma(ps, x2, 0x1.6124613a86d09p-33);
+    ps = __builtin_fma(ps, x2, -0x1.ae64567f544e4p-26);
+    ps = __builtin_fma(ps, x2, 0x1.71de3a556c734p-19);
+    ps = __builtin_fma(ps, x2, -0x1.a01a01a01a01ap-13);
+    ps = __builtin_fma(ps, x2, 0x1.1111111111111p-7);
+    ps = __builtin_fma(ps, x2, -0x1.5555555555555p-3);
+    s = __builtin_fma(x * x2, ps, x);
+    double pc = 0x1.e542ba4020225p-62;
+    pc = __builtin_fma(pc, x2, -0x1.6827863b97d97p-53);
+    pc = __builtin_fma(pc, x2, 0x1.ae7f3e733b81fp-45);
+    pc = __builtin_fma(pc, x2, -0x1.93974a8c07c9dp-37);
+    pc = __builtin_fma(pc, x2, 0x1.1eed8eff8d898p-29);
+    pc = __builtin_fma(pc, x2, -0x1.27e4fb7789f5cp-22);
+    pc = __builtin_fma(pc, x2, 0x1.a01a01a01a01ap-16);
+    pc = __builtin_fma(pc, x2, -0x1.6c16c16c16c17p-10);
+    pc = __builtin_fma(pc, x2, 0x1.5555555555555p-5);
+    pc = __builtin_fma(pc, x2, -0x1p-1);
+    cm1 = x2 * pc;
+}
+
+// sin(2 pi t), cos(2 pi t) for t in [0, 1).
+__device__ __forceinline__ void sincos_turns(double t, double &s, double &c) {
+    double q = __builtin_rint(4.0 * t);
+    double tr = t - q * 0.25;
+    double x = tr * 0x1.921fb54442d18p+2;
+    double ss, cm1;
+    sincos_small(x, ss, cm1);
+    double cc = 1.0 + cm1;
+    int quad = ((int)q) & 3;
+    if (quad == 0) { s = ss; c = cc; }
+    else if (quad == 1) { s = cc; c = -ss; }
+    else if (quad == 2) { s = -ss; c = -cc; }
+    else { s = -cc; c = ss; }
+}
+
+__device__ __forceinline__ double log_pos(double w) {
+    int e;
+    double m = __builtin_frexp(w, &e);
+    if (m < 0x1.6a09e667f3bcdp-1) { m = m * 2.0; e -= 1; }
+    double f = (m - 1.0) / (m + 1.0);
+    double f2 = f * f;
+    double q = 0x1.642c8590b2164p-5;
+    q = __builtin_fma(q, f2, 0x1.8618618618618p-5);
+    q = __builtin_fma(q, f2, 0x1.af286bca1af28p-5);
+    q = __builtin_fma(q, f2, 0x1.e1e1e1e1e1e1ep-5);
+    q = __builtin_fma(q, f2, 0x1.1111111111111p-4);
+    q = __builtin_fma(q, f2, 0x1.3b13b13b13b14p-4);
+    q = __builtin_fma(q, f2, 0x1.745d1745d1746p-4);
+    q = __builtin_fma(q, f2, 0x1.c71c71c71c71cp-4);
+    q = __builtin_fma(q, f2, 0x1.2492492492492p-3);
+    q = __builtin_fma(q, f2, 0x1.999999999999ap-3);
+    q = __builtin_fma(q, f2, 0x1.5555555555555p-2);
+    double tf = 2.0 * f;
+    double lm = __builtin_fma(tf * f2, q, tf);
+    double de = (double)e;
+    return __builtin_fma(de, 0x1.62e42fefa39efp-1, __builtin_fma(de, 0x1.abc9e3b39803fp-56, lm));
+}
+
+__device__ __forceinline__ double log1p_dev(double y) {
+    double w = 1.0 + y;
+    if (w == 1.0) return y;
+    return log_pos(w) + (y - (w - 1.0)) / w;
+}
+
+__device__ __forceinline__ double expm1_nonpos(double x) {
+    if (x > -0x1.62e42fefa39efp-2) {
+        double p = 0x1.93974a8c07c9dp-37;
+        p = __builtin_fma(p, x, 0x1.6124613a86d09p-33);
+        p = __builtin_fma(p, x, 0x1.1eed8eff8d898p-29);
+        p = __builtin_fma(p, x, 0x1.ae64567f544e4p-26);
+        p = __builtin_fma(p, x, 0x1.27e4fb7789f5cp-22);
+        p = __builtin_fma(p, x, 0x1.71de3a556c734p-19);
+        p = __builtin_fma(p, x, 0x1.a01a01a01a01ap-16);
+        p = __builtin_fma(p, x, 0x1.a01a01a01a01ap-13);
+        p = __builtin_fma(p, x, 0x1.6c16c16c16c17p-10);
+        p = __builtin_fma(p, x, 0x1.1111111111111p-7);
+        p = __builtin_fma(p, x, 0x1.5555555555555p-5);
+        p = __builtin_fma(p, x, 0x1.5555555555555p-3);
+        p = __builtin_fma(p, x, 0x1p-1);
+        return __builtin_fma(p, x * x, x);
+    }
+    return exp_neg(-x) - 1.0;
+}
+
+// ------------------------------------------------------------- geometry
+struct Geom {
+    int n[4];
+    int n3p;   // N3 rounded up to even
+    int half;  // N3p / 2: slots per row and parity
+    int vh;    // slots per (mu, parity) plane
+    int volume;
+};
+
+// Site of slot s in parity p; returns false for the padding slot of odd N3.
+__device__ __forceinline__ bool site_of(const Geom &g, int p, int s, int x[4]) {
+    int row = s / g.half;
+    int j = s - row * g.half;
+    int r2 = row / g.n[2];
+    x[2] = row - r2 * g.n[2];
+    x[0] = r2 / g.n[1];
+    x[1] = r2 - x[0] * g.n[1];
+    x[3] = 2 * j + ((p + x[0] + x[1] + x[2]) & 1);
+    return x[3] < g.n[3];
+}
+__device__ __forceinline__ int slot_of(const Geom &g, const int x[4]) {
+    int row = (x[0] * g.n[1] + x[1]) * g.n[2] + x[2];
+    return (row * g.n3p + x[3]) >> 1;
+}
+__device__ __forceinline__ int lex_of(const Geom &g, const int x[4]) {
+    return ((x[0] * g.n[1] + x[1]) * g.n[2] + x[2]) * g.n[3] + x[3];
+}
+__device__ __forceinline__ int parity_of(const int x[4]) { return (x[0] + x[1] + x[2] + x[3]) & 1; }
+__device__ __forceinline__ void shifted(const Geom &g, const int x[4], int d, int s, int y[4]) {
+    y[0] = x[0]; y[1] = x[1]; y[2] = x[2]; y[3] = x[3];
+    int v = x[d] + s;
+    if (v < 0) v += g.n[d];
+    if (v >= g.n[d]) v -= g.n[d];
+    y[d] = v;
+}
+
+// Logical block index with consecutive tiles kept on one XCD: the hardware
+// deals workgroups round-robin over the 8 XCDs (block b -> XCD b % 8), so XCD
+// k receives a contiguous range of logical tiles and its L2 sees the
+// neighbouring slots of its own tiles.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    int xcd = b & 7, idx = b >> 3;
+    int q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// ------------------------------------------------------------ SU(3) types
+template <typename Real> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+template <typename Real> struct M3 { Real v[18]; };  // row-major, re/im interleaved
+
+#define YM_RE(m, i, j) (m).v[2 * (3 * (i) + (j))]
+#define YM_IM(m, i, j) (m).v[2 * (3 * (i) + (j)) + 1]
+
+__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// SU(3) links: for each (mu, parity) 9 planes of Vh complex pairs.
+template <typename Real>
+__device__ __forceinline__ M3<Real> load3(const typename Vec2<Real>::type *__restrict__ L, int vh, int mu, int par, int s) {
+    M3<Real> m;
+    const typename Vec2<Real>::type *base = L + (size_t)(mu * 2 + par) * 9 * vh + s;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        typename Vec2<Real>::type t = base[(size_t)e * vh];
+        m.v[2 * e] = t.x;
+        m.v[2 * e + 1] = t.y;
+    }
+    return m;
+}
+template <typename Real>
+__device__ __forceinline__ void store3(typename Vec2<Real>::type *__restrict__ L, int vh, int mu, int par, int s, const M3<Real> &m) {
+    typename Vec2<Real>::type *base = L + (size_t)(mu * 2 + par) * 9 * vh + s;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        typename Vec2<Real>::type t;
+        t.x = m.v[2 * e];
+        t.y = m.v[2 * e + 1];
+        base[(size_t)e * vh] = t;
+    }
+}
+
+// T = X Y^dagger
+template <typename Real>
+__device__ __forceinline__ void mul_nd(const M3<Real> &X, const M3<Real> &Y, M3<Real> &T) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(X, i, 0) * YM_RE(Y, j, 0);
+            re = fmaR(YM_IM(X, i, 0), YM_IM(Y, j, 0), re);
+            re = fmaR(YM_RE(X, i, 1), YM_RE(Y, j, 1), re);
+            re = fmaR(YM_IM(X, i, 1), YM_IM(Y, j, 1), re);
+            re = fmaR(YM_RE(X, i, 2), YM_RE(Y, j, 2), re);
+            re = fmaR(YM_IM(X, i, 2), YM_IM(Y, j, 2), re);
+            Real im = YM_IM(X, i, 0) * YM_RE(Y, j, 0);
+            im = fmaR(-YM_RE(X, i, 0), YM_IM(Y, j, 0), im);
+            im = fmaR(YM_IM(X, i, 1), YM_RE(Y, j, 1), im);
+            im = fmaR(-YM_RE(X, i, 1), YM_IM(Y, j, 1), im);
+            im = fmaR(YM_IM(X, i, 2), YM_RE(Y, j, 2), im);
+            im = fmaR(-YM_RE(X, i, 2), YM_IM(Y, j, 2), im);
+            YM_RE(T, i, j) = re;
+            YM_IM(T, i, j) = im;
+        }
+}
+// T = X^dagger Y^dagger
+template <typename Real>
+__device__ __forceinline__ void mul_dd(const M3<Real> &X, const M3<Real> &Y, M3<Real> &T) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(X, 0, i) * YM_RE(Y, j, 0);
+            re = fmaR(-YM_IM(X, 0, i), YM_IM(Y, j, 0), re);
+            re = fmaR(YM_RE(X, 1, i), YM_RE(Y, j, 1), re);
+            re = fmaR(-YM_IM(X, 1, i), YM_IM(Y, j, 1), re);
+            re = fmaR(YM_RE(X, 2, i), YM_RE(Y, j, 2), re);
+            re = fmaR(-YM_IM(X, 2, i), YM_IM(Y, j, 2), re);
+            Real im = (-YM_RE(X, 0, i)) * YM_IM(Y, j, 0);
+            im = fmaR(-YM_IM(X, 0, i), YM_RE(Y, j, 0), im);
+            im = fmaR(-YM_RE(X, 1, i), YM_IM(Y, j, 1), im);
+            im = fmaR(-YM_IM(X, 1, i), YM_RE(Y, j, 1), im);
+            im = fmaR(-YM_RE(X, 2, i), YM_IM(Y, j, 2), im);
+            im = fmaR(-YM_IM(X, 2, i), YM_RE(Y, j, 2), im);
+            YM_RE(T, i, j) = re;
+            YM_IM(T, i, j) = im;
+        }
+}
+// A += T Z^dagger
+template <typename Real>
+__device__ __forceinline__ void acc_nd(M3<Real> &A, const M3<Real> &T, const M3<Real> &Z) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(A, i, j), im = YM_IM(A, i, j);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                re = fmaR(YM_RE(T, i, k), YM_RE(Z, j, k), re);
+                re = fmaR(YM_IM(T, i, k), YM_IM(Z, j, k), re);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                im = fmaR(YM_IM(T, i, k), YM_RE(Z, j, k), im);
+                im = fmaR(-YM_RE(T, i, k), YM_IM(Z, j, k), im);
+            }
+            YM_RE(A, i, j) = re;
+            YM_IM(A, i, j) = im;
+        }
+}
+// A += T Z
+template <typename Real>
+__device__ __forceinline__ void acc_nn(M3<Real> &A, const M3<Real> &T, const M3<Real> &Z) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(A, i, j), im = YM_IM(A, i, j);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                re = fmaR(YM_RE(T, i, k), YM_RE(Z, k, j), re);
+                re = fmaR(-YM_IM(T, i, k), YM_IM(Z, k, j), re);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                im = fmaR(YM_RE(T, i, k), YM_IM(Z, k, j), im);
+                im = fmaR(YM_IM(T, i, k), YM_RE(Z, k, j), im);
+            }
+            YM_RE(A, i, j) = re;
+            YM_IM(A, i, j) = im;
+        }
+}
+
+// Plain (not FMA-chained) product used by observables: C = X Y.
+template <typename Real>
+__device__ __forceinline__ M3<Real> mul_nn(const M3<Real> &X, const M3<Real> &Y) {
+    M3<Real> C;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(X, i, 0) * YM_RE(Y, 0, j);
+            re = fmaR(-YM_IM(X, i, 0), YM_IM(Y, 0, j), re);
+            re = fmaR(YM_RE(X, i, 1), YM_RE(Y, 1, j), re);
+            re = fmaR(-YM_IM(X, i, 1), YM_IM(Y, 1, j), re);
+            re = fmaR(YM_RE(X, i, 2), YM_RE(Y, 2, j), re);
+            re = fmaR(-YM_IM(X, i, 2), YM_IM(Y, 2, j), re);
+            Real im = YM_RE(X, i, 0) * YM_IM(Y, 0, j);
+            im = fmaR(YM_IM(X, i, 0), YM_RE(Y, 0, j), im);
+            im = fmaR(YM_RE(X, i, 1), YM_IM(Y, 1, j), im);
+            im = fmaR(YM_IM(X, i, 1), YM_RE(Y, 1, j), im);
+            im = fmaR(YM_RE(X, i, 2), YM_IM(Y, 2, j), im);
+            im = fmaR(YM_IM(X, i, 2), YM_RE(Y, 2, j), im);
+            YM_RE(C, i, j) = re;
+            YM_IM(C, i, j) = im;
+        }
+    return C;
+}
+template <typename Real>
+__device__ __forceinline__ M3<Real> dagger3(const M3<Real> &X) {
+    M3<Real> D;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            YM_RE(D, i, j) = YM_RE(X, j, i);
+            YM_IM(D, i, j) = -YM_IM(X, j, i);
+        }
+    return D;
+}
+// Re Tr(X Y) without forming the product.
+template <typename Real>
+__device__ __forceinline__ double re_trace_mul(const M3<Real> &X, const M3<Real> &Y) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            t = __builtin_fma((double)YM_RE(X, i, k), (double)YM_RE(Y, k, i), t);
+            t = __builtin_fma(-(double)YM_IM(X, i, k), (double)YM_IM(Y, k, i), t);
+        }
+    return t;
+}
+template <typename Real>
+__device__ __forceinline__ M3<Real> identity3() {
+    M3<Real> m;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) m.v[c] = 0;
+    m.v[0] = 1; m.v[8] = 1; m.v[16] = 1;
+    return m;
+}
+
+// ------------------------------------------------ FP64 link algebra (SU(3))
+// rows i,j <- [[c, i s],[i s, c]] (lambda_1 / lambda_4 / lambda_6 block)
+__device__ __forceinline__ void rot_sym(M3<double> &m, int i, int j, double c, double s) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double xr = YM_RE(m, i, k), xi = YM_IM(m, i, k), yr = YM_RE(m, j, k), yi = YM_IM(m, j, k);
+        YM_RE(m, i, k) = __builtin_fma(c, xr, -(s * yi));
+        YM_IM(m, i, k) = __builtin_fma(c, xi, s * yr);
+        YM_RE(m, j, k) = __builtin_fma(c, yr, -(s * xi));
+        YM_IM(m, j, k) = __builtin_fma(c, yi, s * xr);
+    }
+}
+// rows i,j <- [[c, s],[-s, c]] (lambda_2 / lambda_5 / lambda_7 block)
+__device__ __forceinline__ void rot_asym(M3<double> &m, int i, int j, double c, double s) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double xr = YM_RE(m, i, k), xi = YM_IM(m, i, k), yr = YM_RE(m, j, k), yi = YM_IM(m, j, k);
+        YM_RE(m, i, k) = __builtin_fma(c, xr, s * yr);
+        YM_IM(m, i, k) = __builtin_fma(c, xi, s * yi);
+        YM_RE(m, j, k) = __builtin_fma(c, yr, -(s * xr));
+        YM_IM(m, j, k) = __builtin_fma(c, yi, -(s * xi));
+    }
+}
+// row i <- (c + i s) row i
+__device__ __forceinline__ void rot_phase(M3<double> &m, int i, double c, double s) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double xr = YM_RE(m, i, k), xi = YM_IM(m, i, k);
+        YM_RE(m, i, k) = __builtin_fma(c, xr, -(s * xi));
+        YM_IM(m, i, k) = __builtin_fma(c, xi, s * xr);
+    }
+}
+
+// Gram-Schmidt on rows 0 and 1, row 2 = conj(row0 x row1).
+__device__ __forceinline__ void reunitarize3(M3<double> &u) {
+    double n = YM_RE(u, 0, 0) * YM_RE(u, 0, 0);
+    n = __builtin_fma(YM_IM(u, 0, 0), YM_IM(u, 0, 0), n);
+    n = __builtin_fma(YM_RE(u, 0, 1), YM_RE(u, 0, 1), n);
+    n = __builtin_fma(YM_IM(u, 0, 1), YM_IM(u, 0, 1), n);
+    n = __builtin_fma(YM_RE(u, 0, 2), YM_RE(u, 0, 2), n);
+    n = __builtin_fma(YM_IM(u, 0, 2), YM_IM(u, 0, 2), n);
+    double inv = 1.0 / __builtin_sqrt(n);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { YM_RE(u, 0, k) *= inv; YM_IM(u, 0, k) *= inv; }
+    double pr = YM_RE(u, 0, 0) * YM_RE(u, 1, 0);
+    pr = __builtin_fma(YM_IM(u, 0, 0), YM_IM(u, 1, 0), pr);
+    pr = __builtin_fma(YM_RE(u, 0, 1), YM_RE(u, 1, 1), pr);
+    pr = __builtin_fma(YM_IM(u, 0, 1), YM_IM(u, 1, 1), pr);
+    pr = __builtin_fma(YM_RE(u, 0, 2), YM_RE(u, 1, 2), pr);
+    pr = __builtin_fma(YM_IM(u, 0, 2), YM_IM(u, 1, 2), pr);
+    double pi = YM_RE(u, 0, 0) * YM_IM(u, 1, 0);
+    pi = __builtin_fma(-YM_IM(u, 0, 0), YM_RE(u, 1, 0), pi);
+    pi = __builtin_fma(YM_RE(u, 0, 1), YM_IM(u, 1, 1), pi);
+    pi = __builtin_fma(-YM_IM(u, 0, 1), YM_RE(u, 1, 1), pi);
+    pi = __builtin_fma(YM_RE(u, 0, 2), YM_IM(u, 1, 2), pi);
+    pi = __builtin_fma(-YM_IM(u, 0, 2), YM_RE(u, 1, 2), pi);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double ar = YM_RE(u, 0, k), ai = YM_IM(u, 0, k);
+        YM_RE(u, 1, k) = __builtin_fma(-pr, ar, __builtin_fma(pi, ai, YM_RE(u, 1, k)));
+        YM_IM(u, 1, k) = __builtin_fma(-pr, ai, __builtin_fma(-pi, ar, YM_IM(u, 1, k)));
+    }
+    n = YM_RE(u, 1, 0) * YM_RE(u, 1, 0);
+    n = __builtin_fma(YM_IM(u, 1, 0), YM_IM(u, 1, 0), n);
+    n = __builtin_fma(YM_RE(u, 1, 1), YM_RE(u, 1, 1), n);
+    n = __builtin_fma(YM_IM(u, 1, 1), YM_IM(u, 1, 1), n);
+    n = __builtin_fma(YM_RE(u, 1, 2), YM_RE(u, 1, 2), n);
+    n = __builtin_fma(YM_IM(u, 1, 2), YM_IM(u, 1, 2), n);
+    inv = 1.0 / __builtin_sqrt(n);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { YM_RE(u, 1, k) *= inv; YM_IM(u, 1, k) *= inv; }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int p = (k + 1) % 3, q = (k + 2) % 3;
+        double ar = YM_RE(u, 0, p), ai = YM_IM(u, 0, p), br = YM_RE(u, 1, q), bi = YM_IM(u, 1, q);
+        double cr = YM_RE(u, 0, q), ci = YM_IM(u, 0, q), dr = YM_RE(u, 1, p), di = YM_IM(u, 1, p);
+        double re = ar * br;
+        re = __builtin_fma(-ai, bi, re);
+        re = __builtin_fma(-cr, dr, re);
+        re = __builtin_fma(ci, di, re);
+        double im = ar * bi;
+        im = __builtin_fma(ai, br, im);
+        im = __builtin_fma(-cr, di, im);
+        im = __builtin_fma(-ci, dr, im);
+        YM_RE(u, 2, k) = re;
+        YM_IM(u, 2, k) = -im;
+    }
+}
+
+// ------------------------------------------------------------ SU(2) types
+template <typename Real> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+
+template <typename Real> struct Q4 { Real c[4]; };
+
+template <typename Real>
+__device__ __forceinline__ Q4<Real> load2(const typename Vec4<Real>::type *__restrict__ L, int vh, int mu, int par, int s) {
+    typename Vec4<Real>::type t = L[(size_t)(mu * 2 + par) * vh + s];
+    Q4<Real> q;
+    q.c[0] = t.x; q.c[1] = t.y; q.c[2] = t.z; q.c[3] = t.w;
+    return q;
+}
+template <typename Real>
+__device__ __forceinline__ void store2(typename Vec4<Real>::type *__restrict__ L, int vh, int mu, int par, int s, const Q4<Real> &q) {
+    typename Vec4<Real>::type t;
+    t.x = q.c[0]; t.y = q.c[1]; t.z = q.c[2]; t.w = q.c[3];
+    L[(size_t)(mu * 2 + par) * vh + s] = t;
+}
+// Quaternion product in the reference convention (lib.rs:212-224) as a fixed FMA chain.
+template <typename Real>
+__device__ __forceinline__ Q4<Real> qmul(const Q4<Real> &a, const Q4<Real> &b) {
+    Q4<Real> r;
+    Real r0 = a.c[0] * b.c[0];
+    r0 = fmaR(-a.c[1], b.c[1], r0); r0 = fmaR(-a.c[2], b.c[2], r0); r0 = fmaR(-a.c[3], b.c[3], r0);
+    Real r1 = a.c[0] * b.c[1];
+    r1 = fmaR(a.c[1], b.c[0], r1); r1 = fmaR(a.c[2], b.c[3], r1); r1 = fmaR(-a.c[3], b.c[2], r1);
+    Real r2 = a.c[0] * b.c[2];
+    r2 = fmaR(-a.c[1], b.c[3], r2); r2 = fmaR(a.c[2], b.c[0], r2); r2 = fmaR(a.c[3], b.c[1], r2);
+    Real r3 = a.c[0] * b.c[3];
+    r3 = fmaR(a.c[1], b.c[2], r3); r3 = fmaR(-a.c[2], b.c[1], r3); r3 = fmaR(a.c[3], b.c[0], r3);
+    r.c[0] = r0; r.c[1] = r1; r.c[2] = r2; r.c[3] = r3;
+    return r;
+}
+template <typename Real>
+__device__ __forceinline__ Q4<Real> qdag(const Q4<Real> &a) {
+    Q4<Real> r;
+    r.c[0] = a.c[0]; r.c[1] = -a.c[1]; r.c[2] = -a.c[2]; r.c[3] = -a.c[3];
+    return r;
+}
+__device__ __forceinline__ double qnorm2(const Q4<double> &a) {
+    double n = a.c[0] * a.c[0];
+    n = __builtin_fma(a.c[1], a.c[1], n);
+    n = __builtin_fma(a.c[2], a.c[2], n);
+    n = __builtin_fma(a.c[3], a.c[3], n);
+    return n;
+}
+__device__ __forceinline__ void qnormalize(Q4<double> &a) {
+    double inv = 1.0 / __builtin_sqrt(qnorm2(a));
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a.c[c] *= inv;
+}
+
+// ----------------------------------------------------- block reductions
+// 64-wide wavefront reduction of a double, then across the block's waves
+// through LDS.  Every thread of the block must call it.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) t += lds[w];
+    }
+    __syncthreads();
+    return t;  // valid in thread 0
+}
+template <int BLOCK>
+__device__ __forceinline__ double block_max(double v, double *lds) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) t = fmax(t, lds[w]);
+    }
+    __syncthreads();
+    return t;
+}
+template <int BLOCK>
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long *lds) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) t += lds[w];
+    }
+    __syncthreads();
+    return t;
+}
+
+}  // namespace ym

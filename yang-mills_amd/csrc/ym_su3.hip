@@ -1,0 +1,359 @@
+// SU(3) link-update kernels for gfx950.
+//
+// k_su3_metropolis: one checkerboard sub-pass (direction MU, parity p) of
+// single-generator Metropolis.  One thread owns one link of the sub-pass; no
+// link it reads is written in the same sub-pass, so all threads run
+// independently.  Per link:
+//   1. staple A (6 terms) in the storage precision with a fixed FMA chain;
+//   2. M = U A in FP64, so that Re Tr(U' A) - Re Tr(U A) = Re Tr((R - I) M)
+//      for U' = R U costs a few flops per hit;
+//   3. `hits` proposals, generator h mod 8, angle from the reference's paired
+//      grid driven by Philox(site, mu, hit, sweep); accept iff dS <= 0 or
+//      u < exp(-dS) in FP64; on accept U <- R U and M <- R M;
+//   4. FP64 Gram-Schmidt reunitarisation, then one rounding to storage.
+// The CPU replay in the oracle performs the same operations in the same order.
+#include "ym_device.hpp"
+#include "ym_internal.hpp"
+
+namespace ym {
+
+template <typename Real, int MU>
+__global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::type *__restrict__ L, Geom g, int p,
+                                                        double nbt, double angle, int hits, uint64_t key,
+                                                        uint64_t sweep, int reunit,
+                                                        unsigned long long *__restrict__ acc_out) {
+    __shared__ unsigned long long red[4];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int s = tile * 256 + threadIdx.x;
+    unsigned long long nacc = 0;
+    int x[4];
+    if (s < g.vh && site_of(g, p, s, x)) {
+        const int q = p ^ 1;
+        M3<Real> A;
+#pragma unroll
+        for (int c = 0; c < 18; ++c) A.v[c] = 0;
+        int xpm[4];
+        shifted(g, x, MU, +1, xpm);
+        const int s_pm = slot_of(g, xpm);
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) {
+            if (nu == MU) continue;
+            int xpn[4], xmn[4], xpmmn[4];
+            shifted(g, x, nu, +1, xpn);
+            shifted(g, x, nu, -1, xmn);
+            shifted(g, xpm, nu, -1, xpmmn);
+            const int s_pn = slot_of(g, xpn), s_mn = slot_of(g, xmn), s_pmmn = slot_of(g, xpmmn);
+            M3<Real> T;
+            {
+                M3<Real> X = load3<Real>(L, g.vh, nu, q, s_pm);
+                M3<Real> Y = load3<Real>(L, g.vh, MU, q, s_pn);
+                mul_nd(X, Y, T);
+                M3<Real> Z = load3<Real>(L, g.vh, nu, p, s);
+                acc_nd(A, T, Z);
+            }
+            {
+                M3<Real> X = load3<Real>(L, g.vh, nu, p, s_pmmn);
+                M3<Real> Y = load3<Real>(L, g.vh, MU, q, s_mn);
+                mul_dd(X, Y, T);
+                M3<Real> Z = load3<Real>(L, g.vh, nu, q, s_mn);
+                acc_nn(A, T, Z);
+            }
+        }
+        M3<double> u, M;
+        {
+            M3<Real> Uf = load3<Real>(L, g.vh, MU, p, s);
+#pragma unroll
+            for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double re = YM_RE(u, i, 0) * (double)YM_RE(A, 0, j);
+                re = __builtin_fma(-YM_IM(u, i, 0), (double)YM_IM(A, 0, j), re);
+                re = __builtin_fma(YM_RE(u, i, 1), (double)YM_RE(A, 1, j), re);
+                re = __builtin_fma(-YM_IM(u, i, 1), (double)YM_IM(A, 1, j), re);
+                re = __builtin_fma(YM_RE(u, i, 2), (double)YM_RE(A, 2, j), re);
+                re = __builtin_fma(-YM_IM(u, i, 2), (double)YM_IM(A, 2, j), re);
+                double im = YM_RE(u, i, 0) * (double)YM_IM(A, 0, j);
+                im = __builtin_fma(YM_IM(u, i, 0), (double)YM_RE(A, 0, j), im);
+                im = __builtin_fma(YM_RE(u, i, 1), (double)YM_IM(A, 1, j), im);
+                im = __builtin_fma(YM_IM(u, i, 1), (double)YM_RE(A, 1, j), im);
+                im = __builtin_fma(YM_RE(u, i, 2), (double)YM_IM(A, 2, j), im);
+                im = __builtin_fma(YM_IM(u, i, 2), (double)YM_RE(A, 2, j), im);
+                YM_RE(M, i, j) = re;
+                YM_IM(M, i, j) = im;
+            }
+        const uint32_t site = (uint32_t)lex_of(g, x);
+        for (int h0 = 0; h0 < hits; h0 += 8) {
+#pragma unroll
+            for (int gen = 0; gen < 8; ++gen) {
+                const int h = h0 + gen;
+                if (h >= hits) break;
+                uint32_t o[4];
+                draw(key, site, MU, KIND_SU3_HIT, (uint32_t)h, sweep, o);
+                const double theta = angle_from_raw(o[0], o[1], angle);
+                const double uacc = unit53(o[2], o[3]);
+                double sn, cm1, s2 = 0.0, cm2 = 0.0, dre;
+                const int bi = (gen <= 2 || gen == 7) ? 0 : (gen <= 4 ? 0 : 1);
+                const int bj = (gen <= 2 || gen == 7) ? 1 : 2;
+                if (gen == 7) {
+                    const double psi = theta * 0x1.279a74590331cp-2;
+                    sincos_small(psi, sn, cm1);
+                    sincos_small(-2.0 * psi, s2, cm2);
+                    const double t1 = YM_RE(M, 0, 0) + YM_RE(M, 1, 1), t2 = YM_IM(M, 0, 0) + YM_IM(M, 1, 1);
+                    const double d1 = __builtin_fma(cm1, t1, -(sn * t2));
+                    const double d2 = __builtin_fma(cm2, YM_RE(M, 2, 2), -(s2 * YM_IM(M, 2, 2)));
+                    dre = d1 + d2;
+                } else if (gen == 2) {
+                    sincos_small(theta * 0.5, sn, cm1);
+                    const double t1 = YM_RE(M, 0, 0) + YM_RE(M, 1, 1), t2 = YM_IM(M, 0, 0) - YM_IM(M, 1, 1);
+                    dre = __builtin_fma(cm1, t1, -(sn * t2));
+                } else if (gen == 0 || gen == 3 || gen == 5) {
+                    sincos_small(theta * 0.5, sn, cm1);
+                    const double t1 = YM_RE(M, bi, bi) + YM_RE(M, bj, bj);
+                    const double t2 = YM_IM(M, bi, bj) + YM_IM(M, bj, bi);
+                    dre = __builtin_fma(cm1, t1, -(sn * t2));
+                } else {
+                    sincos_small(theta * 0.5, sn, cm1);
+                    const double t1 = YM_RE(M, bi, bi) + YM_RE(M, bj, bj);
+                    const double t2 = YM_RE(M, bj, bi) - YM_RE(M, bi, bj);
+                    dre = __builtin_fma(cm1, t1, sn * t2);
+                }
+                const double dS = nbt * dre;
+                const bool accept = (dS <= 0.0) || (uacc < exp_neg(dS));
+                if (accept) {
+                    nacc += 1;
+                    const double c = 1.0 + cm1;
+                    if (gen == 7) {
+                        const double c2 = 1.0 + cm2;
+                        rot_phase(u, 0, c, sn); rot_phase(u, 1, c, sn); rot_phase(u, 2, c2, s2);
+                        rot_phase(M, 0, c, sn); rot_phase(M, 1, c, sn); rot_phase(M, 2, c2, s2);
+                    } else if (gen == 2) {
+                        rot_phase(u, 0, c, sn); rot_phase(u, 1, c, -sn);
+                        rot_phase(M, 0, c, sn); rot_phase(M, 1, c, -sn);
+                    } else if (gen == 0 || gen == 3 || gen == 5) {
+                        rot_sym(u, bi, bj, c, sn); rot_sym(M, bi, bj, c, sn);
+                    } else {
+                        rot_asym(u, bi, bj, c, sn); rot_asym(M, bi, bj, c, sn);
+                    }
+                }
+            }
+        }
+        if (reunit) reunitarize3(u);
+        M3<Real> Uo;
+#pragma unroll
+        for (int c = 0; c < 18; ++c) Uo.v[c] = (Real)u.v[c];
+        store3<Real>(L, g.vh, MU, p, s, Uo);
+    }
+    unsigned long long tot = block_sum_u64<256>(nacc, red);
+    if (threadIdx.x == 0 && tot) atomicAdd(acc_out, tot);
+}
+
+template <typename Real>
+static void launch_metro_t(void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
+                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *acc, hipStream_t st) {
+    using V2 = typename Vec2<Real>::type;
+    const int nb = (g.vh + 255) / 256;
+    const double nbt = -(beta / 3.0);
+    V2 *L = (V2 *)links;
+    switch (mu) {
+    case 0: hipLaunchKernelGGL((k_su3_metropolis<Real, 0>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
+    case 1: hipLaunchKernelGGL((k_su3_metropolis<Real, 1>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
+    case 2: hipLaunchKernelGGL((k_su3_metropolis<Real, 2>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
+    default: hipLaunchKernelGGL((k_su3_metropolis<Real, 3>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
+    }
+}
+void launch_su3_metropolis(int prec, void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
+                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *acc, hipStream_t st) {
+    if (prec == 32) launch_metro_t<float>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, acc, st);
+    else launch_metro_t<double>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, acc, st);
+}
+
+// ----------------------------------------------------------- hot start
+// Haar-random SU(3): two rows of complex Gaussians (Box-Muller from Philox),
+// Gram-Schmidt, third row = conj(row0 x row1).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su3_hot(typename Vec2<Real>::type *__restrict__ L, Geom g, uint64_t key) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y, mu = blockIdx.z;
+    int x[4];
+    if (s >= g.vh) return;
+    if (!site_of(g, p, s, x)) {
+        store3<Real>(L, g.vh, mu, p, s, identity3<Real>());
+        return;
+    }
+    const uint32_t site = (uint32_t)lex_of(g, x);
+    M3<double> u;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        uint32_t o[4];
+        draw(key, site, mu, KIND_HOT_START, (uint32_t)k, 0, o);
+        const double u1 = 1.0 - unit53(o[0], o[1]);
+        const double u2 = unit53(o[2], o[3]);
+        const double r = __builtin_sqrt(-2.0 * log_pos(u1));
+        double sn, cs;
+        sincos_turns(u2, sn, cs);
+        u.v[2 * k] = r * cs;
+        u.v[2 * k + 1] = r * sn;
+    }
+#pragma unroll
+    for (int c = 12; c < 18; ++c) u.v[c] = 0.0;
+    reunitarize3(u);
+    M3<Real> o;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) o.v[c] = (Real)u.v[c];
+    store3<Real>(L, g.vh, mu, p, s, o);
+}
+void launch_su3_hot(int prec, void *links, const Geom &g, uint64_t key, hipStream_t st) {
+    dim3 grid((g.vh + 255) / 256, 2, 4);
+    if (prec == 32) hipLaunchKernelGGL((k_su3_hot<float>), grid, dim3(256), 0, st, (float2 *)links, g, key);
+    else hipLaunchKernelGGL((k_su3_hot<double>), grid, dim3(256), 0, st, (double2 *)links, g, key);
+}
+
+// --------------------------------------------------------- staple probe
+// Staple at one site, same arithmetic as the sweep, for parity tests.
+template <typename Real>
+__global__ void k_su3_staple_probe(const typename Vec2<Real>::type *__restrict__ L, Geom g, int x0, int x1, int x2,
+                                   int x3, int mu, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int x[4] = {x0, x1, x2, x3};
+    const int p = parity_of(x), q = p ^ 1, s = slot_of(g, x);
+    M3<Real> A;
+    for (int c = 0; c < 18; ++c) A.v[c] = 0;
+    int xpm[4];
+    shifted(g, x, mu, +1, xpm);
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        int xpn[4], xmn[4], xpmmn[4];
+        shifted(g, x, nu, +1, xpn);
+        shifted(g, x, nu, -1, xmn);
+        shifted(g, xpm, nu, -1, xpmmn);
+        M3<Real> T;
+        mul_nd(load3<Real>(L, g.vh, nu, parity_of(xpm), slot_of(g, xpm)), load3<Real>(L, g.vh, mu, parity_of(xpn), slot_of(g, xpn)), T);
+        acc_nd(A, T, load3<Real>(L, g.vh, nu, p, s));
+        mul_dd(load3<Real>(L, g.vh, nu, parity_of(xpmmn), slot_of(g, xpmmn)), load3<Real>(L, g.vh, mu, parity_of(xmn), slot_of(g, xmn)), T);
+        acc_nn(A, T, load3<Real>(L, g.vh, nu, parity_of(xmn), slot_of(g, xmn)));
+    }
+    (void)q;
+    for (int c = 0; c < 18; ++c) out[c] = (double)A.v[c];
+}
+void launch_su3_staple_probe(int prec, const void *links, const Geom &g, const int x[4], int mu, double *out, hipStream_t st) {
+    if (prec == 32) hipLaunchKernelGGL((k_su3_staple_probe<float>), dim3(1), dim3(64), 0, st, (const float2 *)links, g, x[0], x[1], x[2], x[3], mu, out);
+    else hipLaunchKernelGGL((k_su3_staple_probe<double>), dim3(1), dim3(64), 0, st, (const double2 *)links, g, x[0], x[1], x[2], x[3], mu, out);
+}
+
+// ------------------------------------------------------ reunitarisation
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su3_reunit(typename Vec2<Real>::type *__restrict__ L, Geom g) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y, mu = blockIdx.z;
+    int x[4];
+    if (s >= g.vh || !site_of(g, p, s, x)) return;
+    M3<Real> a = load3<Real>(L, g.vh, mu, p, s);
+    M3<double> u;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) u.v[c] = (double)a.v[c];
+    reunitarize3(u);
+#pragma unroll
+    for (int c = 0; c < 18; ++c) a.v[c] = (Real)u.v[c];
+    store3<Real>(L, g.vh, mu, p, s, a);
+}
+void launch_su3_reunit(int prec, void *links, const Geom &g, hipStream_t st) {
+    dim3 grid((g.vh + 255) / 256, 2, 4);
+    if (prec == 32) hipLaunchKernelGGL((k_su3_reunit<float>), grid, dim3(256), 0, st, (float2 *)links, g);
+    else hipLaunchKernelGGL((k_su3_reunit<double>), grid, dim3(256), 0, st, (double2 *)links, g);
+}
+
+// ------------------------------------------------------------ residuals
+// Per block: max over its links of max|U^+U - 1| (both orders) and of
+// max(|det U - 1|, |det U^+ - 1|), in FP64 (su3.rs:219-229 semantics).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su3_residuals(const typename Vec2<Real>::type *__restrict__ L, Geom g,
+                                                       double *__restrict__ part) {
+    __shared__ double red[4];
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y, mu = blockIdx.z;
+    const int blin = (mu * 2 + p) * gridDim.x + blockIdx.x;
+    double ru = 0.0, rd = 0.0;
+    int x[4];
+    if (s < g.vh && site_of(g, p, s, x)) {
+        M3<Real> a = load3<Real>(L, g.vh, mu, p, s);
+        M3<double> u;
+        for (int c = 0; c < 18; ++c) u.v[c] = (double)a.v[c];
+        M3<double> ud = dagger3(u);
+        M3<double> l = mul_nn(ud, u), r = mul_nn(u, ud);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double e = (i == j) ? 1.0 : 0.0;
+                ru = fmax(ru, hypot(YM_RE(l, i, j) - e, YM_IM(l, i, j)));
+                ru = fmax(ru, hypot(YM_RE(r, i, j) - e, YM_IM(r, i, j)));
+            }
+        // det via cofactor expansion along row 0
+        double dr = 0.0, di = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const int a1 = (k + 1) % 3, a2 = (k + 2) % 3;
+            double cr = YM_RE(u, 1, a1) * YM_RE(u, 2, a2) - YM_IM(u, 1, a1) * YM_IM(u, 2, a2) -
+                        (YM_RE(u, 1, a2) * YM_RE(u, 2, a1) - YM_IM(u, 1, a2) * YM_IM(u, 2, a1));
+            double ci = YM_RE(u, 1, a1) * YM_IM(u, 2, a2) + YM_IM(u, 1, a1) * YM_RE(u, 2, a2) -
+                        (YM_RE(u, 1, a2) * YM_IM(u, 2, a1) + YM_IM(u, 1, a2) * YM_RE(u, 2, a1));
+            dr += YM_RE(u, 0, k) * cr - YM_IM(u, 0, k) * ci;
+            di += YM_RE(u, 0, k) * ci + YM_IM(u, 0, k) * cr;
+        }
+        // det(U^+) = conj(det U): both residuals coincide up to rounding.
+        rd = hypot(dr - 1.0, di);
+    }
+    const double mu_ = block_max<256>(ru, red);
+    const double md = block_max<256>(rd, red);
+    if (threadIdx.x == 0) { part[2 * blin] = mu_; part[2 * blin + 1] = md; }
+}
+int launch_su3_residuals(int prec, const void *links, const Geom &g, double *part, hipStream_t st) {
+    dim3 grid((g.vh + 255) / 256, 2, 4);
+    if (prec == 32) hipLaunchKernelGGL((k_su3_residuals<float>), grid, dim3(256), 0, st, (const float2 *)links, g, part);
+    else hipLaunchKernelGGL((k_su3_residuals<double>), grid, dim3(256), 0, st, (const double2 *)links, g, part);
+    return (int)(grid.x * 8);
+}
+
+// ------------------------------------------------------- layout change
+// AoS FP64 (reference layout: link site*4+mu, 18 doubles) <-> device planes.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su3_from_aos(typename Vec2<Real>::type *__restrict__ L, Geom g,
+                                                      const double *__restrict__ aos) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y, mu = blockIdx.z;
+    int x[4];
+    if (s >= g.vh) return;
+    if (!site_of(g, p, s, x)) {
+        store3<Real>(L, g.vh, mu, p, s, identity3<Real>());
+        return;
+    }
+    const size_t base = ((size_t)lex_of(g, x) * 4 + mu) * 18;
+    M3<Real> m;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) m.v[c] = (Real)aos[base + c];
+    store3<Real>(L, g.vh, mu, p, s, m);
+}
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su3_to_aos(const typename Vec2<Real>::type *__restrict__ L, Geom g,
+                                                    double *__restrict__ aos) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y, mu = blockIdx.z;
+    int x[4];
+    if (s >= g.vh || !site_of(g, p, s, x)) return;
+    const size_t base = ((size_t)lex_of(g, x) * 4 + mu) * 18;
+    M3<Real> m = load3<Real>(L, g.vh, mu, p, s);
+#pragma unroll
+    for (int c = 0; c < 18; ++c) aos[base + c] = (double)m.v[c];
+}
+void launch_su3_from_aos(int prec, void *links, const Geom &g, const double *aos, hipStream_t st) {
+    dim3 grid((g.vh + 255) / 256, 2, 4);
+    if (prec == 32) hipLaunchKernelGGL((k_su3_from_aos<float>), grid, dim3(256), 0, st, (float2 *)links, g, aos);
+    else hipLaunchKernelGGL((k_su3_from_aos<double>), grid, dim3(256), 0, st, (double2 *)links, g, aos);
+}
+void launch_su3_to_aos(int prec, const void *links, const Geom &g, double *aos, hipStream_t st) {
+    dim3 grid((g.vh + 255) / 256, 2, 4);
+    if (prec == 32) hipLaunchKernelGGL((k_su3_to_aos<float>), grid, dim3(256), 0, st, (const float2 *)links, g, aos);
+    else hipLaunchKernelGGL((k_su3_to_aos<double>), grid, dim3(256), 0, st, (const double2 *)links, g, aos);
+}
+
+}  // namespace ym

@@ -1,0 +1,102 @@
+"""Loader and ctypes signatures for libym_amd.so (include/ym_amd.h)."""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.dirname(_HERE)
+
+
+def library_path():
+    return os.environ.get("YM_AMD_LIB", os.path.join(_PKG, "libym_amd.so"))
+
+
+i32, i64, u64, dbl, flt = C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_float
+P = C.POINTER
+
+
+class ChainDiagnostics(C.Structure):
+    _fields_ = [("mean", dbl), ("variance", dbl), ("positive_window_lag", i64),
+                ("tau_int", dbl), ("ess", dbl), ("standard_error", dbl)]
+
+
+class Su3ChainConfig(C.Structure):
+    _fields_ = [("extents", i64 * 4), ("beta", dbl), ("master_seed", u64), ("chain_id", u64),
+                ("thermalization_sweeps", i64), ("measurements", i64),
+                ("sweeps_between_measurements", i64), ("proposal_angle", dbl),
+                ("autocorrelation_max_lag", i64), ("hits", i32), ("precision", i32),
+                ("hot_start", i32), ("reunitarize", i32)]
+
+
+class Su3ChainReport(C.Structure):
+    _fields_ = [("derived_seed", u64), ("total_sweeps", i64), ("proposals", u64), ("accepts", u64),
+                ("mean_acceptance", dbl), ("final_unitarity_residual", dbl),
+                ("final_determinant_residual", dbl), ("diagnostics", ChainDiagnostics)]
+
+
+# name -> (restype, argtypes); the exported symbol set of include/ym_amd.h
+SIGNATURES = {
+    "ym_lattice_create": (i32, [i32, P(i64), i32, P(C.c_void_p)]),
+    "ym_lattice_destroy": (i32, [C.c_void_p]),
+    "ym_lattice_bytes": (i32, [C.c_void_p, P(i64)]),
+    "ym_lattice_set_identity": (i32, [C.c_void_p]),
+    "ym_lattice_hot_start": (i32, [C.c_void_p, u64]),
+    "ym_lattice_upload": (i32, [C.c_void_p, P(dbl), i64]),
+    "ym_lattice_download": (i32, [C.c_void_p, P(dbl), i64]),
+    "ym_lattice_copy": (i32, [C.c_void_p, C.c_void_p]),
+    "ym_su3_metropolis_sweeps": (i32, [C.c_void_p, dbl, dbl, i32, u64, u64, i32, i32, P(u64), P(u64)]),
+    "ym_su3_metropolis_sweeps_profiled": (i32, [C.c_void_p, dbl, dbl, i32, u64, u64, i32, i32, P(u64), P(u64),
+                                                P(flt), P(flt), P(i32)]),
+    "ym_su2_heatbath_sweeps": (i32, [C.c_void_p, dbl, u64, u64, i32, i32, P(u64), P(u64)]),
+    "ym_su2_overrelax_sweeps": (i32, [C.c_void_p, i32]),
+    "ym_average_plaquette": (i32, [C.c_void_p, P(dbl)]),
+    "ym_wilson_action": (i32, [C.c_void_p, dbl, P(dbl)]),
+    "ym_staple": (i32, [C.c_void_p, P(i64), i32, P(dbl)]),
+    "ym_wilson_loops": (i32, [C.c_void_p, i32, i32, i32, P(dbl), P(dbl)]),
+    "ym_polyakov_loop": (i32, [C.c_void_p, i32, P(dbl), P(dbl), P(dbl)]),
+    "ym_su3_timeslice_a1pp": (i32, [C.c_void_p, P(dbl), P(dbl)]),
+    "ym_su3_stout_spatial": (i32, [C.c_void_p, C.c_void_p, dbl, i32]),
+    "ym_link_residuals": (i32, [C.c_void_p, P(dbl), P(dbl)]),
+    "ym_reunitarize": (i32, [C.c_void_p]),
+    "ym_su3_run_chain": (i32, [P(Su3ChainConfig), P(dbl), P(Su3ChainReport)]),
+    "ym_diagnostics": (i32, [P(dbl), i64, i64, P(ChainDiagnostics), P(dbl)]),
+    "ym_derive_chain_seed": (u64, [u64, u64]),
+    "ym_derive_su3_chain_seed": (u64, [u64, u64]),
+    "ym_timer_begin": (i32, [C.c_void_p]),
+    "ym_timer_end": (i32, [C.c_void_p, P(flt)]),
+    "ym_status_string": (C.c_char_p, [i32]),
+}
+
+
+class YmError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = int(status)
+        super().__init__(f"{where}: status {self.status} ({status_string(self.status)})")
+
+
+_lib = None
+
+
+def lib():
+    """Load libym_amd.so; raises if the HIP library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise ImportError(f"libym_amd.so not built at {path}; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def status_string(code):
+    return lib().ym_status_string(int(code)).decode()
+
+
+def check(status, where):
+    if status != 0:
+        raise YmError(status, where)

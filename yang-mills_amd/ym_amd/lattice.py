@@ -1,0 +1,288 @@
+"""Lattice handles and reference-named entry points over the C ABI."""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import ChainDiagnostics, Su3ChainConfig, Su3ChainReport, check, lib
+
+
+def _ext(extents):
+    return (C.c_int64 * 4)(*[int(v) for v in extents])
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class _Lattice:
+    """Device-resident lattice (lib.rs:235-756 Lattice4 / su3_lattice.rs:20-313)."""
+
+    GROUP = 0
+    NCOMP = 0
+
+    def __init__(self, extents, precision=32):
+        self.extents = tuple(int(v) for v in extents)
+        self.precision = int(precision)
+        h = C.c_void_p()
+        check(lib().ym_lattice_create(self.GROUP, _ext(self.extents), self.precision, C.byref(h)),
+              "ym_lattice_create")
+        self._h = h
+
+    # lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ym_lattice_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def new_identity(cls, extents, precision=32):
+        return cls(extents, precision)
+
+    # sizes ----------------------------------------------------------------
+    def site_count(self):
+        return int(np.prod(self.extents))
+
+    def link_count(self):
+        return 4 * self.site_count()
+
+    def device_bytes(self):
+        b = C.c_int64()
+        check(lib().ym_lattice_bytes(self._h, C.byref(b)), "ym_lattice_bytes")
+        return b.value
+
+    # data -----------------------------------------------------------------
+    def set_identity(self):
+        check(lib().ym_lattice_set_identity(self._h), "ym_lattice_set_identity")
+
+    def hot_start(self, seed):
+        check(lib().ym_lattice_hot_start(self._h, int(seed)), "ym_lattice_hot_start")
+
+    def upload(self, aos):
+        a = np.ascontiguousarray(aos, dtype=np.float64).reshape(-1)
+        check(lib().ym_lattice_upload(self._h, _dp(a), a.size), "ym_lattice_upload")
+
+    def download(self):
+        a = np.empty(self.link_count() * self.NCOMP, dtype=np.float64)
+        check(lib().ym_lattice_download(self._h, _dp(a), a.size), "ym_lattice_download")
+        return a
+
+    def copy_from(self, other):
+        check(lib().ym_lattice_copy(self._h, other._h), "ym_lattice_copy")
+
+    def clone(self):
+        out = type(self)(self.extents, self.precision)
+        out.copy_from(self)
+        return out
+
+    # observables ----------------------------------------------------------
+    def average_plaquette(self):
+        v = C.c_double()
+        check(lib().ym_average_plaquette(self._h, C.byref(v)), "ym_average_plaquette")
+        return v.value
+
+    def wilson_action(self, beta):
+        v = C.c_double()
+        check(lib().ym_wilson_action(self._h, float(beta), C.byref(v)), "ym_wilson_action")
+        return v.value
+
+    def staple(self, x, mu):
+        out = np.zeros(self.NCOMP)
+        check(lib().ym_staple(self._h, _ext(x), int(mu), _dp(out)), "ym_staple")
+        return out
+
+    def wilson_loops(self, rmax, tmax, plane_set=0):
+        re = np.zeros(rmax * tmax)
+        im = np.zeros(rmax * tmax)
+        check(lib().ym_wilson_loops(self._h, int(rmax), int(tmax), int(plane_set), _dp(re), _dp(im)),
+              "ym_wilson_loops")
+        return re.reshape(rmax, tmax), im.reshape(rmax, tmax)
+
+    def polyakov_loop(self, direction):
+        re, im, ab = C.c_double(), C.c_double(), C.c_double()
+        check(lib().ym_polyakov_loop(self._h, int(direction), C.byref(re), C.byref(im), C.byref(ab)),
+              "ym_polyakov_loop")
+        return complex(re.value, im.value), ab.value
+
+    def link_residuals(self):
+        u, d = C.c_double(), C.c_double()
+        check(lib().ym_link_residuals(self._h, C.byref(u), C.byref(d)), "ym_link_residuals")
+        return u.value, d.value
+
+    def reunitarize(self):
+        check(lib().ym_reunitarize(self._h), "ym_reunitarize")
+
+    # timing ---------------------------------------------------------------
+    def timer_begin(self):
+        check(lib().ym_timer_begin(self._h), "ym_timer_begin")
+
+    def timer_end(self):
+        ms = C.c_float()
+        check(lib().ym_timer_end(self._h, C.byref(ms)), "ym_timer_end")
+        return ms.value
+
+
+class Lattice4(_Lattice):
+    """SU(2) lattice (lib.rs:235)."""
+
+    GROUP = 2
+    NCOMP = 4
+
+    def max_link_unit_residual(self):
+        return self.link_residuals()[0]
+
+    def heatbath_sweeps(self, beta, key, first_sweep, n_sweeps=1, max_attempts=10_000):
+        p, u = C.c_uint64(0), C.c_uint64(0)
+        st = lib().ym_su2_heatbath_sweeps(self._h, float(beta), int(key), int(first_sweep), int(n_sweeps),
+                                          int(max_attempts), C.byref(p), C.byref(u))
+        check(st, "ym_su2_heatbath_sweeps")
+        return p.value, u.value
+
+    def overrelax_sweeps(self, n_sweeps=1):
+        check(lib().ym_su2_overrelax_sweeps(self._h, int(n_sweeps)), "ym_su2_overrelax_sweeps")
+
+
+class Su3Lattice4(_Lattice):
+    """SU(3) lattice (su3_lattice.rs:20)."""
+
+    GROUP = 3
+    NCOMP = 18
+
+    def max_link_unitarity_residual(self):
+        return self.link_residuals()[0]
+
+    def max_link_determinant_residual(self):
+        return self.link_residuals()[1]
+
+    def metropolis_sweeps(self, beta, angle, key, first_sweep, n_sweeps=1, hits=8, reunitarize=True):
+        p, a = C.c_uint64(0), C.c_uint64(0)
+        st = lib().ym_su3_metropolis_sweeps(self._h, float(beta), float(angle), int(hits), int(key),
+                                            int(first_sweep), int(n_sweeps), 1 if reunitarize else 0,
+                                            C.byref(p), C.byref(a))
+        check(st, "ym_su3_metropolis_sweeps")
+        return p.value, a.value
+
+    def metropolis_sweeps_profiled(self, beta, angle, key, first_sweep, n_sweeps, hits=8, reunitarize=True):
+        """Same sweeps with HIP events around every sub-pass launch.
+
+        Returns (proposals, accepts, total_ms, kernel_ms_sum, launches)."""
+        p, a = C.c_uint64(0), C.c_uint64(0)
+        tot, ksum, nl = C.c_float(), C.c_float(), C.c_int32()
+        st = lib().ym_su3_metropolis_sweeps_profiled(self._h, float(beta), float(angle), int(hits), int(key),
+                                                     int(first_sweep), int(n_sweeps), 1 if reunitarize else 0,
+                                                     C.byref(p), C.byref(a), C.byref(tot), C.byref(ksum),
+                                                     C.byref(nl))
+        check(st, "ym_su3_metropolis_sweeps_profiled")
+        return p.value, a.value, tot.value, ksum.value, nl.value
+
+    def timeslice_a1pp(self):
+        T = self.extents[3]
+        pl, rc = np.zeros(T), np.zeros(T)
+        check(lib().ym_su3_timeslice_a1pp(self._h, _dp(pl), _dp(rc)), "ym_su3_timeslice_a1pp")
+        return pl, rc
+
+
+# ------------------------------------------------------------ free functions
+def metropolis_sweep(lattice, beta, proposal_angle, key, sweep, hits=8):
+    """su3_ensemble.rs:342-391 (checkerboard + Philox order): (proposals, accepts)."""
+    return lattice.metropolis_sweeps(beta, proposal_angle, key, sweep, 1, hits)
+
+
+def heatbath_sweep(lattice, beta, key, sweep, max_attempts_per_link=10_000):
+    """su2_ensemble.rs:321-342: (proposals, updates)."""
+    return lattice.heatbath_sweeps(beta, key, sweep, 1, max_attempts_per_link)
+
+
+def overrelaxation_sweep(lattice):
+    """su2_ensemble.rs:344-346."""
+    lattice.overrelax_sweeps(1)
+
+
+def average_plaquette(lattice):
+    """su2_ensemble.rs:348-373 / su3_ensemble.rs:228-259."""
+    return lattice.average_plaquette()
+
+
+def spatial_stout_smear(lattice, rho, levels):
+    """glueball_smearing.rs:257-268: returns a new lattice."""
+    out = Su3Lattice4(lattice.extents, lattice.precision)
+    check(lib().ym_su3_stout_spatial(out.handle, lattice.handle, float(rho), int(levels)), "ym_su3_stout_spatial")
+    return out
+
+
+def scalar_a1pp_operator_basis(lattice, time_slice):
+    """glueball_operators.rs:143-160 for one slice: (plaquette_1x1, rectangle_2x1)."""
+    pl, rc = lattice.timeslice_a1pp()
+    return pl[time_slice], rc[time_slice]
+
+
+def derive_chain_seed(master_seed, chain_id):
+    return lib().ym_derive_chain_seed(int(master_seed), int(chain_id))
+
+
+def derive_su3_chain_seed(master_seed, chain_id):
+    return lib().ym_derive_su3_chain_seed(int(master_seed), int(chain_id))
+
+
+def diagnostics(values, max_lag):
+    """su2_ensemble.rs:375-463: (ChainDiagnostics, autocorrelation array)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    ac = np.zeros(int(max_lag) + 1)
+    d = ChainDiagnostics()
+    check(lib().ym_diagnostics(_dp(v), v.size, int(max_lag), C.byref(d), _dp(ac)), "ym_diagnostics")
+    return d, ac
+
+
+@dataclass
+class ChainReport:
+    derived_seed: int
+    total_sweeps: int
+    proposals: int
+    accepts: int
+    mean_acceptance: float
+    measurements: np.ndarray
+    diagnostics: ChainDiagnostics
+    final_unitarity_residual: float
+    final_determinant_residual: float
+    extra: dict = field(default_factory=dict)
+
+
+def run_chain(extents, beta, master_seed, chain_id, thermalization_sweeps, measurements,
+              sweeps_between_measurements, proposal_angle, autocorrelation_max_lag, hits=8,
+              precision=32, hot_start=False, reunitarize=True):
+    """su3_ensemble.rs:408-480 on the device."""
+    cfg = Su3ChainConfig()
+    cfg.extents[:] = [int(v) for v in extents]
+    cfg.beta = float(beta)
+    cfg.master_seed = int(master_seed)
+    cfg.chain_id = int(chain_id)
+    cfg.thermalization_sweeps = int(thermalization_sweeps)
+    cfg.measurements = int(measurements)
+    cfg.sweeps_between_measurements = int(sweeps_between_measurements)
+    cfg.proposal_angle = float(proposal_angle)
+    cfg.autocorrelation_max_lag = int(autocorrelation_max_lag)
+    cfg.hits = int(hits)
+    cfg.precision = int(precision)
+    cfg.hot_start = 1 if hot_start else 0
+    cfg.reunitarize = 1 if reunitarize else 0
+    meas = np.zeros(int(measurements))
+    rep = Su3ChainReport()
+    check(lib().ym_su3_run_chain(C.byref(cfg), _dp(meas), C.byref(rep)), "ym_su3_run_chain")
+    return ChainReport(rep.derived_seed, rep.total_sweeps, rep.proposals, rep.accepts, rep.mean_acceptance,
+                       meas, rep.diagnostics, rep.final_unitarity_residual, rep.final_determinant_residual)
