@@ -14,11 +14,9 @@ def pytest_configure(config):
 
 
 def gpu_available():
-    try:
-        import torch  # noqa: F401  (device counting only, no HIP init here)
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return False
+    """True when a HIP device node is visible (no HIP runtime is initialised)."""
+    return os.path.exists("/dev/kfd") and any(n.startswith("renderD") for n in os.listdir("/dev/dri")) \
+        if os.path.isdir("/dev/dri") else False
 
 
 @pytest.fixture(scope="session")
