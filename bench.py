@@ -6,13 +6,15 @@ launches) of single-generator Metropolis over an SU(3) Wilson-action
 lattice, 8 proposals per link (the reference sweep's proposal set,
 su3_ensemble.rs:342-391), links stored FP32, accept test and
 reunitarisation FP64.  Default workload: 32^4, beta = 6.0, proposal angle
-0.35, Haar-random hot start; the links stay resident in HBM for the whole
-timed region.
+0.35, cold start as the reference's run_chain (su3_ensemble.rs:412) followed
+by the warm-up sweeps; the links stay resident in HBM for the whole timed
+region.
 
 Prints one JSON line with the throughput (link updates/s), the roofline of
-the dominant kernel (k_su3_metropolis: algorithmic bytes per launch / mean
-launch duration from HIP events on its stream) and the CPU baseline (the
-reference-exact restatement of metropolis_sweep timed on the host cores).
+the dominant kernel (k_su3_metropolis, one launch per checkerboard sub-pass:
+algorithmic bytes per launch / mean launch duration from HIP events on its
+stream) and the CPU baseline (the reference-exact restatement of
+metropolis_sweep timed on the host cores).
 """
 import argparse
 import ctypes as C
@@ -99,6 +101,11 @@ def main():
     ap.add_argument("--beta", type=float, default=6.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--hot-start", action="store_true",
+                    help="Haar-random start instead of the reference's cold (identity) start")
+    ap.add_argument("--reunit-calibration", action="store_true",
+                    help="after the timed region run one k_su3_reunit pass (reads and writes every "
+                         "link once) to calibrate HBM byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -114,9 +121,11 @@ def main():
     V = L ** 4
     angle = 0.35
     key = ym_amd.derive_su3_chain_seed(0x2C02198105010002, 0)
-    log(f"SU(3) {L}^4 FP32 hot start, beta={args.beta}, hits={args.hits}")
+    start = "hot" if args.hot_start else "cold"
+    log(f"SU(3) {L}^4 FP32 {start} start, beta={args.beta}, hits={args.hits}")
     lat = ym_amd.Su3Lattice4(n, 32)
-    lat.hot_start(key)
+    if args.hot_start:
+        lat.hot_start(key)
     log(f"warmup {args.warmup} sweeps")
     lat.metropolis_sweeps(args.beta, angle, key, 0, args.warmup, args.hits)
     log(f"timed {args.steps} sweeps")
@@ -124,6 +133,9 @@ def main():
     prop, acc, total_ms, ksum_ms, launches = lat.metropolis_sweeps_profiled(
         args.beta, angle, key, args.warmup, args.steps, args.hits)
     wall = time.perf_counter() - t0
+    if args.reunit_calibration:
+        log(f"calibration: k_su3_reunit over {lat.device_bytes()} bytes")
+        lat.reunitarize()
     plaq = lat.average_plaquette()
     unit_res, det_res = lat.link_residuals()
     updates = 4 * V * args.steps
@@ -131,7 +143,8 @@ def main():
     avg_launch_s = ksum_ms / launches / 1e3
     bytes_per_launch = (V // 2) * BYTES_PER_UPDATE_FP32
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    log(f"{value:.4e} link updates/s, kernel avg {avg_launch_s * 1e6:.1f} us, {achieved:.0f} GB/s algorithmic")
+    log(f"{value:.4e} link updates/s, k_su3_metropolis mean {avg_launch_s * 1e6:.1f} us, "
+        f"{achieved:.0f} GB/s algorithmic")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -150,7 +163,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: Haar-random hot start from Philox4x32-10, seed derive_su3_chain_seed(0x2c02198105010002, 0)",
+        "data": ("synthetic: Haar-random hot start from Philox4x32-10" if args.hot_start else
+                 "synthetic: cold (identity) start as the reference's run_chain") +
+                ", key derive_su3_chain_seed(0x2c02198105010002, 0)",
         "config": {
             "workload": f"SU(3) Wilson action {L}^4, beta={args.beta}, single-generator Metropolis, "
                         f"{args.hits} proposals/link/sweep, checkerboard sweeps",

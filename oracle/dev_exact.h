@@ -15,8 +15,9 @@
  *    (row * N3p + x3) >> 1 with N3p = N3 rounded up to even;
  *  - SU(3): FP32 or FP64 staple with a fixed FMA chain, M = U*A in FP64,
  *    closed-form single-generator rotations, Delta S = -(beta/3) Re Tr((R-I)M),
- *    accept iff Delta S <= 0 or u < exp(-Delta S), FP64 Gram-Schmidt
- *    reunitarisation before the store;
+ *    accept iff Delta S <= 0 or u < exp(-Delta S), U <- R U and M <- R M
+ *    with R the identity rotation (c, s) = (1, 0) when rejected, FP64
+ *    Gram-Schmidt reunitarisation before the store;
  *  - SU(2): the reference heat-bath algorithm (truncated-exponential inverse
  *    CDF + sqrt(1-r0^2) rejection) and over-relaxation on FP32/FP64 storage;
  *  - transcendental functions from named published methods (Cody-Waite

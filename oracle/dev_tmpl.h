@@ -318,20 +318,22 @@ static void F(su3_update_link)(REAL *L, const geom *g, const int64_t x[4], int m
         }
         double dS = nbt * dre;
         int accept = (dS <= 0.0) || (uacc < ymd_exp_neg(dS));
-        if (!accept) continue;
-        *acc += 1;
-        double c = 1.0 + cm1;
+        *acc += accept ? 1 : 0;
+        /* branch-free: a rejected proposal applies (c, s) = (1, 0) */
+        double c = accept ? 1.0 + cm1 : 1.0;
+        double sa = accept ? s : 0.0;
         if (gen == 7) {
-            double c2 = 1.0 + cm2;
-            rot_phase(u, 0, c, s); rot_phase(u, 1, c, s); rot_phase(u, 2, c2, s2);
-            rot_phase(M, 0, c, s); rot_phase(M, 1, c, s); rot_phase(M, 2, c2, s2);
+            double c2 = accept ? 1.0 + cm2 : 1.0;
+            double s2a = accept ? s2 : 0.0;
+            rot_phase(u, 0, c, sa); rot_phase(u, 1, c, sa); rot_phase(u, 2, c2, s2a);
+            rot_phase(M, 0, c, sa); rot_phase(M, 1, c, sa); rot_phase(M, 2, c2, s2a);
         } else if (gen == 2) {
-            rot_phase(u, 0, c, s); rot_phase(u, 1, c, -s);
-            rot_phase(M, 0, c, s); rot_phase(M, 1, c, -s);
+            rot_phase(u, 0, c, sa); rot_phase(u, 1, c, -sa);
+            rot_phase(M, 0, c, sa); rot_phase(M, 1, c, -sa);
         } else {
             int i = blk[gen][0], j = blk[gen][1];
-            if (gen == 0 || gen == 3 || gen == 5) { rot_sym(u, i, j, c, s); rot_sym(M, i, j, c, s); }
-            else { rot_asym(u, i, j, c, s); rot_asym(M, i, j, c, s); }
+            if (gen == 0 || gen == 3 || gen == 5) { rot_sym(u, i, j, c, sa); rot_sym(M, i, j, c, sa); }
+            else { rot_asym(u, i, j, c, sa); rot_asym(M, i, j, c, sa); }
         }
     }
     if (P->reunitarize) reunitarize3(u);

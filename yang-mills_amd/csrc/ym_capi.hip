@@ -167,13 +167,11 @@ int32_t ym_lattice_create(int32_t group, const int64_t extents[4], int32_t preci
     if (!l) return YM_ALLOCATION_FAILED;
     l->group = group;
     l->prec = precision;
-    for (int d = 0; d < 4; ++d) { l->n[d] = extents[d]; l->g.n[d] = (int)extents[d]; }
-    l->g.n3p = (int)(extents[3] + (extents[3] & 1));
-    l->g.half = l->g.n3p / 2;
-    l->g.vh = (int)(extents[0] * extents[1] * extents[2] * l->g.half);
-    l->g.volume = (int)vol;
+    for (int d = 0; d < 4; ++d) l->n[d] = extents[d];
+    l->g = ym::make_geom(extents);
     l->ncomp = group == 3 ? 18 : 4;
-    l->link_bytes = (size_t)8 * l->ncomp * l->g.vh * (precision / 8);
+    l->link_bytes = group == 3 ? ym::su3_pairs(l->g.vh) * 2 * (precision / 8)
+                               : (size_t)8 * 4 * l->g.vh * (precision / 8);
     int32_t st = YM_OK;
     if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess) st = YM_HIP_ERROR;
     if (!st && hipMalloc(&l->links, l->link_bytes) != hipSuccess) st = YM_LATTICE_TOO_LARGE;

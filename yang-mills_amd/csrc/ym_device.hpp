@@ -182,7 +182,40 @@ struct Geom {
     int half;  // N3p / 2: slots per row and parity
     int vh;    // slots per (mu, parity) plane
     int volume;
+    int tk;    // 256-slot tiles per (x0, x1) pair, 0 when N2 * half is not a multiple of 256
+    int t1;    // x1 strip width of the tile visiting order, 0 = plain order
 };
+
+// Geometry of an N0 x N1 x N2 x N3 lattice (extents already validated).
+__host__ inline Geom make_geom(const int64_t n[4]) {
+    Geom g{};
+    for (int d = 0; d < 4; ++d) g.n[d] = (int)n[d];
+    g.n3p = g.n[3] + (g.n[3] & 1);
+    g.half = g.n3p / 2;
+    g.vh = g.n[0] * g.n[1] * g.n[2] * g.half;
+    g.volume = g.n[0] * g.n[1] * g.n[2] * g.n[3];
+    const int row_slots = g.n[2] * g.half;
+    g.tk = (row_slots % 256 == 0) ? row_slots / 256 : 0;
+    g.t1 = (g.tk > 0 && g.n[1] % 4 == 0) ? 4 : 0;
+    return g;
+}
+
+// Order in which the blocks of a sub-pass visit the 256-slot tiles.  In slot
+// order the x0-neighbour of a tile lies N1 * N2 * half / 256 tiles away, far
+// beyond what one XCD's L2 still holds when that tile is reached.  When every
+// (x0, x1) pair owns whole tiles, tiles are visited in strips of t1 values of
+// x1, with x0 running inside a strip: x0- and x1-neighbours are then at most
+// t1 * tk tiles apart.  A permutation of independent tiles: results do not
+// depend on it.
+__host__ __device__ __forceinline__ int tile_order(const Geom &g, int t) {
+    if (g.t1 == 0) return t;
+    const int j = t % g.tk;
+    int r = t / g.tk;
+    const int x1i = r % g.t1;
+    r /= g.t1;
+    const int x0 = r % g.n[0], x1t = r / g.n[0];
+    return (x0 * g.n[1] + x1t * g.t1 + x1i) * g.tk + j;
+}
 
 // Site of slot s in parity p; returns false for the padding slot of odd N3.
 __device__ __forceinline__ bool site_of(const Geom &g, int p, int s, int x[4]) {
@@ -234,14 +267,28 @@ template <typename Real> struct M3 { Real v[18]; };  // row-major, re/im interle
 __device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// SU(3) links: for each (mu, parity) 9 planes of Vh complex pairs.
+// SU(3) links, tiled (AoSoA) layout: the Vh slots of a (mu, parity) group are
+// cut into tiles of 64 slots (one wavefront); a tile stores its 9 complex
+// entries as 9 consecutive runs of 64 (re, im) pairs, so the entry e of slot s
+// is pair ((group * ntile + (s >> 6)) * 9 + e) * 64 + (s & 63), with
+// ntile = ceil(Vh / 64).  A wavefront reading one link touches one
+// contiguous 9 * 64-pair block (4.5 KB in FP32) instead of 9 planes Vh pairs
+// apart, which keeps address translation local.
+__host__ __device__ __forceinline__ int ntile_of(int vh) { return (vh + 63) >> 6; }
+__host__ __device__ __forceinline__ size_t su3_pairs(int vh) { return (size_t)8 * ntile_of(vh) * 576; }
+
+template <typename Real>
+__device__ __forceinline__ const typename Vec2<Real>::type *link3_ptr(const typename Vec2<Real>::type *L, int vh, int mu,
+                                                                     int par, int s) {
+    return L + ((size_t)(mu * 2 + par) * ntile_of(vh) + (size_t)(s >> 6)) * 576 + (s & 63);
+}
 template <typename Real>
 __device__ __forceinline__ M3<Real> load3(const typename Vec2<Real>::type *__restrict__ L, int vh, int mu, int par, int s) {
     M3<Real> m;
-    const typename Vec2<Real>::type *base = L + (size_t)(mu * 2 + par) * 9 * vh + s;
+    const typename Vec2<Real>::type *base = link3_ptr<Real>(L, vh, mu, par, s);
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
-        typename Vec2<Real>::type t = base[(size_t)e * vh];
+        typename Vec2<Real>::type t = base[e * 64];
         m.v[2 * e] = t.x;
         m.v[2 * e + 1] = t.y;
     }
@@ -249,13 +296,13 @@ __device__ __forceinline__ M3<Real> load3(const typename Vec2<Real>::type *__res
 }
 template <typename Real>
 __device__ __forceinline__ void store3(typename Vec2<Real>::type *__restrict__ L, int vh, int mu, int par, int s, const M3<Real> &m) {
-    typename Vec2<Real>::type *base = L + (size_t)(mu * 2 + par) * 9 * vh + s;
+    typename Vec2<Real>::type *base = const_cast<typename Vec2<Real>::type *>(link3_ptr<Real>(L, vh, mu, par, s));
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
         typename Vec2<Real>::type t;
         t.x = m.v[2 * e];
         t.y = m.v[2 * e + 1];
-        base[(size_t)e * vh] = t;
+        base[e * 64] = t;
     }
 }
 
