@@ -59,9 +59,11 @@ enum {
     YM_NON_FINITE_RHO = 60,
     YM_RHO_OUT_OF_RANGE = 61,
     YM_TOO_MANY_LEVELS = 62,
+    YM_NON_FINITE_OBSERVABLE = 54,    /* GlueballCorrelatorError::NonFiniteObservable */
     YM_EMPTY_CONFIGURATION_SET = 70,
     YM_TOO_MANY_CONFIGURATIONS = 71,
     YM_TIME_EXTENT_TOO_LARGE = 72,
+    YM_NON_FINITE_ACCUMULATOR = 74,   /* GlueballCorrelatorError::NonFiniteAccumulator */
     YM_CORRELATOR_SHAPE = 75,
     YM_EFF_INVALID_LENGTH = 80,
     YM_EFF_INVALID_LAG = 81,
@@ -70,6 +72,8 @@ enum {
     YM_EFF_INVALID_DIAGNOSTIC = 84,
     YM_EFF_DEGENERATE = 85,
     YM_EFF_NON_FINITE = 86,
+    YM_WRONG_CONFIGURATION = 90,      /* benchmark gate: schedule does not allow the split */
+    YM_STATIONARITY_FAILED = 91,      /* benchmark gate: constant half or zero spread */
     YM_INVALID_ARGUMENT = 96,         /* null pointer, wrong group/precision, size mismatch */
     YM_NO_DEVICE = 97,
     YM_HIP_ERROR = 98,
@@ -213,6 +217,57 @@ int32_t ym_su3_run_chain(const ym_su3_chain_config *config, double *measurements
 /* ------------------------------------------------- host FP64 statistics */
 /* su2_ensemble.rs:375-463 diagnostics.  autocorr (optional) gets max_lag+1. */
 int32_t ym_diagnostics(const double *values, int64_t n, int64_t max_lag, ym_chain_diagnostics *out, double *autocorr);
+
+typedef struct {
+    ym_chain_diagnostics diagnostics;
+    double reweighting_efficiency, weight_only_ess, combined_ess_proxy;
+    double total_cost, sampling_cost_per_ess, total_cost_per_ess;
+} ym_efficiency_report;
+
+/* sampling_efficiency.rs:57-135 evaluate: 2 <= n <= 4096, 1 <= max_lag <
+ * min(n, 257); log_weights NULL = unit weights (Kish efficiency 1); costs
+ * finite and >= 0 with sampling > 0 (seconds, or any cost unit). */
+int32_t ym_evaluate(const double *values, int64_t n, const double *log_weights, int64_t max_lag, double setup_cost,
+                    double thermalization_cost, double sampling_cost, ym_efficiency_report *out);
+
+typedef struct {
+    double first_half_mean, second_half_mean, first_half_se, second_half_se, drift_z;
+} ym_stationarity_report;
+
+/* su3_benchmark.rs:186-237 / su2_benchmark.rs:188-245 stationarity gate:
+ * n even, half_lag < n/2; z = |mean_1 - mean_2| / sqrt(se_1^2 + se_2^2). */
+int32_t ym_stationarity(const double *values, int64_t n, int64_t half_lag, ym_stationarity_report *out);
+
+typedef struct {
+    double pooled_mean, pooled_se, aggregate_ess;
+    double max_pairwise_z, max_stationarity_z, min_ess;
+} ym_pool_summary;
+
+/* su3_confirmation.rs:105-263 (k chains) and su3_benchmark.rs:239-362 /
+ * su2_benchmark.rs:247-362 (k = 2): `series` holds k chains of n values
+ * each.  Pooled mean = mean of chain means, pooled SE = sqrt(sum SE^2) / k,
+ * aggregate ESS = sum of ESS, max pairwise z.  half_lag > 0 also applies the
+ * stationarity gate per chain.  Per-chain arrays (k entries each) may be
+ * NULL. */
+int32_t ym_pool_chains(const double *series, int64_t k, int64_t n, int64_t max_lag, int64_t half_lag,
+                       double *chain_mean, double *chain_se, double *chain_ess, double *chain_drift_z,
+                       ym_pool_summary *out);
+
+/* glueball_correlators.rs:111-172 periodic connected estimator over
+ * nconf x T samples (row-major, one row per configuration):
+ * raw(tau) = sum_c sum_t O_c(t) O_c(t+tau) / (nconf T), conn = raw - mean^2. */
+int32_t ym_periodic_connected(const double *samples, int64_t nconf, int64_t T, double *mean, double *raw,
+                              double *conn);
+/* glueball_scalar_production.rs:383-443 connected_from_samples; omitted < 0
+ * keeps every configuration, otherwise that configuration is left out. */
+int32_t ym_connected_correlator(const double *samples, int64_t nconf, int64_t T, int64_t omitted, double *conn);
+/* glueball_scalar_production.rs:445-470 correlator_evidence (full estimate
+ * plus the nconf delete-one replicas) and the jackknife error of
+ * glueball_effective_mass.rs:161-215, sqrt((N-1)/N sum_i (C_i - C_bar)^2),
+ * per lag.  replicas (nconf x T) may be NULL. */
+int32_t ym_jackknife_correlator(const double *samples, int64_t nconf, int64_t T, double *full, double *replicas,
+                                double *jackknife_mean, double *jackknife_error);
+
 /* su2_ensemble.rs:46-55 mix64 / derive_chain_seed and su3_ensemble.rs:189-192. */
 uint64_t ym_derive_chain_seed(uint64_t master_seed, uint64_t chain_id);
 uint64_t ym_derive_su3_chain_seed(uint64_t master_seed, uint64_t chain_id);

@@ -131,6 +131,21 @@ const char *ym_status_string(int32_t s) {
     case YM_NON_FINITE_RHO: return "stout rho must be finite";
     case YM_RHO_OUT_OF_RANGE: return "stout rho must satisfy 0 < rho <= 0.25";
     case YM_TOO_MANY_LEVELS: return "stout level count exceeds the limit";
+    case YM_NON_FINITE_OBSERVABLE: return "operator sample is non-finite";
+    case YM_EMPTY_CONFIGURATION_SET: return "configuration set is empty";
+    case YM_TOO_MANY_CONFIGURATIONS: return "too many configurations";
+    case YM_TIME_EXTENT_TOO_LARGE: return "time extent exceeds the limit";
+    case YM_NON_FINITE_ACCUMULATOR: return "correlator accumulator became non-finite";
+    case YM_CORRELATOR_SHAPE: return "configuration rows must be non-empty and of one length";
+    case YM_EFF_INVALID_LENGTH: return "series length must be in 2..=4096";
+    case YM_EFF_INVALID_LAG: return "maximum lag must be in 1..min(n, 257)";
+    case YM_EFF_INVALID_WEIGHT: return "log weights must be finite";
+    case YM_EFF_INVALID_COST: return "costs must be finite, non-negative, with positive sampling cost";
+    case YM_EFF_INVALID_DIAGNOSTIC: return "chain diagnostics rejected the series";
+    case YM_EFF_DEGENERATE: return "series is constant";
+    case YM_EFF_NON_FINITE: return "efficiency report is non-finite or non-positive";
+    case YM_WRONG_CONFIGURATION: return "schedule does not admit the half split";
+    case YM_STATIONARITY_FAILED: return "stationarity gate failed (constant half or zero spread)";
     case YM_INVALID_ARGUMENT: return "invalid argument";
     case YM_NO_DEVICE: return "no HIP device";
     case YM_HIP_ERROR: return "HIP runtime error";

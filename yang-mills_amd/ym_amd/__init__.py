@@ -13,6 +13,7 @@ on a machine without the built library raises, and device calls without a
 GPU return YM_NO_DEVICE as a ``YmError``.
 """
 from ._lib import YmError, lib, library_path, status_string  # noqa: F401
+from . import stats  # noqa: F401
 from .lattice import (  # noqa: F401
     Lattice4,
     Su3Lattice4,

@@ -33,6 +33,22 @@ class Su3ChainReport(C.Structure):
                 ("final_determinant_residual", dbl), ("diagnostics", ChainDiagnostics)]
 
 
+class EfficiencyReport(C.Structure):
+    _fields_ = [("diagnostics", ChainDiagnostics), ("reweighting_efficiency", dbl), ("weight_only_ess", dbl),
+                ("combined_ess_proxy", dbl), ("total_cost", dbl), ("sampling_cost_per_ess", dbl),
+                ("total_cost_per_ess", dbl)]
+
+
+class StationarityReport(C.Structure):
+    _fields_ = [("first_half_mean", dbl), ("second_half_mean", dbl), ("first_half_se", dbl),
+                ("second_half_se", dbl), ("drift_z", dbl)]
+
+
+class PoolSummary(C.Structure):
+    _fields_ = [("pooled_mean", dbl), ("pooled_se", dbl), ("aggregate_ess", dbl), ("max_pairwise_z", dbl),
+                ("max_stationarity_z", dbl), ("min_ess", dbl)]
+
+
 # name -> (restype, argtypes); the exported symbol set of include/ym_amd.h
 SIGNATURES = {
     "ym_lattice_create": (i32, [i32, P(i64), i32, P(C.c_void_p)]),
@@ -59,6 +75,12 @@ SIGNATURES = {
     "ym_reunitarize": (i32, [C.c_void_p]),
     "ym_su3_run_chain": (i32, [P(Su3ChainConfig), P(dbl), P(Su3ChainReport)]),
     "ym_diagnostics": (i32, [P(dbl), i64, i64, P(ChainDiagnostics), P(dbl)]),
+    "ym_evaluate": (i32, [P(dbl), i64, P(dbl), i64, dbl, dbl, dbl, P(EfficiencyReport)]),
+    "ym_stationarity": (i32, [P(dbl), i64, i64, P(StationarityReport)]),
+    "ym_pool_chains": (i32, [P(dbl), i64, i64, i64, i64, P(dbl), P(dbl), P(dbl), P(dbl), P(PoolSummary)]),
+    "ym_periodic_connected": (i32, [P(dbl), i64, i64, P(dbl), P(dbl), P(dbl)]),
+    "ym_connected_correlator": (i32, [P(dbl), i64, i64, i64, P(dbl)]),
+    "ym_jackknife_correlator": (i32, [P(dbl), i64, i64, P(dbl), P(dbl), P(dbl), P(dbl)]),
     "ym_derive_chain_seed": (u64, [u64, u64]),
     "ym_derive_su3_chain_seed": (u64, [u64, u64]),
     "ym_timer_begin": (i32, [C.c_void_p]),
