@@ -214,6 +214,52 @@ typedef struct {
  * config->measurements doubles. */
 int32_t ym_su3_run_chain(const ym_su3_chain_config *config, double *measurements_out, ym_su3_chain_report *report);
 
+/* ------------------------------------------------------ batched ensembles */
+/* Independent SU(3) chains of uneven geometries in one device buffer; every
+ * sweep and measurement is one batched launch over all chains.  Reference
+ * counterpart: generate_independent_configurations
+ * (glueball_scalar_production.rs:257-297), which runs the chains one after
+ * another with key derive_su3_chain_seed(MASTER_SEED, chain_id), a cold
+ * start (su3_ensemble.rs:412) and metropolis_sweep (su3_ensemble.rs:342);
+ * chain c of an ensemble reproduces bit for bit a single lattice created
+ * with the same extents and precision and swept with keys[c]. */
+typedef struct ym_ensemble ym_ensemble;
+
+/* extents: nchains x 4 (all even, >= 2); keys: nchains Philox keys
+ * (normally derive_su3_chain_seed(master, chain_id)).  Cold start, sweep
+ * index 0. */
+int32_t ym_ensemble_create(int32_t nchains, const int64_t *extents, const uint64_t *keys, int32_t precision,
+                           ym_ensemble **out);
+int32_t ym_ensemble_destroy(ym_ensemble *ens);
+/* Cold start of every chain (su3_ensemble.rs:412); resets the sweep index. */
+int32_t ym_ensemble_set_identity(ym_ensemble *ens);
+/* Haar-random start of every chain from its key (no reference counterpart). */
+int32_t ym_ensemble_hot_start(ym_ensemble *ens);
+int32_t ym_ensemble_chain_count(const ym_ensemble *ens, int32_t *nchains, int64_t *total_sites, int64_t *time_sum);
+/* Reference AoS FP64 layout of one chain (volume x 4 x 18 doubles). */
+int32_t ym_ensemble_upload_chain(ym_ensemble *ens, int32_t chain, const double *aos, int64_t count);
+int32_t ym_ensemble_download_chain(const ym_ensemble *ens, int32_t chain, double *aos, int64_t count);
+int32_t ym_ensemble_sweep_index(const ym_ensemble *ens, uint64_t *sweep);
+/* Length of one cycle's operator block for `stout_levels` and each chain's
+ * timeslice offset t_off[c]: chain c, level l, timeslice t, kind k (0 1x1
+ * plaquettes, 1 2x1 rectangles) sits at ((l * time_sum + t_off[c] + t) * 2 + k). */
+int32_t ym_ensemble_ops_length(const ym_ensemble *ens, int32_t stout_levels, int64_t *ops_len, int64_t *chain_offsets);
+/* n_cycles x [sweeps_per_cycle checkerboard Metropolis sweeps of every chain
+ * (sweep indices continue from the ensemble's counter), then, when plaquette
+ * is not NULL, one measurement: average_plaquette per chain
+ * (su3_ensemble.rs:228-259) into plaquette[cycle * nchains + c] and the A1++
+ * timeslice basis (glueball_operators.rs:93-160) of the raw links and of
+ * stout levels 1..stout_levels (glueball_smearing.rs:137-268, rho) into
+ * ops[cycle * ops_len + ...]].  use_graph != 0 captures one cycle as a
+ * hipGraph and replays it; results are identical either way.  accepts
+ * (nchains, may be NULL) is accumulated; device_ms (may be NULL) is the
+ * device time of all cycles.  Bounds as ym_su3_metropolis_sweeps and
+ * ym_su3_stout_spatial (cubic spatial extents >= 3 when measuring). */
+int32_t ym_ensemble_run_cycles(ym_ensemble *ens, double beta, double proposal_angle, int32_t hits,
+                               int32_t reunitarize, int32_t sweeps_per_cycle, int32_t stout_levels, double rho,
+                               int32_t n_cycles, int32_t use_graph, double *plaquette, double *ops,
+                               uint64_t *accepts, float *device_ms);
+
 /* ------------------------------------------------- host FP64 statistics */
 /* su2_ensemble.rs:375-463 diagnostics.  autocorr (optional) gets max_lag+1. */
 int32_t ym_diagnostics(const double *values, int64_t n, int64_t max_lag, ym_chain_diagnostics *out, double *autocorr);

@@ -1,0 +1,217 @@
+// Device functions of the gauge observables (group traits, plaquette site
+// sum, rectangular loops, A1++ site sums, stout step), shared by the
+// single-lattice and the batched-ensemble kernels.
+#pragma once
+
+#include "ym_device.hpp"
+
+namespace ym {
+
+template <typename Real> struct G3 {
+    using Mat = M3<Real>;
+    using Store = typename Vec2<Real>::type;
+    static constexpr int N = 3;
+    __device__ static Mat load(const Store *L, const Geom &g, const int x[4], int mu) {
+        return load3<Real>(L, g.vh, mu, parity_of(x), slot_of(g, x));
+    }
+    __device__ static Mat mul(const Mat &a, const Mat &b) { return mul_nn(a, b); }
+    __device__ static Mat dag(const Mat &a) { return dagger3(a); }
+    __device__ static Mat id() { return identity3<Real>(); }
+    __device__ static double retr(const Mat &a) {
+        return ((double)a.v[0] + (double)a.v[8] + (double)a.v[16]) / 3.0;
+    }
+    __device__ static double imtr(const Mat &a) {
+        return ((double)a.v[1] + (double)a.v[9] + (double)a.v[17]) / 3.0;
+    }
+    // Re Tr(a b^dagger) / N
+    __device__ static double retr_nd(const Mat &a, const Mat &b) {
+        double t = 0.0;
+#pragma unroll
+        for (int c = 0; c < 18; ++c) t = __builtin_fma((double)a.v[c], (double)b.v[c], t);
+        return t / 3.0;
+    }
+};
+template <typename Real> struct G2 {
+    using Mat = Q4<Real>;
+    using Store = typename Vec4<Real>::type;
+    static constexpr int N = 2;
+    __device__ static Mat load(const Store *L, const Geom &g, const int x[4], int mu) {
+        return load2<Real>(L, g.vh, mu, parity_of(x), slot_of(g, x));
+    }
+    __device__ static Mat mul(const Mat &a, const Mat &b) { return qmul(a, b); }
+    __device__ static Mat dag(const Mat &a) { return qdag(a); }
+    __device__ static Mat id() {
+        Mat m;
+        m.c[0] = 1; m.c[1] = 0; m.c[2] = 0; m.c[3] = 0;
+        return m;
+    }
+    __device__ static double retr(const Mat &a) { return (double)a.c[0]; }
+    __device__ static double imtr(const Mat &) { return 0.0; }
+    // (1/2) Re Tr(a b^dagger) = a . b for unit quaternions
+    __device__ static double retr_nd(const Mat &a, const Mat &b) {
+        double t = (double)a.c[0] * (double)b.c[0];
+        t = __builtin_fma((double)a.c[1], (double)b.c[1], t);
+        t = __builtin_fma((double)a.c[2], (double)b.c[2], t);
+        t = __builtin_fma((double)a.c[3], (double)b.c[3], t);
+        return t;
+    }
+};
+
+// Sum over mu < nu of (1/N) Re Tr P_{mu nu}(x) for the site of parity p at
+// slot s (0 for the padding slot of odd N3), with
+// P = [U_mu(x) U_nu(x+mu)] [U_nu(x) U_mu(x+nu)]^dagger.
+template <class G>
+__device__ __forceinline__ double plaquette_site(const typename G::Store *__restrict__ L, const Geom &g, int p,
+                                                 int s) {
+    double acc = 0.0;
+    int x[4];
+    if (!site_of(g, p, s, x)) return acc;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        int xm[4];
+        shifted(g, x, mu, +1, xm);
+        const typename G::Mat Umu = G::load(L, g, x, mu);
+#pragma unroll
+        for (int nu = mu + 1; nu < 4; ++nu) {
+            int xn[4];
+            shifted(g, x, nu, +1, xn);
+            const typename G::Mat a = G::mul(Umu, G::load(L, g, xm, nu));
+            const typename G::Mat b = G::mul(G::load(L, g, x, nu), G::load(L, g, xn, mu));
+            acc += G::retr_nd(a, b);
+        }
+    }
+    return acc;
+}
+
+template <typename Real>
+__device__ __forceinline__ double rect_loop_re(const typename Vec2<Real>::type *__restrict__ L, const Geom &g,
+                                               const int x0[4], int m, int n, int r, int t) {
+    using G = G3<Real>;
+    typename G::Mat h = G::id();
+    int c[4] = {x0[0], x0[1], x0[2], x0[3]}, y[4];
+    for (int k = 0; k < r; ++k) { h = G::mul(h, G::load(L, g, c, m)); shifted(g, c, m, +1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; }
+    for (int k = 0; k < t; ++k) { h = G::mul(h, G::load(L, g, c, n)); shifted(g, c, n, +1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; }
+    for (int k = 0; k < r; ++k) { shifted(g, c, m, -1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; h = G::mul(h, G::dag(G::load(L, g, c, m))); }
+    for (int k = 0; k < t; ++k) { shifted(g, c, n, -1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; h = G::mul(h, G::dag(G::load(L, g, c, n))); }
+    return G::retr(h);
+}
+// A1++ sums of the spatial site b (x2 fastest) at timeslice t (time
+// direction 3): the three 1x1 spatial plaquettes (0,1),(0,2),(1,2) into ps
+// and the six 2x1 rectangles (0,1),(0,2),(1,0),(1,2),(2,0),(2,1) into rs,
+// (1/3) Re Tr each (glueball_operators.rs:24-26, 93-160).
+template <typename Real>
+__device__ __forceinline__ void a1pp_site(const typename Vec2<Real>::type *__restrict__ L, const Geom &g, int b, int t,
+                                          double &ps, double &rs) {
+    int x[4];
+    x[2] = b % g.n[2];
+    x[1] = (b / g.n[2]) % g.n[1];
+    x[0] = b / (g.n[2] * g.n[1]);
+    x[3] = t;
+    ps += rect_loop_re<Real>(L, g, x, 0, 1, 1, 1);
+    ps += rect_loop_re<Real>(L, g, x, 0, 2, 1, 1);
+    ps += rect_loop_re<Real>(L, g, x, 1, 2, 1, 1);
+    rs += rect_loop_re<Real>(L, g, x, 0, 1, 2, 1);
+    rs += rect_loop_re<Real>(L, g, x, 0, 2, 2, 1);
+    rs += rect_loop_re<Real>(L, g, x, 1, 0, 2, 1);
+    rs += rect_loop_re<Real>(L, g, x, 1, 2, 2, 1);
+    rs += rect_loop_re<Real>(L, g, x, 2, 0, 2, 1);
+    rs += rect_loop_re<Real>(L, g, x, 2, 1, 2, 1);
+}
+
+// ------------------------------------------------------ spatial stout
+__device__ __forceinline__ M3<double> expm_i_hermitian(const M3<double> &Q) {
+    // X = i Q
+    M3<double> X;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) { X.v[2 * c] = -Q.v[2 * c + 1]; X.v[2 * c + 1] = Q.v[2 * c]; }
+    double norm = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        double cs = 0.0;
+        for (int i = 0; i < 3; ++i) cs += hypot(YM_RE(X, i, j), YM_IM(X, i, j));
+        norm = fmax(norm, cs);
+    }
+    int sq = 0;
+    while (norm > 0.5 && sq < 32) { norm *= 0.5; ++sq; }
+    const double scale = __builtin_ldexp(1.0, -sq);
+#pragma unroll
+    for (int c = 0; c < 18; ++c) X.v[c] *= scale;
+    M3<double> sum = identity3<double>(), term = identity3<double>();
+    for (int k = 1; k <= 64; ++k) {
+        term = mul_nn(term, X);
+        const double inv = 1.0 / (double)k;
+        double mx = 0.0;
+#pragma unroll
+        for (int c = 0; c < 18; ++c) { term.v[c] *= inv; sum.v[c] += term.v[c]; mx = fmax(mx, fabs(term.v[c])); }
+        if (mx <= 1.0e-18) break;
+    }
+    for (int k = 0; k < sq; ++k) sum = mul_nn(sum, sum);
+    return sum;
+}
+
+// One stout step of the link (mu, parity p, slot s < vh) from `in` into `out`.
+template <typename Real>
+__device__ __forceinline__ void stout_link(const typename Vec2<Real>::type *__restrict__ in,
+                                           typename Vec2<Real>::type *__restrict__ out, const Geom &g, double rho,
+                                           int p, int s, int mu) {
+    using G = G3<Real>;
+    int x[4];
+    if (!site_of(g, p, s, x) || mu == 3) {
+        store3<Real>(out, g.vh, mu, p, s, load3<Real>(in, g.vh, mu, p, s));
+        return;
+    }
+    M3<double> C;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) C.v[c] = 0.0;
+    int xpm[4];
+    shifted(g, x, mu, +1, xpm);
+    for (int nu = 0; nu < 3; ++nu) {
+        if (nu == mu) continue;
+        int xpn[4], xmn[4], xmnpm[4];
+        shifted(g, x, nu, +1, xpn);
+        shifted(g, x, nu, -1, xmn);
+        shifted(g, xmn, mu, +1, xmnpm);
+        const typename G::Mat f = G::mul(G::mul(G::load(in, g, x, nu), G::load(in, g, xpn, mu)), G::dag(G::load(in, g, xpm, nu)));
+        const typename G::Mat b = G::mul(G::mul(G::dag(G::load(in, g, xmn, nu)), G::load(in, g, xmn, mu)), G::load(in, g, xmnpm, nu));
+#pragma unroll
+        for (int c = 0; c < 18; ++c) C.v[c] += (double)f.v[c] + (double)b.v[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 18; ++c) C.v[c] *= rho;
+    const typename G::Mat Uf = load3<Real>(in, g.vh, mu, p, s);
+    M3<double> U;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) U.v[c] = (double)Uf.v[c];
+    const M3<double> Om = mul_nn(C, dagger3(U));
+    // A = Omega^+ - Omega (anti-Hermitian); Q = (i/2) A - (i/6) Tr(A) 1
+    M3<double> Q;
+    double trr = 0.0, tri = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        trr += YM_RE(Om, i, i) - YM_RE(Om, i, i);
+        tri += -YM_IM(Om, i, i) - YM_IM(Om, i, i);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double ar = YM_RE(Om, j, i) - YM_RE(Om, i, j);
+            const double ai = -YM_IM(Om, j, i) - YM_IM(Om, i, j);
+            // (i/2)(ar + i ai) = (-ai/2) + i (ar/2)
+            double qr = -0.5 * ai, qi = 0.5 * ar;
+            if (i == j) {
+                // - (i/6)(trr + i tri) = (tri/6) - i (trr/6)
+                qr += tri / 6.0;
+                qi -= trr / 6.0;
+            }
+            YM_RE(Q, i, j) = qr;
+            YM_IM(Q, i, j) = qi;
+        }
+    const M3<double> R = expm_i_hermitian(Q);
+    const M3<double> Un = mul_nn(R, U);
+    M3<Real> o;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) o.v[c] = (Real)Un.v[c];
+    store3<Real>(out, g.vh, mu, p, s, o);
+}
+
+}  // namespace ym

@@ -1,0 +1,171 @@
+// Device functions of the SU(3) single-generator Metropolis update, shared
+// by the single-lattice and the batched-ensemble kernels.
+#pragma once
+
+#include "ym_device.hpp"
+
+namespace ym {
+
+// One single-generator proposal on the FP64 link u with M = u A (staple A):
+// generator GEN = h mod 8, angle from the reference's paired grid, accept iff
+// dS <= 0 or u < exp(-dS), then u <- R u and M <- R M, where a rejected
+// proposal applies the identity rotation (c, s) = (1, 0) so the update is
+// branch-free and leaves every entry unchanged.
+template <int GEN, int MU>
+__device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t key, uint32_t site, int h,
+                                        uint64_t sweep, double angle, double nbt, unsigned long long &nacc) {
+    constexpr int bi = (GEN <= 2 || GEN == 7) ? 0 : (GEN <= 4 ? 0 : 1);
+    constexpr int bj = (GEN <= 2 || GEN == 7) ? 1 : 2;
+    uint32_t o[4];
+    draw(key, site, MU, KIND_SU3_HIT, (uint32_t)h, sweep, o);
+    const double theta = angle_from_raw(o[0], o[1], angle);
+    const double uacc = unit53(o[2], o[3]);
+    double sn, cm1, s2 = 0.0, cm2 = 0.0, dre;
+    if constexpr (GEN == 7) {
+        const double psi = theta * 0x1.279a74590331cp-2;
+        sincos_small(psi, sn, cm1);
+        sincos_small(-2.0 * psi, s2, cm2);
+        const double t1 = YM_RE(M, 0, 0) + YM_RE(M, 1, 1), t2 = YM_IM(M, 0, 0) + YM_IM(M, 1, 1);
+        const double d1 = __builtin_fma(cm1, t1, -(sn * t2));
+        const double d2 = __builtin_fma(cm2, YM_RE(M, 2, 2), -(s2 * YM_IM(M, 2, 2)));
+        dre = d1 + d2;
+    } else if constexpr (GEN == 2) {
+        sincos_small(theta * 0.5, sn, cm1);
+        const double t1 = YM_RE(M, 0, 0) + YM_RE(M, 1, 1), t2 = YM_IM(M, 0, 0) - YM_IM(M, 1, 1);
+        dre = __builtin_fma(cm1, t1, -(sn * t2));
+    } else if constexpr (GEN == 0 || GEN == 3 || GEN == 5) {
+        sincos_small(theta * 0.5, sn, cm1);
+        const double t1 = YM_RE(M, bi, bi) + YM_RE(M, bj, bj);
+        const double t2 = YM_IM(M, bi, bj) + YM_IM(M, bj, bi);
+        dre = __builtin_fma(cm1, t1, -(sn * t2));
+    } else {
+        sincos_small(theta * 0.5, sn, cm1);
+        const double t1 = YM_RE(M, bi, bi) + YM_RE(M, bj, bj);
+        const double t2 = YM_RE(M, bj, bi) - YM_RE(M, bi, bj);
+        dre = __builtin_fma(cm1, t1, sn * t2);
+    }
+    const double dS = nbt * dre;
+    const bool accept = (dS <= 0.0) || (uacc < exp_neg(dS));
+    nacc += accept ? 1ull : 0ull;
+    const double c = accept ? 1.0 + cm1 : 1.0;
+    const double sa = accept ? sn : 0.0;
+    if constexpr (GEN == 7) {
+        const double c2 = accept ? 1.0 + cm2 : 1.0;
+        const double s2a = accept ? s2 : 0.0;
+        rot_phase(u, 0, c, sa); rot_phase(u, 1, c, sa); rot_phase(u, 2, c2, s2a);
+        rot_phase(M, 0, c, sa); rot_phase(M, 1, c, sa); rot_phase(M, 2, c2, s2a);
+    } else if constexpr (GEN == 2) {
+        rot_phase(u, 0, c, sa); rot_phase(u, 1, c, -sa);
+        rot_phase(M, 0, c, sa); rot_phase(M, 1, c, -sa);
+    } else if constexpr (GEN == 0 || GEN == 3 || GEN == 5) {
+        rot_sym(u, bi, bj, c, sa); rot_sym(M, bi, bj, c, sa);
+    } else {
+        rot_asym(u, bi, bj, c, sa); rot_asym(M, bi, bj, c, sa);
+    }
+}
+
+// Staple A(x) of the link U_MU(x), x of parity p at slot s: the six staples
+// in the storage precision with the fixed FMA chain (mul_nd/acc_nd forward,
+// mul_dd/acc_nn backward, nu ascending).
+template <typename Real, int MU>
+__device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *__restrict__ L, const Geom &g, int p,
+                                              int s, const int x[4], M3<Real> &A) {
+    const int q = p ^ 1;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) A.v[c] = 0;
+    int xpm[4];
+    shifted(g, x, MU, +1, xpm);
+    const int s_pm = slot_of(g, xpm);
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == MU) continue;
+        int xpn[4], xmn[4], xpmmn[4];
+        shifted(g, x, nu, +1, xpn);
+        shifted(g, x, nu, -1, xmn);
+        shifted(g, xpm, nu, -1, xpmmn);
+        const int s_pn = slot_of(g, xpn), s_mn = slot_of(g, xmn), s_pmmn = slot_of(g, xpmmn);
+        M3<Real> T;
+        {
+            const M3<Real> X = load3<Real>(L, g.vh, nu, q, s_pm);
+            const M3<Real> Y = load3<Real>(L, g.vh, MU, q, s_pn);
+            mul_nd(X, Y, T);
+            const M3<Real> Z = load3<Real>(L, g.vh, nu, p, s);
+            acc_nd(A, T, Z);
+        }
+        {
+            const M3<Real> X = load3<Real>(L, g.vh, nu, p, s_pmmn);
+            const M3<Real> Y = load3<Real>(L, g.vh, MU, q, s_mn);
+            mul_dd(X, Y, T);
+            const M3<Real> Z = load3<Real>(L, g.vh, nu, q, s_mn);
+            acc_nn(A, T, Z);
+        }
+    }
+}
+
+// Update of the link U_MU(x) with staple A: M = U A in FP64, `hits`
+// proposals, optional FP64 reunitarisation, one rounding to storage.
+// Returns the number of accepted proposals.
+template <typename Real, int MU>
+__device__ __forceinline__ unsigned long long su3_update_at(typename Vec2<Real>::type *__restrict__ L, const Geom &g,
+                                                            int p, int s, const int x[4], const M3<Real> &A,
+                                                            double nbt, double angle, int hits, uint64_t key,
+                                                            uint64_t sweep, int reunit) {
+    unsigned long long nacc = 0;
+    M3<double> u, M;
+    {
+        const M3<Real> Uf = load3<Real>(L, g.vh, MU, p, s);
+#pragma unroll
+        for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double re = YM_RE(u, i, 0) * (double)YM_RE(A, 0, j);
+            re = __builtin_fma(-YM_IM(u, i, 0), (double)YM_IM(A, 0, j), re);
+            re = __builtin_fma(YM_RE(u, i, 1), (double)YM_RE(A, 1, j), re);
+            re = __builtin_fma(-YM_IM(u, i, 1), (double)YM_IM(A, 1, j), re);
+            re = __builtin_fma(YM_RE(u, i, 2), (double)YM_RE(A, 2, j), re);
+            re = __builtin_fma(-YM_IM(u, i, 2), (double)YM_IM(A, 2, j), re);
+            double im = YM_RE(u, i, 0) * (double)YM_IM(A, 0, j);
+            im = __builtin_fma(YM_IM(u, i, 0), (double)YM_RE(A, 0, j), im);
+            im = __builtin_fma(YM_RE(u, i, 1), (double)YM_IM(A, 1, j), im);
+            im = __builtin_fma(YM_IM(u, i, 1), (double)YM_RE(A, 1, j), im);
+            im = __builtin_fma(YM_RE(u, i, 2), (double)YM_IM(A, 2, j), im);
+            im = __builtin_fma(YM_IM(u, i, 2), (double)YM_RE(A, 2, j), im);
+            YM_RE(M, i, j) = re;
+            YM_IM(M, i, j) = im;
+        }
+    const uint32_t site = (uint32_t)lex_of(g, x);
+    int h = 0;
+    for (; h + 8 <= hits; h += 8) {
+        su3_hit<0, MU>(u, M, key, site, h + 0, sweep, angle, nbt, nacc);
+        su3_hit<1, MU>(u, M, key, site, h + 1, sweep, angle, nbt, nacc);
+        su3_hit<2, MU>(u, M, key, site, h + 2, sweep, angle, nbt, nacc);
+        su3_hit<3, MU>(u, M, key, site, h + 3, sweep, angle, nbt, nacc);
+        su3_hit<4, MU>(u, M, key, site, h + 4, sweep, angle, nbt, nacc);
+        su3_hit<5, MU>(u, M, key, site, h + 5, sweep, angle, nbt, nacc);
+        su3_hit<6, MU>(u, M, key, site, h + 6, sweep, angle, nbt, nacc);
+        su3_hit<7, MU>(u, M, key, site, h + 7, sweep, angle, nbt, nacc);
+    }
+    for (; h < hits; ++h) {
+        switch (h & 7) {
+        case 0: su3_hit<0, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        case 1: su3_hit<1, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        case 2: su3_hit<2, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        case 3: su3_hit<3, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        case 4: su3_hit<4, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        case 5: su3_hit<5, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        case 6: su3_hit<6, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        default: su3_hit<7, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        }
+    }
+    if (reunit) reunitarize3(u);
+    M3<Real> Uo;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) Uo.v[c] = (Real)u.v[c];
+    store3<Real>(L, g.vh, MU, p, s, Uo);
+    return nacc;
+}
+
+}  // namespace ym

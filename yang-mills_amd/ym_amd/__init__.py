@@ -14,6 +14,8 @@ GPU return YM_NO_DEVICE as a ``YmError``.
 """
 from ._lib import YmError, lib, library_path, status_string  # noqa: F401
 from . import stats  # noqa: F401
+from . import ensemble  # noqa: F401
+from .ensemble import Ensemble  # noqa: F401
 from .lattice import (  # noqa: F401
     Lattice4,
     Su3Lattice4,

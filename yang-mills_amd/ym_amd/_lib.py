@@ -75,6 +75,17 @@ SIGNATURES = {
     "ym_reunitarize": (i32, [C.c_void_p]),
     "ym_su3_run_chain": (i32, [P(Su3ChainConfig), P(dbl), P(Su3ChainReport)]),
     "ym_diagnostics": (i32, [P(dbl), i64, i64, P(ChainDiagnostics), P(dbl)]),
+    "ym_ensemble_create": (i32, [i32, P(i64), P(u64), i32, P(C.c_void_p)]),
+    "ym_ensemble_destroy": (i32, [C.c_void_p]),
+    "ym_ensemble_set_identity": (i32, [C.c_void_p]),
+    "ym_ensemble_hot_start": (i32, [C.c_void_p]),
+    "ym_ensemble_chain_count": (i32, [C.c_void_p, P(i32), P(i64), P(i64)]),
+    "ym_ensemble_upload_chain": (i32, [C.c_void_p, i32, P(dbl), i64]),
+    "ym_ensemble_download_chain": (i32, [C.c_void_p, i32, P(dbl), i64]),
+    "ym_ensemble_sweep_index": (i32, [C.c_void_p, P(u64)]),
+    "ym_ensemble_ops_length": (i32, [C.c_void_p, i32, P(i64), P(i64)]),
+    "ym_ensemble_run_cycles": (i32, [C.c_void_p, dbl, dbl, i32, i32, i32, i32, dbl, i32, i32, P(dbl), P(dbl),
+                                     P(u64), P(flt)]),
     "ym_evaluate": (i32, [P(dbl), i64, P(dbl), i64, dbl, dbl, dbl, P(EfficiencyReport)]),
     "ym_stationarity": (i32, [P(dbl), i64, i64, P(StationarityReport)]),
     "ym_pool_chains": (i32, [P(dbl), i64, i64, i64, i64, P(dbl), P(dbl), P(dbl), P(dbl), P(PoolSummary)]),
