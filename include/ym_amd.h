@@ -260,6 +260,55 @@ int32_t ym_ensemble_run_cycles(ym_ensemble *ens, double beta, double proposal_an
                                int32_t n_cycles, int32_t use_graph, double *plaquette, double *ops,
                                uint64_t *accepts, float *device_ms);
 
+/* ------------------------------------------------- learned link proposal */
+/* No reference counterpart (the reference samples with plain
+ * single-generator Metropolis, su3_ensemble.rs:336-391).  An MLP
+ * (12 gauge-invariant staple features -> 64 tanh -> 64 tanh -> 16) gives the
+ * mean and log-width of a Gaussian in the exponential chart around the
+ * staple-aligned link W(A); each link gets one independence proposal with
+ * the exact Metropolis-Hastings ratio (Haar Jacobian included), followed by
+ * `metro_hits` plain Metropolis hits.  The handle owns the FP32 parameters
+ * (6032: W1[64][12], b1[64], W2[64][64], b2[64], W3[16][64], b3[16]), Adam
+ * moments and a device dataset of (features, chart coordinates) pairs. */
+typedef struct ym_mlp ym_mlp;
+int32_t ym_mlp_create(uint64_t seed, int64_t dataset_capacity, ym_mlp **out);
+int32_t ym_mlp_destroy(ym_mlp *mlp);
+int32_t ym_mlp_get_params(const ym_mlp *mlp, float *params);
+/* Also resets the Adam moments and step count. */
+int32_t ym_mlp_set_params(ym_mlp *mlp, const float *params);
+int32_t ym_mlp_dataset_size(const ym_mlp *mlp, int64_t *n, int64_t *capacity);
+int32_t ym_mlp_clear_pairs(ym_mlp *mlp);
+/* Appends one (staple features, chart coordinates, valid) row per link of an
+ * SU(3) FP32 lattice (8 Vh rows, group-major). */
+int32_t ym_mlp_collect_pairs(ym_mlp *mlp, const ym_lattice *lat, int64_t *total);
+int32_t ym_mlp_download_pairs(const ym_mlp *mlp, int64_t first, int64_t count, float *features, float *targets,
+                              int32_t *valid);
+/* Replaces the dataset with `count` host rows. */
+int32_t ym_mlp_upload_pairs(ym_mlp *mlp, const float *features, const float *targets, const int32_t *valid,
+                            int64_t count);
+/* out[n][16] = MLP(features[n][12]); bf16 != 0 rounds weights and layer
+ * inputs to bfloat16 (FP32 accumulation). */
+int32_t ym_mlp_forward(const ym_mlp *mlp, const float *features, int64_t n, int32_t bf16, float *out);
+/* Mean Gaussian negative log-likelihood of dataset rows [first, first+count)
+ * and its gradient (6032). */
+int32_t ym_mlp_loss_grad(ym_mlp *mlp, int64_t first, int64_t count, float *loss, float *grad);
+/* Adam (lr, beta1 0.9, beta2 0.999) over shuffled minibatches of the
+ * dataset; epoch_loss (epochs, may be NULL) gets the mean loss per epoch. */
+int32_t ym_mlp_train(ym_mlp *mlp, int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss,
+                     float *device_ms);
+/* n_sweeps checkerboard sweeps (SU(3), FP32 storage): per link one learned
+ * independence proposal with MH correction, then metro_hits Metropolis hits
+ * (generator h mod 8, proposal_angle), FP64 reunitarisation.  Counts are
+ * accumulated. */
+int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *mlp, double beta, uint64_t key, uint64_t first_sweep,
+                              int32_t n_sweeps, int32_t metro_hits, double proposal_angle, int32_t bf16,
+                              uint64_t *learned_proposals, uint64_t *learned_accepts, uint64_t *metro_accepts,
+                              float *device_ms);
+/* Probe of the link (x, mu): out[0..17] staple A, [18..35] frame W(A),
+ * [36..43] chart coordinates of the link, [44..46] chart phases,
+ * [47..58] features, [59] frame ok, [60] chart ok, [61] log Haar Jacobian. */
+int32_t ym_su3_link_chart(const ym_lattice *lat, const int64_t x[4], int32_t mu, double *out62);
+
 /* ------------------------------------------------- host FP64 statistics */
 /* su2_ensemble.rs:375-463 diagnostics.  autocorr (optional) gets max_lag+1. */
 int32_t ym_diagnostics(const double *values, int64_t n, int64_t max_lag, ym_chain_diagnostics *out, double *autocorr);

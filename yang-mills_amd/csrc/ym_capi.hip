@@ -57,6 +57,18 @@ struct ym_lattice {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+namespace ym {
+int32_t lattice_info(const ym_lattice *l, int32_t *group, int32_t *prec, int64_t ext[4]) {
+    if (!l) return YM_INVALID_ARGUMENT;
+    *group = l->group;
+    *prec = l->prec;
+    for (int d = 0; d < 4; ++d) ext[d] = l->n[d];
+    return YM_OK;
+}
+void *lattice_links(const ym_lattice *l) { return l->links; }
+hipStream_t lattice_stream(const ym_lattice *l) { return l->stream; }
+}  // namespace ym
+
 namespace {
 
 constexpr int64_t kMaxSites = (int64_t)1 << 30;

@@ -4,9 +4,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct ym_lattice;
+
 namespace ym {
 
 struct Geom;
+
+// Accessors of a lattice handle for the other kernel families.
+int32_t lattice_info(const ym_lattice *lat, int32_t *group, int32_t *prec, int64_t ext[4]);
+void *lattice_links(const ym_lattice *lat);
+hipStream_t lattice_stream(const ym_lattice *lat);
 
 void launch_su3_metropolis(int prec, void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
                            uint64_t key, uint64_t sweep, int reunit, unsigned long long *acc, hipStream_t st);

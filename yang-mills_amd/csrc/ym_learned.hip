@@ -1,0 +1,969 @@
+// Learned SU(3) link proposal with an exact Metropolis-Hastings correction.
+//
+// Proposal (no reference counterpart; the reference sampler is plain
+// single-generator Metropolis, su3_ensemble.rs:336-391).  For the link
+// U = U_mu(x) with staple A (Re Tr(U A) is the link's part of the Wilson
+// action, beta/3 times):
+//   * W(A) = e^{-i psi/3} polar(A^dagger), psi = arg det polar(A^dagger): the
+//     SU(3) element aligned with the staple (a gauge-covariant function of A);
+//   * chart: U = exp(i X) W with X = sum_a eps_a lambda_a / 2 Hermitian
+//     traceless, eigenvalues in (-pi, pi) (principal logarithm, su3_log);
+//   * an MLP maps 12 gauge-invariant features of A (singular values, det
+//     phase) to mu_a and log sigma_a (a = 1..8); eps' ~ N(mu, diag sigma^2),
+//     U' = exp(i X(eps')) W;
+//   * the proposal density with respect to Haar measure is
+//     q(U') = N(eps(U'); mu, sigma) / J(eps(U')), J the Haar Jacobian of the
+//     exponential map, prod_{j<k} sinc^2((x_j - x_k)/2);
+//   * accept with probability min(1, exp[(beta/3) Re Tr((U' - U) A)]
+//     q(U) / q(U')).  Because A does not depend on U, the proposal does not
+//     either (an independence proposal per link), so this is an exact MH step
+//     for the conditional distribution of U.  Proposals eps' whose X has an
+//     eigenvalue outside (-pi, pi) are rejected, which keeps eps -> U'
+//     one to one on the support of q.  Links whose chart fails (log
+//     residual, phase at +-pi) are outside the support of q and never move
+//     through this step; `metro_hits` plain Metropolis hits follow on every
+//     link, which keeps the chain ergodic.
+// The chart is taken of the FP64 Gram-Schmidt projection of the stored
+// link (FP32 storage is special unitary only to about 1e-7).
+// Random numbers: Philox(site, mu | KIND_LEARNED << 2 | k << 5, sweep) for
+// k = 0..3 (eight Box-Muller normals) and k = 4 (accept uniform).
+//
+// Training: maximum likelihood of the chart coordinates eps of (staple,
+// link) pairs from plain chains under N(mu(A), sigma(A)^2), with FP32
+// forward/backward kernels, deterministic fixed-order gradient reduction and
+// Adam.
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "../../include/ym_amd.h"
+#include "ym_device.hpp"
+#include "ym_internal.hpp"
+#include "ym_linalg.hpp"
+#include "ym_su3_dev.hpp"
+
+namespace ym {
+
+// ------------------------------------------------------------------ MLP
+constexpr int kIn = 12, kHid = 64, kOut = 16;
+constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn, oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid, oW3 = oB2 + kHid,
+              oB3 = oW3 + kOut * kHid, kParams = oB3 + kOut;
+static_assert(kParams == 6032, "parameter layout");
+constexpr float kLogSigmaMin = -6.0f, kLogSigmaMax = 1.0f;
+
+// Round to bfloat16 (nearest even) and back: the BF16 inference variant
+// rounds weights and layer inputs, accumulating in FP32.
+__host__ __device__ __forceinline__ float bf16r(float x) {
+    uint32_t u;
+    __builtin_memcpy(&u, &x, 4);
+    const uint32_t lsb = (u >> 16) & 1u;
+    u = (u + 0x7fffu + lsb) & 0xffff0000u;
+    float y;
+    __builtin_memcpy(&y, &u, 4);
+    return y;
+}
+
+template <bool BF16>
+__device__ __forceinline__ float qz(float x) {
+    if constexpr (BF16) return bf16r(x);
+    else return x;
+}
+
+// out[16] = W3 tanh(W2 tanh(W1 f + b1) + b2) + b3; P points to the 6032
+// parameters (LDS or global).
+template <bool BF16>
+__device__ __forceinline__ void mlp_forward(const float *__restrict__ P, const float f[kIn], float out[kOut]) {
+    float fin[kIn];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) fin[i] = qz<BF16>(f[i]);
+    float h1[kHid];
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) {
+        float a = P[oB1 + j];
+#pragma unroll
+        for (int i = 0; i < kIn; ++i) a = __builtin_fmaf(qz<BF16>(P[oW1 + j * kIn + i]), fin[i], a);
+        h1[j] = qz<BF16>(tanhf(a));
+    }
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) out[o] = P[oB3 + o];
+    for (int j = 0; j < kHid; ++j) {
+        float a = P[oB2 + j];
+#pragma unroll
+        for (int i = 0; i < kHid; ++i) a = __builtin_fmaf(qz<BF16>(P[oW2 + j * kHid + i]), h1[i], a);
+        const float t = qz<BF16>(tanhf(a));
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) out[o] = __builtin_fmaf(qz<BF16>(P[oW3 + o * kHid + j]), t, out[o]);
+    }
+}
+
+// ---------------------------------------------------------- link chart
+__device__ __forceinline__ la::C3 to_c3(const M3<double> &m) {
+    la::C3 c;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) c.m[i][j] = {YM_RE(m, i, j), YM_IM(m, i, j)};
+    return c;
+}
+__device__ __forceinline__ M3<double> from_c3(const la::C3 &c) {
+    M3<double> m;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            YM_RE(m, i, j) = c.m[i][j].r;
+            YM_IM(m, i, j) = c.m[i][j].i;
+        }
+    return m;
+}
+
+// Staple-aligned frame W(A) and the 12 features of A.  Returns false when A
+// is singular (no frame; the learned step leaves such a link unchanged).
+__device__ __forceinline__ bool staple_frame(const la::C3 &A, la::C3 &W, float feat[kIn]) {
+    la::C3 Ad = la::c3_dag(A), Z;
+    if (!la::c3_polar_unitary(Ad, Z)) return false;
+    const la::cx dz = la::c3_det(Z);
+    const double phz = atan2(dz.i, dz.r) / 3.0;
+    const la::cx rot = {cos(phz), -sin(phz)};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) W.m[i][j] = la::cmul(rot, Z.m[i][j]);
+    // singular values of A from the eigenvalues of A A^dagger; det phase of A
+    const la::C3 AAd = la::c3_mul(A, Ad);
+    double ev[3];
+    la::herm_eigvals(AAd, ev);
+    double s[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = sqrt(ev[k] > 0.0 ? ev[k] : 0.0) / 3.0;
+    const la::cx da = la::c3_det(A);
+    const double psi = atan2(da.i, da.r);
+    feat[0] = (float)s[0];
+    feat[1] = (float)s[1];
+    feat[2] = (float)s[2];
+    feat[3] = (float)(s[0] * s[1]);
+    feat[4] = (float)(s[0] * s[2]);
+    feat[5] = (float)(s[1] * s[2]);
+    feat[6] = (float)(s[0] * s[1] * s[2]);
+    feat[7] = (float)((s[0] + s[1] + s[2]) / 3.0);
+    feat[8] = (float)cos(psi);
+    feat[9] = (float)sin(psi);
+    feat[10] = (float)cos(psi / 3.0);
+    feat[11] = (float)sin(psi / 3.0);
+    return true;
+}
+
+// Chart coordinates of U in the frame W: U W^dagger = exp(i X(eps)).
+__device__ __forceinline__ bool chart_coords(const la::C3 &U, const la::C3 &W, double eps[8], double ph[3]) {
+    const la::C3 Y = la::c3_mul(U, la::c3_dag(W));
+    la::C3 X;
+    if (!la::su3_log(Y, X, ph)) return false;
+    la::su3_coords(X, eps);
+    return true;
+}
+
+__device__ __forceinline__ double re_tr_mul(const la::C3 &U, const la::C3 &A) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) t += U.m[i][k].r * A.m[k][i].r - U.m[i][k].i * A.m[k][i].i;
+    return t;
+}
+
+// log q up to the constant shared by every state of one link.
+__device__ __forceinline__ double log_q(const double eps[8], const double ph[3], const double mu[8],
+                                       const double inv_sig[8]) {
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const double d = (eps[a] - mu[a]) * inv_sig[a];
+        acc -= 0.5 * d * d;
+    }
+    return acc - la::haar_log_jacobian(ph);
+}
+
+__device__ __forceinline__ double unit_open(uint32_t lo, uint32_t hi) {
+    // (0, 1]: ((raw >> 11) + 1) 2^-53
+    const uint64_t v = ((uint64_t)hi << 32) | lo;
+    return (double)((v >> 11) + 1ull) * 0x1p-53;
+}
+
+template <int MU, bool BF16>
+__global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geom g, int p,
+                                                     const float *__restrict__ params, double beta, double angle,
+                                                     int metro_hits, uint64_t key, uint64_t sweep,
+                                                     unsigned long long *__restrict__ counts) {
+    __shared__ float P[kParams];
+    __shared__ unsigned long long red[4];
+    for (int i = threadIdx.x; i < kParams; i += 256) P[i] = params[i];
+    __syncthreads();
+    const int tile = tile_order(g, xcd_tile(blockIdx.x, gridDim.x));
+    const int s = tile * 256 + threadIdx.x;
+    unsigned long long tried = 0, accepted = 0, macc = 0;
+    int x[4];
+    if (s < g.vh && site_of(g, p, s, x)) {
+        M3<float> Af;
+        su3_staple_at<float, MU>(L, g, p, s, x, Af);
+        M3<double> u;
+        {
+            const M3<float> Uf = load3<float>(L, g.vh, MU, p, s);
+#pragma unroll
+            for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
+        }
+        // the chart needs an exactly special-unitary link: the FP32 link is
+        // first projected back with the FP64 Gram-Schmidt of the update
+        reunitarize3(u);
+        M3<double> Ad;
+#pragma unroll
+        for (int c = 0; c < 18; ++c) Ad.v[c] = (double)Af.v[c];
+        const la::C3 A = to_c3(Ad);
+        const uint32_t site = (uint32_t)lex_of(g, x);
+        la::C3 W;
+        float feat[kIn];
+        double eps0[8], ph0[3];
+        if (staple_frame(A, W, feat)) {
+            const la::C3 U = to_c3(u);
+            if (chart_coords(U, W, eps0, ph0)) {
+                float out[kOut];
+                mlp_forward<BF16>(P, feat, out);
+                double mu[8], sig[8], isig[8];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    const float ls = fminf(fmaxf(out[8 + a], kLogSigmaMin), kLogSigmaMax);
+                    mu[a] = (double)out[a];
+                    sig[a] = exp((double)ls);
+                    isig[a] = 1.0 / sig[a];
+                }
+                double epsn[8];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    uint32_t o[4];
+                    draw(key, site, MU, KIND_LEARNED, (uint32_t)k, sweep, o);
+                    const double u1 = unit_open(o[0], o[1]), u2 = unit53(o[2], o[3]);
+                    const double r = sqrt(-2.0 * log(u1));
+                    double sn, cs;
+                    sincos_turns(u2, sn, cs);
+                    epsn[2 * k] = mu[2 * k] + sig[2 * k] * (r * cs);
+                    epsn[2 * k + 1] = mu[2 * k + 1] + sig[2 * k + 1] * (r * sn);
+                }
+                uint32_t o[4];
+                draw(key, site, MU, KIND_LEARNED, 4u, sweep, o);
+                const double uacc = unit53(o[0], o[1]);
+                // X' = V diag(x) V^dagger; proposals outside the chart (an
+                // eigenvalue beyond +-(pi - 1e-6)) are rejected: on the chart
+                // eps -> U' is one to one, so the chart coordinates of U' are
+                // eps' and its phases the eigenvalues x
+                la::C3 Xd = la::su3_from_coords(epsn), V;
+                la::herm_jacobi(Xd, V);
+                double ph1[3];
+                bool in_chart = true;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    ph1[k] = Xd.m[k][k].r;
+                    in_chart = in_chart && fabs(ph1[k]) < 3.14159165358979;
+                }
+                tried = 1;
+                if (in_chart) {
+                    la::C3 E;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            la::cx acc = {0.0, 0.0};
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                const la::cx e = {cos(ph1[k]), sin(ph1[k])};
+                                acc = la::cadd(acc, la::cmul(la::cmul(V.m[i][k], e), la::cconj(V.m[j][k])));
+                            }
+                            E.m[i][j] = acc;
+                        }
+                    const la::C3 Un = la::c3_mul(E, W);
+                    const double lr = (beta / 3.0) * (re_tr_mul(Un, A) - re_tr_mul(U, A)) +
+                                      log_q(eps0, ph0, mu, isig) - log_q(epsn, ph1, mu, isig);
+                    if (lr >= 0.0 || uacc < exp(lr)) {
+                        accepted = 1;
+                        u = from_c3(Un);
+                    }
+                }
+            }
+        }
+        if (metro_hits > 0) {
+            M3<double> M = mul_nn(u, Ad);
+            const double nbt = -(beta / 3.0);
+            for (int h = 0; h < metro_hits; ++h) {
+                switch (h & 7) {
+                case 0: su3_hit<0, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                case 1: su3_hit<1, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                case 2: su3_hit<2, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                case 3: su3_hit<3, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                case 4: su3_hit<4, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                case 5: su3_hit<5, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                case 6: su3_hit<6, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                default: su3_hit<7, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                }
+            }
+        }
+        reunitarize3(u);
+        M3<float> Uo;
+#pragma unroll
+        for (int c = 0; c < 18; ++c) Uo.v[c] = (float)u.v[c];
+        store3<float>(L, g.vh, MU, p, s, Uo);
+    }
+    const unsigned long long t0 = block_sum_u64<256>(tried, red);
+    const unsigned long long t1 = block_sum_u64<256>(accepted, red);
+    const unsigned long long t2 = block_sum_u64<256>(macc, red);
+    if (threadIdx.x == 0) {
+        if (t0) atomicAdd(counts + 0, t0);
+        if (t1) atomicAdd(counts + 1, t1);
+        if (t2) atomicAdd(counts + 2, t2);
+    }
+}
+
+// Training pairs of every link of one (mu, parity) group: features of the
+// staple and chart coordinates of the link; valid = 0 where the frame or the
+// chart fails.  Row r = base + s.
+template <int MU>
+__global__ __launch_bounds__(256) void k_learn_pairs(const float2 *__restrict__ L, Geom g, int p, int64_t base,
+                                                     float *__restrict__ feats, float *__restrict__ targets,
+                                                     int *__restrict__ valid) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= g.vh) return;
+    const int64_t r = base + s;
+    int x[4];
+    int ok = 0;
+    float feat[kIn] = {0};
+    double eps[8] = {0};
+    if (site_of(g, p, s, x)) {
+        M3<float> Af;
+        su3_staple_at<float, MU>(L, g, p, s, x, Af);
+        M3<double> Ad, u;
+        const M3<float> Uf = load3<float>(L, g.vh, MU, p, s);
+#pragma unroll
+        for (int c = 0; c < 18; ++c) {
+            Ad.v[c] = (double)Af.v[c];
+            u.v[c] = (double)Uf.v[c];
+        }
+        reunitarize3(u);
+        la::C3 W;
+        double ph[3];
+        if (staple_frame(to_c3(Ad), W, feat) && chart_coords(to_c3(u), W, eps, ph)) ok = 1;
+    }
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) feats[r * kIn + i] = feat[i];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) targets[r * 8 + a] = (float)eps[a];
+    valid[r] = ok;
+}
+
+// Probe of one link (tests): staple, frame, chart, features.
+template <int MU>
+__global__ void k_learn_probe(const float2 *__restrict__ L, Geom g, int p, int s, double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int x[4];
+    if (!site_of(g, p, s, x)) return;
+    M3<float> Af;
+    su3_staple_at<float, MU>(L, g, p, s, x, Af);
+    M3<double> Ad, u;
+    const M3<float> Uf = load3<float>(L, g.vh, MU, p, s);
+    for (int c = 0; c < 18; ++c) {
+        Ad.v[c] = (double)Af.v[c];
+        u.v[c] = (double)Uf.v[c];
+    }
+    reunitarize3(u);
+    la::C3 W;
+    float feat[kIn] = {0};
+    double eps[8] = {0}, ph[3] = {0};
+    const int okf = staple_frame(to_c3(Ad), W, feat) ? 1 : 0;
+    const int okc = okf && chart_coords(to_c3(u), W, eps, ph) ? 1 : 0;
+    const M3<double> Wm = from_c3(W);
+    for (int c = 0; c < 18; ++c) { out[c] = Ad.v[c]; out[18 + c] = Wm.v[c]; }
+    for (int a = 0; a < 8; ++a) out[36 + a] = eps[a];
+    for (int k = 0; k < 3; ++k) out[44 + k] = ph[k];
+    for (int i = 0; i < kIn; ++i) out[47 + i] = feat[i];
+    out[59] = okf;
+    out[60] = okc;
+    out[61] = okc ? la::haar_log_jacobian(ph) : 0.0;
+}
+
+// ------------------------------------------------------------ training
+// Per sample: forward, loss, output gradient and the backpropagated
+// pre-activation gradients.  rows[k] selects the sample of batch slot k.
+__global__ __launch_bounds__(128) void k_mlp_fwd_bwd(const float *__restrict__ P, const float *__restrict__ feats,
+                                                     const float *__restrict__ targets, const int *__restrict__ valid,
+                                                     const int64_t *__restrict__ rows, int B,
+                                                     float *__restrict__ h1o, float *__restrict__ h2o,
+                                                     float *__restrict__ dout, float *__restrict__ dz2o,
+                                                     float *__restrict__ dz1o, float *__restrict__ loss) {
+    const int k = blockIdx.x * 128 + threadIdx.x;
+    if (k >= B) return;
+    const int64_t r = rows[k];
+    const float w = valid[r] ? 1.0f : 0.0f;
+    float f[kIn];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) f[i] = feats[r * kIn + i];
+    float h1[kHid];
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) {
+        float a = P[oB1 + j];
+#pragma unroll
+        for (int i = 0; i < kIn; ++i) a = __builtin_fmaf(P[oW1 + j * kIn + i], f[i], a);
+        h1[j] = tanhf(a);
+        h1o[(size_t)k * kHid + j] = h1[j];
+    }
+    float out[kOut];
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) out[o] = P[oB3 + o];
+    for (int j = 0; j < kHid; ++j) {
+        float a = P[oB2 + j];
+#pragma unroll
+        for (int i = 0; i < kHid; ++i) a = __builtin_fmaf(P[oW2 + j * kHid + i], h1[i], a);
+        const float t = tanhf(a);
+        h2o[(size_t)k * kHid + j] = t;
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) out[o] = __builtin_fmaf(P[oW3 + o * kHid + j], t, out[o]);
+    }
+    // loss = sum_a 0.5 (e_a - mu_a)^2 exp(-2 ls_a) + ls_a, ls clamped
+    float l = 0.0f, d[kOut];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const float raw = out[8 + a];
+        const float ls = fminf(fmaxf(raw, kLogSigmaMin), kLogSigmaMax);
+        const float iv = expf(-2.0f * ls);
+        const float e = targets[r * 8 + a] - out[a];
+        l += 0.5f * e * e * iv + ls;
+        d[a] = -e * iv * w;
+        d[8 + a] = (raw > kLogSigmaMin && raw < kLogSigmaMax) ? (1.0f - e * e * iv) * w : 0.0f;
+    }
+    loss[k] = l * w;
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) dout[(size_t)k * kOut + o] = d[o];
+    // dz2_j = (sum_o W3[o][j] d_o) (1 - h2_j^2)
+    float dz2[kHid];
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) {
+        float a = 0.0f;
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) a = __builtin_fmaf(P[oW3 + o * kHid + j], d[o], a);
+        const float t = h2o[(size_t)k * kHid + j];
+        dz2[j] = a * (1.0f - t * t);
+        dz2o[(size_t)k * kHid + j] = dz2[j];
+    }
+    // dz1_i = (sum_j W2[j][i] dz2_j) (1 - h1_i^2)
+#pragma unroll
+    for (int i = 0; i < kHid; ++i) {
+        float a = 0.0f;
+        for (int j = 0; j < kHid; ++j) a = __builtin_fmaf(P[oW2 + j * kHid + i], dz2[j], a);
+        dz1o[(size_t)k * kHid + i] = a * (1.0f - h1[i] * h1[i]);
+    }
+}
+
+// Weight-gradient partials: parameter q (thread) summed over batch chunk
+// blockIdx.y of `chunk` samples, in sample order.  partial[y * kParams + q].
+__global__ __launch_bounds__(256) void k_mlp_grad_partial(const float *__restrict__ feats,
+                                                          const int64_t *__restrict__ rows, int B, int chunk,
+                                                          const float *__restrict__ h1, const float *__restrict__ h2,
+                                                          const float *__restrict__ dout,
+                                                          const float *__restrict__ dz2,
+                                                          const float *__restrict__ dz1,
+                                                          float *__restrict__ partial) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= kParams) return;
+    const int k0 = blockIdx.y * chunk, k1 = min(B, k0 + chunk);
+    float acc = 0.0f;
+    if (q < oB1) {
+        const int j = q / kIn, i = q - j * kIn;
+        for (int k = k0; k < k1; ++k) acc = __builtin_fmaf(dz1[(size_t)k * kHid + j], feats[rows[k] * kIn + i], acc);
+    } else if (q < oW2) {
+        const int j = q - oB1;
+        for (int k = k0; k < k1; ++k) acc += dz1[(size_t)k * kHid + j];
+    } else if (q < oB2) {
+        const int j = (q - oW2) / kHid, i = (q - oW2) - j * kHid;
+        for (int k = k0; k < k1; ++k) acc = __builtin_fmaf(dz2[(size_t)k * kHid + j], h1[(size_t)k * kHid + i], acc);
+    } else if (q < oW3) {
+        const int j = q - oB2;
+        for (int k = k0; k < k1; ++k) acc += dz2[(size_t)k * kHid + j];
+    } else if (q < oB3) {
+        const int o = (q - oW3) / kHid, j = (q - oW3) - o * kHid;
+        for (int k = k0; k < k1; ++k) acc = __builtin_fmaf(dout[(size_t)k * kOut + o], h2[(size_t)k * kHid + j], acc);
+    } else {
+        const int o = q - oB3;
+        for (int k = k0; k < k1; ++k) acc += dout[(size_t)k * kOut + o];
+    }
+    partial[(size_t)blockIdx.y * kParams + q] = acc;
+}
+
+// grad = (sum of partials in chunk order) / count; then, when step > 0, one
+// Adam step (Kingma & Ba, 2015) on the parameters.
+__global__ __launch_bounds__(256) void k_mlp_reduce_adam(const float *__restrict__ partial, int nchunks,
+                                                         const float *__restrict__ count, float *__restrict__ grad,
+                                                         float *__restrict__ P, float *__restrict__ m1,
+                                                         float *__restrict__ m2, int step, float lr) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= kParams) return;
+    float acc = 0.0f;
+    for (int c = 0; c < nchunks; ++c) acc += partial[(size_t)c * kParams + q];
+    const float n = count[0];
+    const float gq = n > 0.0f ? acc / n : 0.0f;
+    grad[q] = gq;
+    if (step <= 0) return;
+    const float b1 = 0.9f, b2 = 0.999f, epsa = 1e-8f;
+    const float a = b1 * m1[q] + (1.0f - b1) * gq;
+    const float v = b2 * m2[q] + (1.0f - b2) * gq * gq;
+    m1[q] = a;
+    m2[q] = v;
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    P[q] -= lr * (a / c1) / (sqrtf(v / c2) + epsa);
+}
+
+// Sum of the batch losses and of the valid weights (one block, fixed order).
+__global__ __launch_bounds__(256) void k_mlp_batch_sums(const float *__restrict__ loss, const int *__restrict__ valid,
+                                                        const int64_t *__restrict__ rows, int B,
+                                                        float *__restrict__ out /* [loss_sum, count] */) {
+    __shared__ double red[4];
+    double l = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < B; k += 256) {
+        l += (double)loss[k];
+        c += valid[rows[k]] ? 1.0 : 0.0;
+    }
+    const double sl = block_sum<256>(l, red);
+    const double sc = block_sum<256>(c, red);
+    if (threadIdx.x == 0) {
+        out[0] = (float)sl;
+        out[1] = (float)sc;
+    }
+}
+
+// rows[k] = (a (first + k) + b) mod n: a permutation of the dataset when
+// gcd(a, n) = 1.  a < 2^31 and n < 2^32 keep the product in 64 bits.
+__global__ void k_mlp_rows(int64_t *__restrict__ rows, int B, int64_t first, uint64_t a, uint64_t b, int64_t n) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= B) return;
+    const uint64_t v = a * (uint64_t)(first + k) + b;
+    rows[k] = (int64_t)(v % (uint64_t)n);
+}
+
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ params, const float *__restrict__ feats,
+                                                   int64_t n, float *__restrict__ out) {
+    __shared__ float P[kParams];
+    for (int i = threadIdx.x; i < kParams; i += 256) P[i] = params[i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    float f[kIn], o[kOut];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) f[i] = feats[r * kIn + i];
+    mlp_forward<BF16>(P, f, o);
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) out[r * kOut + k] = o[k];
+}
+
+}  // namespace ym
+
+struct ym_mlp {
+    float *params = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
+    int step = 0;
+    int64_t cap = 0, n = 0;
+    float *feats = nullptr, *targets = nullptr;
+    int *valid = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+int32_t hs(hipError_t e) { return e == hipSuccess ? YM_OK : YM_HIP_ERROR; }
+
+// Xavier-uniform weights from SplitMix64, zero biases.
+void init_params(std::vector<float> &p, uint64_t seed) {
+    uint64_t st = seed;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * 0x1p-53;
+    };
+    p.assign(ym::kParams, 0.0f);
+    auto fill = [&](int off, int fan_out, int fan_in, double scale) {
+        const double lim = scale * std::sqrt(6.0 / (fan_in + fan_out));
+        for (int i = 0; i < fan_out * fan_in; ++i) p[off + i] = (float)((2.0 * next() - 1.0) * lim);
+    };
+    fill(ym::oW1, ym::kHid, ym::kIn, 1.0);
+    fill(ym::oW2, ym::kHid, ym::kHid, 1.0);
+    fill(ym::oW3, ym::kOut, ym::kHid, 0.1);
+}
+
+struct TrainScratch {
+    int B = 0, nchunks = 0, chunk = 0;
+    int64_t *rows = nullptr;
+    float *h1 = nullptr, *h2 = nullptr, *dout = nullptr, *dz2 = nullptr, *dz1 = nullptr, *loss = nullptr;
+    float *partial = nullptr, *sums = nullptr;
+    int32_t alloc(int b) {
+        B = b;
+        chunk = 256;
+        nchunks = (B + chunk - 1) / chunk;
+        int32_t st = YM_OK;
+        auto A = [&](void **p, size_t bytes) {
+            if (!st && hipMalloc(p, bytes) != hipSuccess) st = YM_ALLOCATION_FAILED;
+        };
+        A((void **)&rows, sizeof(int64_t) * B);
+        A((void **)&h1, sizeof(float) * B * ym::kHid);
+        A((void **)&h2, sizeof(float) * B * ym::kHid);
+        A((void **)&dout, sizeof(float) * B * ym::kOut);
+        A((void **)&dz2, sizeof(float) * B * ym::kHid);
+        A((void **)&dz1, sizeof(float) * B * ym::kHid);
+        A((void **)&loss, sizeof(float) * B);
+        A((void **)&partial, sizeof(float) * nchunks * ym::kParams);
+        A((void **)&sums, sizeof(float) * 2);
+        return st;
+    }
+    ~TrainScratch() {
+        void *b[] = {rows, h1, h2, dout, dz2, dz1, loss, partial, sums};
+        for (void *x : b)
+            if (x) (void)hipFree(x);
+    }
+};
+
+// One batch: rows -> forward/backward -> gradient (and Adam step when
+// step > 0).  sums = [loss sum, valid count].
+void enqueue_batch(ym_mlp *m, TrainScratch &t, int B, int64_t first, uint64_t a, uint64_t b, int step, float lr) {
+    hipLaunchKernelGGL(ym::k_mlp_rows, dim3((B + 255) / 256), dim3(256), 0, m->stream, t.rows, B, first, a, b, m->n);
+    hipLaunchKernelGGL(ym::k_mlp_fwd_bwd, dim3((B + 127) / 128), dim3(128), 0, m->stream, m->params, m->feats,
+                       m->targets, m->valid, t.rows, B, t.h1, t.h2, t.dout, t.dz2, t.dz1, t.loss);
+    const int nch = (B + t.chunk - 1) / t.chunk;
+    hipLaunchKernelGGL(ym::k_mlp_grad_partial, dim3((ym::kParams + 255) / 256, nch), dim3(256), 0, m->stream,
+                       m->feats, t.rows, B, t.chunk, t.h1, t.h2, t.dout, t.dz2, t.dz1, t.partial);
+    hipLaunchKernelGGL(ym::k_mlp_batch_sums, dim3(1), dim3(256), 0, m->stream, t.loss, m->valid, t.rows, B, t.sums);
+    hipLaunchKernelGGL(ym::k_mlp_reduce_adam, dim3((ym::kParams + 255) / 256), dim3(256), 0, m->stream, t.partial,
+                       nch, t.sums + 1, m->grad, m->params, m->m1, m->m2, step, lr);
+}
+
+uint64_t gcd64(uint64_t x, uint64_t y) {
+    while (y) {
+        const uint64_t r = x % y;
+        x = y;
+        y = r;
+    }
+    return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ym_mlp_create(uint64_t seed, int64_t capacity, ym_mlp **out) {
+    if (!out || capacity < 0 || capacity >= ((int64_t)1 << 32)) return YM_INVALID_ARGUMENT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return YM_NO_DEVICE;
+    ym_mlp *m = new (std::nothrow) ym_mlp();
+    if (!m) return YM_ALLOCATION_FAILED;
+    m->cap = capacity;
+    int32_t st = YM_OK;
+    auto A = [&](void **p, size_t bytes) {
+        if (!st && bytes && hipMalloc(p, bytes) != hipSuccess) st = YM_ALLOCATION_FAILED;
+    };
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) st = YM_HIP_ERROR;
+    A((void **)&m->params, sizeof(float) * ym::kParams);
+    A((void **)&m->m1, sizeof(float) * ym::kParams);
+    A((void **)&m->m2, sizeof(float) * ym::kParams);
+    A((void **)&m->grad, sizeof(float) * ym::kParams);
+    A((void **)&m->feats, sizeof(float) * ym::kIn * capacity);
+    A((void **)&m->targets, sizeof(float) * 8 * capacity);
+    A((void **)&m->valid, sizeof(int) * capacity);
+    if (!st) {
+        std::vector<float> p;
+        init_params(p, seed);
+        st = ym_mlp_set_params(m, p.data());
+    }
+    if (st) {
+        ym_mlp_destroy(m);
+        return st;
+    }
+    *out = m;
+    return YM_OK;
+}
+
+int32_t ym_mlp_destroy(ym_mlp *m) {
+    if (!m) return YM_INVALID_ARGUMENT;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    void *b[] = {m->params, m->m1, m->m2, m->grad, m->feats, m->targets, m->valid};
+    for (void *x : b)
+        if (x) (void)hipFree(x);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return YM_OK;
+}
+
+int32_t ym_mlp_get_params(const ym_mlp *m, float *params) {
+    if (!m || !params) return YM_INVALID_ARGUMENT;
+    int32_t st = hs(hipMemcpyAsync(params, m->params, sizeof(float) * ym::kParams, hipMemcpyDeviceToHost, m->stream));
+    if (!st) st = hs(hipStreamSynchronize(m->stream));
+    return st;
+}
+
+int32_t ym_mlp_set_params(ym_mlp *m, const float *params) {
+    if (!m || !params) return YM_INVALID_ARGUMENT;
+    for (int i = 0; i < ym::kParams; ++i)
+        if (!std::isfinite(params[i])) return YM_NON_FINITE_STATISTIC;
+    int32_t st = hs(hipMemcpyAsync(m->params, params, sizeof(float) * ym::kParams, hipMemcpyHostToDevice, m->stream));
+    if (!st) {
+        ym::launch_fill_zero((double *)m->m1, ym::kParams / 2, m->stream);
+        ym::launch_fill_zero((double *)m->m2, ym::kParams / 2, m->stream);
+        m->step = 0;
+        st = hs(hipGetLastError());
+    }
+    if (!st) st = hs(hipStreamSynchronize(m->stream));
+    return st;
+}
+
+int32_t ym_mlp_dataset_size(const ym_mlp *m, int64_t *n, int64_t *capacity) {
+    if (!m || !n) return YM_INVALID_ARGUMENT;
+    *n = m->n;
+    if (capacity) *capacity = m->cap;
+    return YM_OK;
+}
+
+int32_t ym_mlp_clear_pairs(ym_mlp *m) {
+    if (!m) return YM_INVALID_ARGUMENT;
+    m->n = 0;
+    return YM_OK;
+}
+
+int32_t ym_mlp_collect_pairs(ym_mlp *m, const ym_lattice *lat, int64_t *total) {
+    if (!m || !lat) return YM_INVALID_ARGUMENT;
+    int32_t group = 0, prec = 0;
+    int64_t ext[4];
+    int32_t st = ym::lattice_info(lat, &group, &prec, ext);
+    if (st) return st;
+    if (group != 3 || prec != 32) return YM_UNSUPPORTED;
+    for (int d = 0; d < 4; ++d)
+        if (ext[d] % 2) return YM_UNSUPPORTED;
+    const ym::Geom g = ym::make_geom(ext);
+    const int64_t need = (int64_t)8 * g.vh;
+    if (m->n + need > m->cap) return YM_TOO_MANY_CONFIGURATIONS;
+    const float2 *L = (const float2 *)ym::lattice_links(lat);
+    hipStream_t ls = ym::lattice_stream(lat);
+    st = hs(hipStreamSynchronize(ls));
+    if (st) return st;
+    const int nb = (g.vh + 255) / 256;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int p = 0; p < 2; ++p) {
+            const int64_t base = m->n + (int64_t)(mu * 2 + p) * g.vh;
+            switch (mu) {
+            case 0: hipLaunchKernelGGL(ym::k_learn_pairs<0>, dim3(nb), dim3(256), 0, m->stream, L, g, p, base, m->feats, m->targets, m->valid); break;
+            case 1: hipLaunchKernelGGL(ym::k_learn_pairs<1>, dim3(nb), dim3(256), 0, m->stream, L, g, p, base, m->feats, m->targets, m->valid); break;
+            case 2: hipLaunchKernelGGL(ym::k_learn_pairs<2>, dim3(nb), dim3(256), 0, m->stream, L, g, p, base, m->feats, m->targets, m->valid); break;
+            default: hipLaunchKernelGGL(ym::k_learn_pairs<3>, dim3(nb), dim3(256), 0, m->stream, L, g, p, base, m->feats, m->targets, m->valid); break;
+            }
+        }
+    st = hs(hipGetLastError());
+    if (!st) st = hs(hipStreamSynchronize(m->stream));
+    if (!st) m->n += need;
+    if (total) *total = m->n;
+    return st;
+}
+
+int32_t ym_mlp_download_pairs(const ym_mlp *m, int64_t first, int64_t count, float *feats, float *targets,
+                              int32_t *valid) {
+    if (!m || first < 0 || count < 0 || first + count > m->n) return YM_INVALID_ARGUMENT;
+    int32_t st = YM_OK;
+    if (feats) st = hs(hipMemcpy(feats, m->feats + first * ym::kIn, sizeof(float) * ym::kIn * count, hipMemcpyDeviceToHost));
+    if (!st && targets) st = hs(hipMemcpy(targets, m->targets + first * 8, sizeof(float) * 8 * count, hipMemcpyDeviceToHost));
+    if (!st && valid) st = hs(hipMemcpy(valid, m->valid + first, sizeof(int) * count, hipMemcpyDeviceToHost));
+    return st;
+}
+
+int32_t ym_mlp_upload_pairs(ym_mlp *m, const float *feats, const float *targets, const int32_t *valid,
+                            int64_t count) {
+    if (!m || !feats || !targets || !valid || count < 1 || count > m->cap) return YM_INVALID_ARGUMENT;
+    int32_t st = hs(hipMemcpy(m->feats, feats, sizeof(float) * ym::kIn * count, hipMemcpyHostToDevice));
+    if (!st) st = hs(hipMemcpy(m->targets, targets, sizeof(float) * 8 * count, hipMemcpyHostToDevice));
+    if (!st) st = hs(hipMemcpy(m->valid, valid, sizeof(int) * count, hipMemcpyHostToDevice));
+    if (!st) m->n = count;
+    return st;
+}
+
+int32_t ym_mlp_forward(const ym_mlp *mc, const float *feats, int64_t n, int32_t bf16, float *out) {
+    ym_mlp *m = const_cast<ym_mlp *>(mc);
+    if (!m || !feats || !out || n < 1) return YM_INVALID_ARGUMENT;
+    float *df = nullptr, *dout = nullptr;
+    if (hipMalloc((void **)&df, sizeof(float) * ym::kIn * n) != hipSuccess) return YM_ALLOCATION_FAILED;
+    if (hipMalloc((void **)&dout, sizeof(float) * ym::kOut * n) != hipSuccess) {
+        (void)hipFree(df);
+        return YM_ALLOCATION_FAILED;
+    }
+    int32_t st = hs(hipMemcpyAsync(df, feats, sizeof(float) * ym::kIn * n, hipMemcpyHostToDevice, m->stream));
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (!st) {
+        if (bf16) hipLaunchKernelGGL(ym::k_mlp_infer<true>, grid, dim3(256), 0, m->stream, m->params, df, n, dout);
+        else hipLaunchKernelGGL(ym::k_mlp_infer<false>, grid, dim3(256), 0, m->stream, m->params, df, n, dout);
+        st = hs(hipGetLastError());
+    }
+    if (!st) st = hs(hipMemcpyAsync(out, dout, sizeof(float) * ym::kOut * n, hipMemcpyDeviceToHost, m->stream));
+    if (!st) st = hs(hipStreamSynchronize(m->stream));
+    (void)hipFree(df);
+    (void)hipFree(dout);
+    return st;
+}
+
+int32_t ym_mlp_loss_grad(ym_mlp *m, int64_t first, int64_t count, float *loss, float *grad) {
+    if (!m || !loss || !grad || first < 0 || count < 1 || first + count > m->n) return YM_INVALID_ARGUMENT;
+    if (count > (1 << 20)) return YM_INVALID_ARGUMENT;
+    TrainScratch t;
+    int32_t st = t.alloc((int)count);
+    // identity permutation (a = 1, b = 0) starting at `first`
+    if (!st) {
+        enqueue_batch(m, t, (int)count, first, 1, 0, 0, 0.0f);
+        st = hs(hipGetLastError());
+    }
+    float sums[2] = {0, 0};
+    if (!st) st = hs(hipMemcpyAsync(sums, t.sums, sizeof(sums), hipMemcpyDeviceToHost, m->stream));
+    if (!st) st = hs(hipMemcpyAsync(grad, m->grad, sizeof(float) * ym::kParams, hipMemcpyDeviceToHost, m->stream));
+    if (!st) st = hs(hipStreamSynchronize(m->stream));
+    if (!st) *loss = sums[1] > 0 ? sums[0] / sums[1] : 0.0f;
+    return st;
+}
+
+int32_t ym_mlp_train(ym_mlp *m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss,
+                     float *device_ms) {
+    if (!m || epochs < 1 || batch < 1 || batch > (1 << 20) || !(lr > 0.0f) || m->n < 1) return YM_INVALID_ARGUMENT;
+    TrainScratch t;
+    int32_t st = t.alloc(batch);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = YM_HIP_ERROR;
+    if (!st) st = hs(hipEventRecord(e0, m->stream));
+    uint64_t s = seed;
+    const int64_t nb = m->n / batch;
+    for (int ep = 0; ep < epochs && !st; ++ep) {
+        // per-epoch affine permutation of the rows
+        s += 0x9E3779B97F4A7C15ull;
+        const uint64_t amax = std::min<uint64_t>((uint64_t)m->n, 1ull << 31);
+        uint64_t a = (s >> 7) % amax;
+        while (a == 0 || gcd64(a, (uint64_t)m->n) != 1) a = (a + 1) % amax;
+        const uint64_t b = (s >> 17) % (uint64_t)m->n;
+        double lsum = 0.0, csum = 0.0;
+        for (int64_t k = 0; k < std::max<int64_t>(nb, 1) && !st; ++k) {
+            const int B = (int)std::min<int64_t>(batch, m->n);
+            enqueue_batch(m, t, B, k * B, a, b, ++m->step, lr);
+            if (epoch_loss) {
+                float sums[2];
+                st = hs(hipMemcpyAsync(sums, t.sums, sizeof(sums), hipMemcpyDeviceToHost, m->stream));
+                if (!st) st = hs(hipStreamSynchronize(m->stream));
+                lsum += sums[0];
+                csum += sums[1];
+            }
+        }
+        if (epoch_loss) epoch_loss[ep] = csum > 0 ? (float)(lsum / csum) : 0.0f;
+    }
+    if (!st) st = hs(hipEventRecord(e1, m->stream));
+    if (!st) st = hs(hipStreamSynchronize(m->stream));
+    if (!st && device_ms) st = hs(hipEventElapsedTime(device_ms, e0, e1));
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return st;
+}
+
+int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *m, double beta, uint64_t key, uint64_t first_sweep,
+                              int32_t n_sweeps, int32_t metro_hits, double angle, int32_t bf16,
+                              uint64_t *learned_proposals, uint64_t *learned_accepts, uint64_t *metro_accepts,
+                              float *device_ms) {
+    if (!lat || !m || n_sweeps < 0 || metro_hits < 0 || metro_hits > 64) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(beta) || beta < 0.0 || beta > 12.0) return YM_BETA_OUT_OF_RANGE;
+    if (metro_hits > 0 && (!std::isfinite(angle) || angle < 1.0e-8 || angle > 1.0)) return YM_ANGLE_OUT_OF_RANGE;
+    int32_t group = 0, prec = 0;
+    int64_t ext[4];
+    int32_t st = ym::lattice_info(lat, &group, &prec, ext);
+    if (st) return st;
+    if (group != 3 || prec != 32) return YM_UNSUPPORTED;
+    for (int d = 0; d < 4; ++d)
+        if (ext[d] % 2) return YM_UNSUPPORTED;
+    const ym::Geom g = ym::make_geom(ext);
+    float2 *L = (float2 *)ym::lattice_links(lat);
+    hipStream_t ls = ym::lattice_stream(lat);
+    unsigned long long *cnt = nullptr;
+    if (hipMalloc((void **)&cnt, sizeof(unsigned long long) * 3) != hipSuccess) return YM_ALLOCATION_FAILED;
+    st = hs(hipStreamSynchronize(m->stream));
+    ym::launch_fill_zero_u64(cnt, 3, ls);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = YM_HIP_ERROR;
+    if (!st) st = hs(hipEventRecord(e0, ls));
+    const int nb = (g.vh + 255) / 256;
+    for (int k = 0; k < n_sweeps && !st; ++k)
+        for (int mu = 0; mu < 4; ++mu)
+            for (int p = 0; p < 2; ++p) {
+                const uint64_t sw = first_sweep + (uint64_t)k;
+#define YM_LEARNED_LAUNCH(MU, BF)                                                                                   \
+    hipLaunchKernelGGL((ym::k_su3_learned<MU, BF>), dim3(nb), dim3(256), 0, ls, L, g, p, m->params, beta, angle, \
+                       metro_hits, key, sw, cnt)
+                if (bf16) {
+                    switch (mu) {
+                    case 0: YM_LEARNED_LAUNCH(0, true); break;
+                    case 1: YM_LEARNED_LAUNCH(1, true); break;
+                    case 2: YM_LEARNED_LAUNCH(2, true); break;
+                    default: YM_LEARNED_LAUNCH(3, true); break;
+                    }
+                } else {
+                    switch (mu) {
+                    case 0: YM_LEARNED_LAUNCH(0, false); break;
+                    case 1: YM_LEARNED_LAUNCH(1, false); break;
+                    case 2: YM_LEARNED_LAUNCH(2, false); break;
+                    default: YM_LEARNED_LAUNCH(3, false); break;
+                    }
+                }
+#undef YM_LEARNED_LAUNCH
+            }
+    if (!st) st = hs(hipGetLastError());
+    if (!st) st = hs(hipEventRecord(e1, ls));
+    unsigned long long h[3] = {0, 0, 0};
+    if (!st) st = hs(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, ls));
+    if (!st) st = hs(hipStreamSynchronize(ls));
+    if (!st && device_ms) st = hs(hipEventElapsedTime(device_ms, e0, e1));
+    if (!st) {
+        if (learned_proposals) *learned_proposals += h[0];
+        if (learned_accepts) *learned_accepts += h[1];
+        if (metro_accepts) *metro_accepts += h[2];
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(cnt);
+    return st;
+}
+
+int32_t ym_su3_link_chart(const ym_lattice *lat, const int64_t x[4], int32_t mu, double *out62) {
+    if (!lat || !x || !out62 || mu < 0 || mu > 3) return YM_INVALID_ARGUMENT;
+    int32_t group = 0, prec = 0;
+    int64_t ext[4];
+    int32_t st = ym::lattice_info(lat, &group, &prec, ext);
+    if (st) return st;
+    if (group != 3 || prec != 32) return YM_UNSUPPORTED;
+    for (int d = 0; d < 4; ++d) {
+        if (ext[d] % 2) return YM_UNSUPPORTED;
+        if (x[d] < 0 || x[d] >= ext[d]) return YM_COORDINATE_OUT_OF_RANGE;
+    }
+    const ym::Geom g = ym::make_geom(ext);
+    const int p = (int)((x[0] + x[1] + x[2] + x[3]) & 1);
+    const int64_t row = (x[0] * ext[1] + x[1]) * ext[2] + x[2];
+    const int s = (int)((row * g.n3p + x[3]) >> 1);
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(double) * 62) != hipSuccess) return YM_ALLOCATION_FAILED;
+    hipStream_t ls = ym::lattice_stream(lat);
+    ym::launch_fill_zero(d, 62, ls);
+    const float2 *L = (const float2 *)ym::lattice_links(lat);
+    switch (mu) {
+    case 0: hipLaunchKernelGGL(ym::k_learn_probe<0>, dim3(1), dim3(64), 0, ls, L, g, p, s, d); break;
+    case 1: hipLaunchKernelGGL(ym::k_learn_probe<1>, dim3(1), dim3(64), 0, ls, L, g, p, s, d); break;
+    case 2: hipLaunchKernelGGL(ym::k_learn_probe<2>, dim3(1), dim3(64), 0, ls, L, g, p, s, d); break;
+    default: hipLaunchKernelGGL(ym::k_learn_probe<3>, dim3(1), dim3(64), 0, ls, L, g, p, s, d); break;
+    }
+    st = hs(hipGetLastError());
+    if (!st) st = hs(hipMemcpyAsync(out62, d, sizeof(double) * 62, hipMemcpyDeviceToHost, ls));
+    if (!st) st = hs(hipStreamSynchronize(ls));
+    (void)hipFree(d);
+    return st;
+}
+
+}  // extern "C"

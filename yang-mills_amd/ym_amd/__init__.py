@@ -16,6 +16,7 @@ from ._lib import YmError, lib, library_path, status_string  # noqa: F401
 from . import stats  # noqa: F401
 from . import ensemble  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
+from . import learned  # noqa: F401
 from .lattice import (  # noqa: F401
     Lattice4,
     Su3Lattice4,

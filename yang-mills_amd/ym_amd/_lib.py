@@ -86,6 +86,21 @@ SIGNATURES = {
     "ym_ensemble_ops_length": (i32, [C.c_void_p, i32, P(i64), P(i64)]),
     "ym_ensemble_run_cycles": (i32, [C.c_void_p, dbl, dbl, i32, i32, i32, i32, dbl, i32, i32, P(dbl), P(dbl),
                                      P(u64), P(flt)]),
+    "ym_mlp_create": (i32, [u64, i64, P(C.c_void_p)]),
+    "ym_mlp_destroy": (i32, [C.c_void_p]),
+    "ym_mlp_get_params": (i32, [C.c_void_p, P(flt)]),
+    "ym_mlp_set_params": (i32, [C.c_void_p, P(flt)]),
+    "ym_mlp_dataset_size": (i32, [C.c_void_p, P(i64), P(i64)]),
+    "ym_mlp_clear_pairs": (i32, [C.c_void_p]),
+    "ym_mlp_collect_pairs": (i32, [C.c_void_p, C.c_void_p, P(i64)]),
+    "ym_mlp_download_pairs": (i32, [C.c_void_p, i64, i64, P(flt), P(flt), P(i32)]),
+    "ym_mlp_upload_pairs": (i32, [C.c_void_p, P(flt), P(flt), P(i32), i64]),
+    "ym_mlp_forward": (i32, [C.c_void_p, P(flt), i64, i32, P(flt)]),
+    "ym_mlp_loss_grad": (i32, [C.c_void_p, i64, i64, P(flt), P(flt)]),
+    "ym_mlp_train": (i32, [C.c_void_p, i32, i32, flt, u64, P(flt), P(flt)]),
+    "ym_su3_learned_sweeps": (i32, [C.c_void_p, C.c_void_p, dbl, u64, u64, i32, i32, dbl, i32, P(u64), P(u64),
+                                    P(u64), P(flt)]),
+    "ym_su3_link_chart": (i32, [C.c_void_p, P(i64), i32, P(dbl)]),
     "ym_evaluate": (i32, [P(dbl), i64, P(dbl), i64, dbl, dbl, dbl, P(EfficiencyReport)]),
     "ym_stationarity": (i32, [P(dbl), i64, i64, P(StationarityReport)]),
     "ym_pool_chains": (i32, [P(dbl), i64, i64, i64, i64, P(dbl), P(dbl), P(dbl), P(dbl), P(PoolSummary)]),
