@@ -10,6 +10,9 @@ reunitarisation FP64.  Default workload: 32^4, beta = 6.0, proposal angle
 by the warm-up sweeps; the links stay resident in HBM for the whole timed
 region.
 
+The second half of the metric, plaquette ESS/s of the learned proposal
+against plain Metropolis, is measured on SU(3) 16^3x32 (learned_vs_plain).
+
 Prints one JSON line with the throughput (link updates/s), the roofline of
 the dominant kernel (k_su3_metropolis, one launch per checkerboard sub-pass:
 algorithmic bytes per launch / mean launch duration from HIP events on its
@@ -82,6 +85,75 @@ def cpu_baseline(seconds_target):
     }
 
 
+def learned_vs_plain(chains=4, sweeps=1500, discard=300, therm=2000, configs=20, epochs=10):
+    """Plaquette ESS/s of the learned proposal (one MH-corrected independence
+    proposal per link per sweep, BF16 MFMA inference, no Metropolis hits)
+    against plain 8-hit Metropolis on SU(3) 16^3x32 at beta = 6.0.  Chain c
+    is thermalised on its own (cold start, `therm` plain sweeps, its own key);
+    the plain and the learned chain c both continue from that configuration
+    for `sweeps` sweeps, measuring the plaquette after every sweep.  ESS from
+    the reference diagnostics (positive-window tau_int, su2_ensemble.rs:
+    375-463) summed over chains (su3_confirmation.rs:214); seconds are the
+    wall time of the sweeps and measurements.  Training (pair collection from
+    one thermalised chain + Adam) is reported separately as setup cost."""
+    import numpy as np
+
+    import ym_amd
+
+    n = (16, 16, 16, 32)
+    keys = [ym_amd.derive_su3_chain_seed(0x2C02198105010002, 1 + c) for c in range(chains)]
+    starts = []
+    for c in range(chains):
+        lat = ym_amd.Su3Lattice4(n, 32)
+        lat.metropolis_sweeps(6.0, 0.35, keys[c], 0, therm, 8)
+        starts.append(lat)
+    t0 = time.perf_counter()
+    mlp = ym_amd.learned.Mlp(seed=11, dataset_capacity=configs * 4 * int(np.prod(n)))
+    src = starts[0].clone()
+    for k in range(configs):
+        src.metropolis_sweeps(6.0, 0.35, keys[0], therm + 5 * k, 5, 8)
+        mlp.collect_pairs(src)
+    pairs = mlp.dataset_size()[0]
+    losses, train_ms = mlp.train(epochs=epochs, batch=16384, lr=2e-3)
+    setup_s = time.perf_counter() - t0
+    out = {"workload": "SU(3) Wilson action 16^3x32, beta=6.0, plaquette after every sweep",
+           "training_pairs": pairs, "training_epochs": epochs, "final_training_loss": float(losses[-1]),
+           "setup_seconds": setup_s, "chains": chains, "sweeps_per_chain": sweeps, "discarded": discard,
+           "thermalisation_sweeps_per_chain": therm}
+    for kind in ("plain", "learned"):
+        rows, acc, wall = [], [], 0.0
+        for c in range(chains):
+            l = starts[c].clone()
+            vals = []
+            t1 = time.perf_counter()
+            for s in range(sweeps):
+                sw = 100000 + s
+                if kind == "plain":
+                    l.metropolis_sweeps(6.0, 0.35, keys[c], sw, 1, 8)
+                else:
+                    r = ym_amd.learned.learned_sweeps(l, mlp, 6.0, keys[c], sw, 1, metro_hits=0, bf16=True)
+                    acc.append(r.learned_accepts / max(1, r.learned_proposals))
+                vals.append(l.average_plaquette())
+            wall += time.perf_counter() - t1
+            rows.append(np.array(vals[discard:]))
+        rows = np.array(rows)
+        pool = ym_amd.stats.pool_chains(rows, 100)
+        taus = [ym_amd.diagnostics(r, 100)[0].tau_int for r in rows]
+        means = rows.mean(axis=1)
+        out[kind] = {"plaquette": pool.pooled_mean, "plaquette_se": pool.pooled_se,
+                     "between_chain_se": float(means.std(ddof=1) / np.sqrt(chains)),
+                     "ess": pool.aggregate_ess, "tau_int": taus, "seconds": wall,
+                     "ess_per_s": pool.aggregate_ess / wall}
+        if acc:
+            out[kind]["acceptance"] = float(np.mean(acc))
+    out["ess_per_s_ratio"] = out["learned"]["ess_per_s"] / out["plain"]["ess_per_s"]
+    d = out["learned"]["plaquette"] - out["plain"]["plaquette"]
+    se = max(out["learned"]["plaquette_se"], out["learned"]["between_chain_se"]) ** 2 + \
+        max(out["plain"]["plaquette_se"], out["plain"]["between_chain_se"]) ** 2
+    out["plaquette_z"] = abs(d) / se ** 0.5
+    return out
+
+
 def load_traffic(extent):
     path = os.path.join(REPO, "profiles", f"su3_metropolis_{extent}x4_traffic.json")
     if not os.path.exists(path):
@@ -101,11 +173,12 @@ def main():
     ap.add_argument("--beta", type=float, default=6.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain ESS/s measurement")
     ap.add_argument("--hot-start", action="store_true",
                     help="Haar-random start instead of the reference's cold (identity) start")
-    ap.add_argument("--reunit-calibration", action="store_true",
-                    help="after the timed region run one k_su3_reunit pass (reads and writes every "
-                         "link once) to calibrate HBM byte counters")
+    ap.add_argument("--traffic-calibration", action="store_true",
+                    help="after the timed region copy the link field once (a plain device copy of "
+                         "known size) to calibrate the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,9 +206,9 @@ def main():
     prop, acc, total_ms, ksum_ms, launches = lat.metropolis_sweeps_profiled(
         args.beta, angle, key, args.warmup, args.steps, args.hits)
     wall = time.perf_counter() - t0
-    if args.reunit_calibration:
-        log(f"calibration: k_su3_reunit over {lat.device_bytes()} bytes")
-        lat.reunitarize()
+    if args.traffic_calibration:
+        log(f"calibration: device copy of {lat.device_bytes()} bytes")
+        lat.clone()
     plaq = lat.average_plaquette()
     unit_res, det_res = lat.link_residuals()
     updates = 4 * V * args.steps
@@ -145,6 +218,13 @@ def main():
     achieved = bytes_per_launch / avg_launch_s / 1e9
     log(f"{value:.4e} link updates/s, k_su3_metropolis mean {avg_launch_s * 1e6:.1f} us, "
         f"{achieved:.0f} GB/s algorithmic")
+
+    learned = None
+    if not args.no_learned:
+        log("learned vs plain plaquette ESS/s (16^3x32)")
+        learned = learned_vs_plain()
+        log(f"ESS/s learned {learned['learned']['ess_per_s']:.1f} vs plain {learned['plain']['ess_per_s']:.1f} "
+            f"(x{learned['ess_per_s_ratio']:.2f}), acceptance {learned['learned'].get('acceptance', 0):.3f}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -194,6 +274,7 @@ def main():
             "mean_launch_us": avg_launch_s * 1e6,
         },
         "cpu_baseline": cpu,
+        "learned_vs_plain": learned,
     }
     print(json.dumps(out), flush=True)
 

@@ -122,7 +122,7 @@ __device__ __forceinline__ M3<double> from_c3(const la::C3 &c) {
 // is singular (no frame; the learned step leaves such a link unchanged).
 __device__ __forceinline__ bool staple_frame(const la::C3 &A, la::C3 &W, float feat[kIn]) {
     la::C3 Ad = la::c3_dag(A), Z;
-    if (!la::c3_polar_unitary(Ad, Z)) return false;
+    if (!la::c3_polar_unitary_fast(Ad, Z)) return false;
     const la::cx dz = la::c3_det(Z);
     const double phz = atan2(dz.i, dz.r) / 3.0;
     const la::cx rot = {cos(phz), -sin(phz)};
@@ -158,7 +158,7 @@ __device__ __forceinline__ bool staple_frame(const la::C3 &A, la::C3 &W, float f
 __device__ __forceinline__ bool chart_coords(const la::C3 &U, const la::C3 &W, double eps[8], double ph[3]) {
     const la::C3 Y = la::c3_mul(U, la::c3_dag(W));
     la::C3 X;
-    if (!la::su3_log(Y, X, ph)) return false;
+    if (!la::su3_log_fast(Y, X, ph)) return false;
     la::su3_coords(X, eps);
     return true;
 }
@@ -190,102 +190,218 @@ __device__ __forceinline__ double unit_open(uint32_t lo, uint32_t hi) {
     return (double)((v >> 11) + 1ull) * 0x1p-53;
 }
 
+// ------------------------------------------------- BF16 MFMA inference
+// One wavefront evaluates the MLP for its 64 links with
+// v_mfma_f32_32x32x16_bf16 (D 32x32 += A 32x16 * B 16x32, FP32
+// accumulation).  Lane layouts (gfx950): A, lane l holds row l % 32, k =
+// 8 (l / 32) .. +7; B, lane l holds column l % 32, the same k range; D, lane
+// l register r holds row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
+// Activations pass between layers through two 64 x 64 bf16 LDS tiles per
+// wave (row-chunk XOR swizzle against bank conflicts); weights are bf16 in
+// LDS, biases FP32; inputs, weights and hidden activations are rounded to
+// bf16 (nearest even), as in the FP32-accumulating restatement.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int swz(int row, int col) {
+    return row * 64 + ((((col >> 3) ^ (row & 7)) << 3) | (col & 7));
+}
+
+struct MlpLds {
+    __bf16 w1[64 * 16];  // W1 [64][16] row-major, k >= 12 zero
+    __bf16 w2[64 * 64];  // W2 swizzled
+    __bf16 w3[32 * 64];  // W3 [32][64] swizzled, rows >= 16 zero
+    float b1[64], b2[64], b3[16];
+};
+
+__device__ __forceinline__ void mlp_lds_fill(MlpLds &S, const float *__restrict__ P) {
+    for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+        const int r = i >> 6, c = i & 63;
+        S.w2[swz(r, c)] = (__bf16)P[oW2 + r * kHid + c];
+        if (r < 32) S.w3[swz(r, c)] = (__bf16)((r < kOut) ? P[oW3 + r * kHid + c] : 0.0f);
+        if (c < 16) S.w1[r * 16 + c] = (__bf16)((c < kIn) ? P[oW1 + r * kIn + c] : 0.0f);
+    }
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+        S.b1[i] = P[oB1 + i];
+        S.b2[i] = P[oB2 + i];
+        if (i < kOut) S.b3[i] = P[oB3 + i];
+    }
+}
+
+__device__ __forceinline__ bf16x8 lds_chunk(const __bf16 *__restrict__ t, int row, int chunk) {
+    return *reinterpret_cast<const bf16x8 *>(t + row * 64 + ((chunk ^ (row & 7)) << 3));
+}
+
+// Must be reached by every thread of the block.  bufA/bufB: this wave's two
+// 64 x 64 bf16 tiles.  Rows of lanes without a link carry zero features.
+__device__ __forceinline__ void mlp_wave_bf16(const MlpLds &S, __bf16 *__restrict__ bufA, __bf16 *__restrict__ bufB,
+                                              const float feat[kIn], float out[kOut]) {
+    const int l = threadIdx.x & 63, lr = l & 31, lh = l >> 5;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) bufB[swz(l, c)] = (__bf16)(c < kIn ? feat[c] : 0.0f);
+    __syncthreads();
+    // layer 1: H1 = tanh(F W1^T + b1) -> bufA
+#pragma unroll
+    for (int R = 0; R < 2; ++R)
+#pragma unroll
+        for (int Cb = 0; Cb < 2; ++Cb) {
+            f32x16 acc = {};
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                lds_chunk(bufB, 32 * R + lr, lh),
+                *reinterpret_cast<const bf16x8 *>(S.w1 + (32 * Cb + lr) * 16 + 8 * lh), acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 32 * R + 8 * (r >> 2) + 4 * lh + (r & 3), col = 32 * Cb + lr;
+                bufA[swz(row, col)] = (__bf16)tanhf(acc[r] + S.b1[col]);
+            }
+        }
+    __syncthreads();
+    // layer 2: H2 = tanh(H1 W2^T + b2) -> bufB
+#pragma unroll
+    for (int R = 0; R < 2; ++R)
+#pragma unroll
+        for (int Cb = 0; Cb < 2; ++Cb) {
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_chunk(bufA, 32 * R + lr, 2 * ks + lh),
+                                                              lds_chunk(S.w2, 32 * Cb + lr, 2 * ks + lh), acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 32 * R + 8 * (r >> 2) + 4 * lh + (r & 3), col = 32 * Cb + lr;
+                bufB[swz(row, col)] = (__bf16)tanhf(acc[r] + S.b2[col]);
+            }
+        }
+    __syncthreads();
+    // layer 3: OUT = H2 W3^T + b3 (columns 16..31 of the block are padding);
+    // FP32 results staged in bufA (64 x 16 floats)
+    float *ob = reinterpret_cast<float *>(bufA);
+#pragma unroll
+    for (int R = 0; R < 2; ++R) {
+        f32x16 acc = {};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_chunk(bufB, 32 * R + lr, 2 * ks + lh),
+                                                          lds_chunk(S.w3, lr, 2 * ks + lh), acc, 0, 0, 0);
+        if (lr < kOut) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 32 * R + 8 * (r >> 2) + 4 * lh + (r & 3);
+                ob[row * kOut + lr] = acc[r] + S.b3[lr];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kOut; ++c) out[c] = ob[l * kOut + c];
+    __syncthreads();
+}
+
+template <bool BF16> struct LearnedLds;
+template <> struct LearnedLds<false> {
+    float P[kParams];
+};
+template <> struct LearnedLds<true> {
+    MlpLds S;
+    __bf16 tiles[4][2][64 * 64];  // per wave of a 256-thread block
+};
+
 template <int MU, bool BF16>
 __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geom g, int p,
                                                      const float *__restrict__ params, double beta, double angle,
                                                      int metro_hits, uint64_t key, uint64_t sweep,
                                                      unsigned long long *__restrict__ counts) {
-    __shared__ float P[kParams];
+    __shared__ LearnedLds<BF16> sh;
     __shared__ unsigned long long red[4];
-    for (int i = threadIdx.x; i < kParams; i += 256) P[i] = params[i];
+    if constexpr (BF16) mlp_lds_fill(sh.S, params);
+    else
+        for (int i = threadIdx.x; i < kParams; i += 256) sh.P[i] = params[i];
     __syncthreads();
     const int tile = tile_order(g, xcd_tile(blockIdx.x, gridDim.x));
     const int s = tile * 256 + threadIdx.x;
     unsigned long long tried = 0, accepted = 0, macc = 0;
-    int x[4];
-    if (s < g.vh && site_of(g, p, s, x)) {
+    int x[4] = {0, 0, 0, 0};
+    const bool live = s < g.vh && site_of(g, p, s, x);
+    // phase 1 (per link): staple, projected link, frame, features, chart
+    M3<double> u, Ad;
+    la::C3 A, W;
+    float feat[kIn];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) feat[i] = 0.0f;
+    double eps0[8], ph0[3];
+    bool ok = false;
+    uint32_t site = 0;
+    if (live) {
         M3<float> Af;
         su3_staple_at<float, MU>(L, g, p, s, x, Af);
-        M3<double> u;
-        {
-            const M3<float> Uf = load3<float>(L, g.vh, MU, p, s);
+        const M3<float> Uf = load3<float>(L, g.vh, MU, p, s);
 #pragma unroll
-            for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
+        for (int c = 0; c < 18; ++c) {
+            u.v[c] = (double)Uf.v[c];
+            Ad.v[c] = (double)Af.v[c];
         }
         // the chart needs an exactly special-unitary link: the FP32 link is
         // first projected back with the FP64 Gram-Schmidt of the update
         reunitarize3(u);
-        M3<double> Ad;
+        A = to_c3(Ad);
+        site = (uint32_t)lex_of(g, x);
+        if (staple_frame(A, W, feat)) ok = chart_coords(to_c3(u), W, eps0, ph0);
+        else {
 #pragma unroll
-        for (int c = 0; c < 18; ++c) Ad.v[c] = (double)Af.v[c];
-        const la::C3 A = to_c3(Ad);
-        const uint32_t site = (uint32_t)lex_of(g, x);
-        la::C3 W;
-        float feat[kIn];
-        double eps0[8], ph0[3];
-        if (staple_frame(A, W, feat)) {
-            const la::C3 U = to_c3(u);
-            if (chart_coords(U, W, eps0, ph0)) {
-                float out[kOut];
-                mlp_forward<BF16>(P, feat, out);
-                double mu[8], sig[8], isig[8];
+            for (int i = 0; i < kIn; ++i) feat[i] = 0.0f;
+        }
+    }
+    // phase 2 (whole block): proposal parameters
+    float out[kOut];
+    if constexpr (BF16) {
+        const int w = threadIdx.x >> 6;
+        mlp_wave_bf16(sh.S, sh.tiles[w][0], sh.tiles[w][1], feat, out);
+    } else {
+        mlp_forward<false>(sh.P, feat, out);
+    }
+    // phase 3 (per link): proposal, MH test, Metropolis hits, store
+    if (live) {
+        if (ok) {
+            double mu[8], sig[8], isig[8];
 #pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    const float ls = fminf(fmaxf(out[8 + a], kLogSigmaMin), kLogSigmaMax);
-                    mu[a] = (double)out[a];
-                    sig[a] = exp((double)ls);
-                    isig[a] = 1.0 / sig[a];
-                }
-                double epsn[8];
+            for (int a = 0; a < 8; ++a) {
+                const float ls = fminf(fmaxf(out[8 + a], kLogSigmaMin), kLogSigmaMax);
+                mu[a] = (double)out[a];
+                sig[a] = exp((double)ls);
+                isig[a] = 1.0 / sig[a];
+            }
+            double epsn[8];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    uint32_t o[4];
-                    draw(key, site, MU, KIND_LEARNED, (uint32_t)k, sweep, o);
-                    const double u1 = unit_open(o[0], o[1]), u2 = unit53(o[2], o[3]);
-                    const double r = sqrt(-2.0 * log(u1));
-                    double sn, cs;
-                    sincos_turns(u2, sn, cs);
-                    epsn[2 * k] = mu[2 * k] + sig[2 * k] * (r * cs);
-                    epsn[2 * k + 1] = mu[2 * k + 1] + sig[2 * k + 1] * (r * sn);
-                }
+            for (int k = 0; k < 4; ++k) {
                 uint32_t o[4];
-                draw(key, site, MU, KIND_LEARNED, 4u, sweep, o);
-                const double uacc = unit53(o[0], o[1]);
-                // X' = V diag(x) V^dagger; proposals outside the chart (an
-                // eigenvalue beyond +-(pi - 1e-6)) are rejected: on the chart
-                // eps -> U' is one to one, so the chart coordinates of U' are
-                // eps' and its phases the eigenvalues x
-                la::C3 Xd = la::su3_from_coords(epsn), V;
-                la::herm_jacobi(Xd, V);
-                double ph1[3];
-                bool in_chart = true;
+                draw(key, site, MU, KIND_LEARNED, (uint32_t)k, sweep, o);
+                const double u1 = unit_open(o[0], o[1]), u2 = unit53(o[2], o[3]);
+                const double r = sqrt(-2.0 * log(u1));
+                double sn, cs;
+                sincos_turns(u2, sn, cs);
+                epsn[2 * k] = mu[2 * k] + sig[2 * k] * (r * cs);
+                epsn[2 * k + 1] = mu[2 * k + 1] + sig[2 * k + 1] * (r * sn);
+            }
+            uint32_t o[4];
+            draw(key, site, MU, KIND_LEARNED, 4u, sweep, o);
+            const double uacc = unit53(o[0], o[1]);
+            // X' = V diag(x) V^dagger; proposals outside the chart (an
+            // eigenvalue beyond +-(pi - 1e-6)) are rejected: on the chart
+            // eps -> U' is one to one, so the chart coordinates of U' are
+            // eps' and its phases the eigenvalues x
+            double ph1[3];
+            const la::C3 E = la::expi_hermitian_fast(la::su3_from_coords(epsn), ph1);
+            bool in_chart = true;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    ph1[k] = Xd.m[k][k].r;
-                    in_chart = in_chart && fabs(ph1[k]) < 3.14159165358979;
-                }
-                tried = 1;
-                if (in_chart) {
-                    la::C3 E;
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-#pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            la::cx acc = {0.0, 0.0};
-#pragma unroll
-                            for (int k = 0; k < 3; ++k) {
-                                const la::cx e = {cos(ph1[k]), sin(ph1[k])};
-                                acc = la::cadd(acc, la::cmul(la::cmul(V.m[i][k], e), la::cconj(V.m[j][k])));
-                            }
-                            E.m[i][j] = acc;
-                        }
-                    const la::C3 Un = la::c3_mul(E, W);
-                    const double lr = (beta / 3.0) * (re_tr_mul(Un, A) - re_tr_mul(U, A)) +
-                                      log_q(eps0, ph0, mu, isig) - log_q(epsn, ph1, mu, isig);
-                    if (lr >= 0.0 || uacc < exp(lr)) {
-                        accepted = 1;
-                        u = from_c3(Un);
-                    }
+            for (int k = 0; k < 3; ++k) in_chart = in_chart && fabs(ph1[k]) < 3.14159165358979;
+            tried = 1;
+            if (in_chart) {
+                const la::C3 Un = la::c3_mul(E, W);
+                const double lr = (beta / 3.0) * (re_tr_mul(Un, A) - re_tr_mul(to_c3(u), A)) +
+                                  log_q(eps0, ph0, mu, isig) - log_q(epsn, ph1, mu, isig);
+                if (lr >= 0.0 || uacc < exp(lr)) {
+                    accepted = 1;
+                    u = from_c3(Un);
                 }
             }
         }
@@ -547,15 +663,22 @@ __global__ void k_mlp_rows(int64_t *__restrict__ rows, int B, int64_t first, uin
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ params, const float *__restrict__ feats,
                                                    int64_t n, float *__restrict__ out) {
-    __shared__ float P[kParams];
-    for (int i = threadIdx.x; i < kParams; i += 256) P[i] = params[i];
+    __shared__ LearnedLds<BF16> sh;
+    if constexpr (BF16) mlp_lds_fill(sh.S, params);
+    else
+        for (int i = threadIdx.x; i < kParams; i += 256) sh.P[i] = params[i];
     __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (r >= n) return;
     float f[kIn], o[kOut];
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) f[i] = feats[r * kIn + i];
-    mlp_forward<BF16>(P, f, o);
+    for (int i = 0; i < kIn; ++i) f[i] = r < n ? feats[r * kIn + i] : 0.0f;
+    if constexpr (BF16) {
+        const int w = threadIdx.x >> 6;
+        mlp_wave_bf16(sh.S, sh.tiles[w][0], sh.tiles[w][1], f, o);
+    } else {
+        mlp_forward<false>(sh.P, f, o);
+    }
+    if (r >= n) return;
 #pragma unroll
     for (int k = 0; k < kOut; ++k) out[r * kOut + k] = o[k];
 }

@@ -297,6 +297,157 @@ __host__ __device__ __forceinline__ bool su3_log(const C3 &y, C3 &x, double phas
     return true;
 }
 
+// Spectral projectors of a Hermitian matrix with eigenvalues ev (descending,
+// from herm_eigvals): P_k = prod_{j != k} (h - ev_j) / (ev_k - ev_j).
+// Returns false when two eigenvalues are closer than 1e-3 of the spread
+// (the interpolation loses accuracy there; callers fall back to Jacobi).
+__host__ __device__ __forceinline__ bool herm_projectors(const C3 &h, const double ev[3], C3 P[3]) {
+    const double spread = ev[0] - ev[2];
+    const double g01 = ev[0] - ev[1], g12 = ev[1] - ev[2];
+    if (!(spread > 1e-300) || !(g01 > 1e-3 * spread) || !(g12 > 1e-3 * spread)) return false;
+    C3 M[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        M[k] = h;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) M[k].m[d][d].r -= ev[k];
+    }
+    const double inv0 = 1.0 / (g01 * (ev[0] - ev[2]));
+    const double inv1 = 1.0 / ((ev[1] - ev[0]) * g12);
+    const double inv2 = 1.0 / ((ev[2] - ev[0]) * (ev[2] - ev[1]));
+    P[0] = c3_mul(M[1], M[2]);
+    P[1] = c3_mul(M[0], M[2]);
+    P[2] = c3_mul(M[0], M[1]);
+    const double inv[3] = {inv0, inv1, inv2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) P[k].m[i][j] = cscale(P[k].m[i][j], inv[k]);
+    return true;
+}
+
+// Unitary polar factor of a through the spectrum of a^dagger a = Q diag(s^2)
+// Q^dagger: w = a sum_k P_k / s_k; falls back to the Newton iteration when
+// two singular values nearly coincide.
+__host__ __device__ __forceinline__ bool c3_polar_unitary_fast(const C3 &a, C3 &w) {
+    const C3 h = c3_mul(c3_dag(a), a);
+    double ev[3];
+    herm_eigvals(h, ev);
+    C3 P[3];
+    if (ev[2] > 1e-4 * ev[0] && herm_projectors(h, ev, P)) {
+        C3 S;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                cx t = {0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) t = cadd(t, cscale(P[k].m[i][j], 1.0 / sqrt(ev[k])));
+                S.m[i][j] = t;
+            }
+        w = c3_mul(a, S);
+        return true;
+    }
+    return c3_polar_unitary(a, w);
+}
+
+// exp(i X) for Hermitian X, with its eigenvalues x (descending): through the
+// spectral projectors, or Jacobi when two eigenvalues nearly coincide.
+__host__ __device__ __forceinline__ C3 expi_hermitian_fast(const C3 &x, double ev[3]) {
+    herm_eigvals(x, ev);
+    C3 P[3];
+    if (herm_projectors(x, ev, P)) {
+        C3 out;
+        cx e[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) e[k] = {cos(ev[k]), sin(ev[k])};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                cx t = {0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) t = cadd(t, cmul(e[k], P[k].m[i][j]));
+                out.m[i][j] = t;
+            }
+        return out;
+    }
+    C3 h = x, v;
+    herm_jacobi(h, v);
+    C3 out;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ev[k] = h.m[k][k].r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            cx t = {0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const cx e = {cos(ev[k]), sin(ev[k])};
+                t = cadd(t, cmul(cmul(v.m[i][k], e), cconj(v.m[j][k])));
+            }
+            out.m[i][j] = t;
+        }
+    return out;
+}
+
+// Principal logarithm as su3_log, first through the spectral projectors of
+// the Hermitian H = (y - y^dagger)/(2i) + c (y + y^dagger)/2 (which commutes
+// with y): phases from tr(y P_k); falls back to su3_log when two eigenvalues
+// of H nearly coincide or a projector is not rank one (|tr(y P_k)| != 1).
+__host__ __device__ __forceinline__ bool su3_log_fast(const C3 &y, C3 &x, double phases[3]) {
+    const double c = 0.3819660112501051518;
+    C3 h;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const cx a = y.m[i][j], b = cconj(y.m[j][i]);
+            h.m[i][j] = {0.5 * (a.i - b.i) + 0.5 * c * (a.r + b.r), -0.5 * (a.r - b.r) + 0.5 * c * (a.i + b.i)};
+        }
+    double ev[3];
+    herm_eigvals(h, ev);
+    C3 P[3];
+    if (herm_projectors(h, ev, P)) {
+        double ph[3], sum = 0.0;
+        bool good = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cx z = {0.0, 0.0};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) z = cadd(z, cmul(y.m[i][j], P[k].m[j][i]));
+            good = good && fabs(cabs2(z) - 1.0) < 1e-9;
+            ph[k] = atan2(z.i, z.r);
+            sum += ph[k];
+        }
+        if (good) {
+            if (!(fabs(sum) < 1e-9)) return false;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (!(fabs(ph[k]) < 3.14159165358979)) return false;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    cx t = {0.0, 0.0};
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) t = cadd(t, cscale(P[k].m[i][j], ph[k]));
+                    x.m[i][j] = t;
+                }
+            if (phases)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) phases[k] = ph[k];
+            return true;
+        }
+    }
+    return su3_log(y, x, phases);
+}
+
 // log of the Haar density of exp(i X) in the coordinates eps (X = eps.lambda/2)
 // relative to Lebesgue measure, up to a constant:
 // sum_{j<k} 2 log sinc((x_j - x_k) / 2), x the eigenvalues of X.

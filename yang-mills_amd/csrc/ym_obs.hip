@@ -12,9 +12,8 @@ namespace ym {
 
 // ------------------------------------------------------------ plaquette
 // Per site: sum over mu < nu of (1/N) Re Tr P_{mu nu}(x) with
-// P = [U_mu(x) U_nu(x+mu)] [U_nu(x) U_mu(x+nu)]^dagger.  part[blk] = sum.
-// Also accumulates per-timeslice spatial / temporal sums when tsl != null:
-// tsl[(t*2 + kind) * nblk + blk] (kind 0 spatial planes, 1 temporal planes).
+// P = [U_mu(x) U_nu(x+mu)] [U_nu(x) U_mu(x+nu)]^dagger (plaquette_site);
+// part[parity * nblk + blk] = block sum.
 template <class G>
 __global__ __launch_bounds__(256) void k_plaquette(const typename G::Store *__restrict__ L, Geom g,
                                                    double *__restrict__ part) {
