@@ -22,6 +22,8 @@ metropolis_sweep timed on the host cores).
 import argparse
 import ctypes as C
 import json
+
+import numpy as np
 import os
 import sys
 import threading
@@ -154,6 +156,46 @@ def learned_vs_plain(chains=4, sweeps=1500, discard=300, therm=2000, configs=20,
     return out
 
 
+def side_workload(name):
+    """Workloads beside the headline sweep, each printed as its own JSON line:
+    FP32-vs-FP64 observable errors for SU(2) (32^4, 48^4 x beta 2.3, 2.5)
+    and SU(3) (24^4, 32^4 with 10 stout levels), and the batched 256-chain
+    glueball production."""
+    import ym_amd
+
+    if name == "su2-precision":
+        rows = []
+        for ext in (32, 48):
+            for beta in (2.3, 2.5):
+                log(f"SU(2) {ext}^4 beta={beta}")
+                rows.append(ym_amd.precision.su2_precision(ext, beta))
+        return {"workload": "SU(2) FP32 storage vs FP64, hot start, 1 HB + 3 OR per update", "results": rows}
+    if name == "su3-precision":
+        rows = []
+        for ext in (24, 32):
+            log(f"SU(3) {ext}^4")
+            rows.append(ym_amd.precision.su3_precision(ext))
+        return {"workload": "SU(3) FP32 storage vs FP64, beta=6.0, 10 stout levels rho=0.1", "results": rows}
+    if name == "ensemble":
+        log("batched production: 256 chains, 68 sweeps, raw + 4 stout levels")
+        t0 = time.perf_counter()
+        rep = ym_amd.ensemble.production(chains_per_geometry=64, stout_levels=4)
+        wall = time.perf_counter() - t0
+        links = 64 * sum(4 * g[0] * g[1] * g[2] * g[3] for g in rep.geometries)
+        corr = {}
+        for (g, lev, kind), ev in rep.correlators.items():
+            if kind == 0 and lev in (0, 2, 4):
+                corr[f"{g[0]}^3x{g[3]} stout{lev} C(1)/C(0)"] = float(ev.full[1] / ev.full[0])
+        return {"workload": "256 SU(3) chains (64 each of 4^3x10, 6^3x12, 8^3x16, 12^3x24), beta=5.7, "
+                            "68 sweeps from cold, A1++ on raw and stout 1-4, C(t) + jackknife",
+                "device_ms": rep.device_ms, "wall_s": wall,
+                "link_updates_per_s": links * rep.sweeps / (rep.device_ms * 1e-3),
+                "mean_plaquette_by_geometry": {f"{g[0]}^3x{g[3]}": float(np.mean(rep.plaquette[i * 64:(i + 1) * 64]))
+                                               for i, g in enumerate(rep.geometries)},
+                "mean_acceptance": float(np.mean(rep.acceptance)), "correlator_ratios": corr}
+    raise ValueError(name)
+
+
 def load_traffic(extent):
     path = os.path.join(REPO, "profiles", f"su3_metropolis_{extent}x4_traffic.json")
     if not os.path.exists(path):
@@ -166,6 +208,9 @@ def load_traffic(extent):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="su3",
+                    choices=["su3", "su2-precision", "su3-precision", "ensemble"],
+                    help="su3 (default): the SU(3) 32^4 sweep metric; the others print their own JSON line")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--extent", type=int, default=32)
@@ -188,6 +233,10 @@ def main():
         sys.exit(2)
 
     import ym_amd
+
+    if args.workload != "su3":
+        print(json.dumps(side_workload(args.workload)), flush=True)
+        return
 
     L = args.extent
     n = (L, L, L, L)

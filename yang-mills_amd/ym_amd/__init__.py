@@ -17,6 +17,7 @@ from . import stats  # noqa: F401
 from . import ensemble  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
 from . import learned  # noqa: F401
+from . import precision  # noqa: F401
 from .lattice import (  # noqa: F401
     Lattice4,
     Su3Lattice4,
