@@ -349,3 +349,20 @@ def test_fixtures_closed_forms(ym):
     a2[0:4] = [0.0, 1.0, 0.0, 0.0]
     su2.upload(a2)
     assert abs(su2.wilson_action(2.0) - 12.0) <= 1e-12
+
+
+def test_autotune_keeps_results_identical(ym):
+    """The tuned tile visiting order permutes independent tiles: the chain is
+    bit-identical to an untuned lattice with the same key."""
+    n = (8, 8, 32, 16)  # N2 * N3 / 2 = 256: whole tiles per (x0, x1), strips 0, 2, 4, 8
+    a = ym.Su3Lattice4(n, 32)
+    b = ym.Su3Lattice4(n, 32)
+    key = ym.derive_su3_chain_seed(7, 3)
+    a.hot_start(key)
+    b.hot_start(key)
+    best, cand = a.autotune(6.0, 0.35, 8, reps=2)
+    assert best in cand and sorted(cand) == [0, 2, 4, 8]
+    pa = a.metropolis_sweeps(6.0, 0.35, key, 0, 3, 8)
+    pb = b.metropolis_sweeps(6.0, 0.35, key, 0, 3, 8)
+    assert pa == pb
+    assert np.array_equal(a.download(), b.download())

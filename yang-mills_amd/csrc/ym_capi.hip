@@ -319,6 +319,57 @@ int32_t ym_su3_metropolis_sweeps(ym_lattice *l, double beta, double angle, int32
     return YM_OK;
 }
 
+int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, int32_t reps, int32_t *best_strip,
+                        float *us_per_subpass) {
+    if (!l || l->group != 3 || hits < 1 || reps < 1 || !best_strip) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(beta) || beta < 0.0 || beta > 12.0) return YM_BETA_OUT_OF_RANGE;
+    if (!std::isfinite(angle) || angle < 1.0e-8 || angle > 1.0) return YM_ANGLE_OUT_OF_RANGE;
+    if (!all_even(l)) return YM_UNSUPPORTED;
+    // Candidate x1-strip widths of the tile visiting order (0 = slot order).
+    // The order permutes independent tiles, so every candidate produces the
+    // same links; the sweeps run on a scratch copy of the field.
+    const int cand[5] = {0, 2, 4, 8, 16};
+    void *scratch = nullptr;
+    if (hipMalloc(&scratch, l->link_bytes) != hipSuccess) return YM_ALLOCATION_FAILED;
+    int32_t st = hip_status(hipMemcpyAsync(scratch, l->links, l->link_bytes, hipMemcpyDeviceToDevice, l->stream));
+    float best = 0.0f;
+    int best_t1 = l->g.t1;
+    for (int c = 0; c < 5 && !st; ++c) {
+        ym::Geom g = l->g;
+        g.t1 = cand[c];
+        const bool valid = cand[c] == 0 || (g.tk > 0 && g.n[1] % cand[c] == 0);
+        if (!valid) {
+            if (us_per_subpass) us_per_subpass[c] = -1.0f;
+            continue;
+        }
+        // one untimed sweep, then `reps` timed sweeps
+        for (int k = 0; k <= reps && !st; ++k) {
+            if (k == 1) st = hip_status(hipEventRecord(l->ev0, l->stream));
+            for (int mu = 0; mu < 4; ++mu)
+                for (int p = 0; p < 2; ++p)
+                    ym::launch_su3_metropolis(l->prec, scratch, g, mu, p, beta, angle, hits, 0x5EEDull, (uint64_t)k,
+                                              1, l->dctr, l->stream);
+        }
+        float ms = 0.0f;
+        if (!st) st = hip_status(hipEventRecord(l->ev1, l->stream));
+        if (!st) st = hip_status(hipEventSynchronize(l->ev1));
+        if (!st) st = hip_status(hipEventElapsedTime(&ms, l->ev0, l->ev1));
+        const float us = ms * 1e3f / (8.0f * (float)reps);
+        if (us_per_subpass) us_per_subpass[c] = us;
+        if (!st && (best == 0.0f || us < best)) {
+            best = us;
+            best_t1 = cand[c];
+        }
+    }
+    if (!st) st = finish(l);
+    (void)hipFree(scratch);
+    if (!st) {
+        l->g.t1 = best_t1;
+        *best_strip = best_t1;
+    }
+    return st;
+}
+
 int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *l, double beta, double angle, int32_t hits, uint64_t key,
                                           uint64_t first_sweep, int32_t n_sweeps, int32_t reunitarize,
                                           uint64_t *proposals, uint64_t *accepts, float *total_ms,

@@ -178,6 +178,15 @@ class Su3Lattice4(_Lattice):
         check(st, "ym_su3_metropolis_sweeps")
         return p.value, a.value
 
+    def autotune(self, beta=6.0, angle=0.35, hits=8, reps=3):
+        """Pick the fastest tile visiting order for this lattice.  Returns
+        (best x1-strip width, {width: us per sub-pass})."""
+        best = C.c_int32()
+        us = (C.c_float * 5)()
+        check(lib().ym_su3_autotune(self._h, float(beta), float(angle), int(hits), int(reps), C.byref(best), us),
+              "ym_su3_autotune")
+        return best.value, {w: us[i] for i, w in enumerate((0, 2, 4, 8, 16)) if us[i] >= 0}
+
     def metropolis_sweeps_profiled(self, beta, angle, key, first_sweep, n_sweeps, hits=8, reunitarize=True):
         """Same sweeps with HIP events around every sub-pass launch.
 
