@@ -249,7 +249,7 @@ def main():
     if args.hot_start:
         lat.hot_start(key)
     best, cand = lat.autotune(args.beta, angle, args.hits)
-    log(f"autotune: tile strip {best} ({', '.join(f'{w}: {t:.1f} us' for w, t in cand.items())})")
+    log(f"autotune: {best} ({', '.join(f'{m}/{w}: {t:.1f} us' for (m, w), t in cand.items())})")
     log(f"warmup {args.warmup} sweeps")
     lat.metropolis_sweeps(args.beta, angle, key, 0, args.warmup, args.hits)
     log(f"timed {args.steps} sweeps")
@@ -312,7 +312,7 @@ def main():
             "unitarity_residual_fp32_storage": unit_res,
             "determinant_residual_fp32_storage": det_res,
             "device_time_ms_per_step": total_ms / args.steps,
-            "autotuned_tile_strip": best,
+            "autotuned_tile_mapping": {"mode": best[0], "strip": best[1]},
         },
         "roofline": {
             "bound": "hbm",

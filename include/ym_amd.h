@@ -120,15 +120,16 @@ int32_t ym_su3_metropolis_sweeps(ym_lattice *lat, double beta, double proposal_a
                                  uint64_t first_sweep, int32_t n_sweeps, int32_t reunitarize, uint64_t *proposals,
                                  uint64_t *accepts);
 /* Autotuner (no reference counterpart): times the Metropolis sub-pass for
- * the candidate x1-strip widths {0 (slot order), 2, 4, 8, 16} of the tile
- * visiting order on a scratch copy of the links (reps timed sweeps each,
- * after one untimed), keeps the fastest for this lattice and returns it in
- * *best_strip; us_per_subpass (5 entries, may be NULL) gets the mean time per
- * sub-pass, -1 for a width that does not divide N1 or a geometry without
- * whole tiles per (x0, x1).  The visiting order permutes independent tiles,
- * so results do not depend on it. */
+ * nine thread-to-link mappings -- row tiles (256 consecutive slots) visited
+ * in x1-strips of width {0 = slot order, 2, 4, 8, 16}, and compact tiles
+ * (2 x 2 (x0, x1) rows x 64 slots) in X1-strips of {1, 2, 4, 8} -- on a
+ * scratch copy of the links (reps timed sweeps each, after one untimed),
+ * keeps the fastest for this lattice and returns it in *best_mode (0 row, 1
+ * compact) and *best_strip; us_per_candidate (9 entries, may be NULL) gets
+ * the mean time per sub-pass, -1 where a mapping does not fit the extents.
+ * Every mapping permutes independent links, so results do not depend on it. */
 int32_t ym_su3_autotune(ym_lattice *lat, double beta, double proposal_angle, int32_t hits, int32_t reps,
-                        int32_t *best_strip, float *us_per_subpass);
+                        int32_t *best_mode, int32_t *best_strip, float *us_per_candidate);
 /* Profiling variant of ym_su3_metropolis_sweeps (same arithmetic and
  * results): HIP events on the handle's stream around every sub-pass launch.
  * *total_ms = first start to last stop; *kernel_ms_sum = sum of the

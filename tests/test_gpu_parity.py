@@ -114,6 +114,26 @@ def test_su3_metropolis_matches_replay_bitwise(ym, prec, n, beta, hits):
 
 
 @pytest.mark.parametrize("prec", [32, 64])
+def test_su3_metropolis_wave_specialised_path_bitwise(ym, prec):
+    """Lattices of >= 256 tiles take the wave-specialised sub-pass kernel
+    (staples of the next tile overlapping the hits of the current one); one
+    sweep of 16^3x32 against the device-order replay."""
+    n = (16, 16, 16, 32)
+    key = ym.derive_su3_chain_seed(0x2c02198105010002, 2)
+    lat = ym.Su3Lattice4(n, prec)
+    lat.hot_start(key ^ 0x77)
+    buf = dev_buffer(n, 18, prec)
+    assert O.lib().ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, key ^ 0x77) == 0
+    gp, ga = lat.metropolis_sweeps(6.0, 0.35, key, 5, 1, 8)
+    params = O.DevSu3Params(beta=6.0, angle=0.35, hits=8, reunitarize=1, key=key, sweep=5)
+    cp, ca = C.c_uint64(), C.c_uint64()
+    assert O.lib().ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, C.byref(params),
+                                           C.byref(cp), C.byref(ca)) == 0
+    assert gp == cp.value and ga == ca.value
+    assert np.array_equal(lat.download(), dev_to_aos(n, buf, prec, 18))
+
+
+@pytest.mark.parametrize("prec", [32, 64])
 def test_su2_heatbath_overrelax_match_replay_bitwise(ym, prec):
     n = (4, 4, 2, 6)
     key = ym.derive_chain_seed(0x2b00201211160101, 0)
@@ -361,7 +381,9 @@ def test_autotune_keeps_results_identical(ym):
     a.hot_start(key)
     b.hot_start(key)
     best, cand = a.autotune(6.0, 0.35, 8, reps=2)
-    assert best in cand and sorted(cand) == [0, 2, 4, 8]
+    assert best in cand
+    assert sorted(cand) == [("compact", 1), ("compact", 2), ("compact", 4), ("row", 0), ("row", 2), ("row", 4),
+                            ("row", 8)]
     pa = a.metropolis_sweeps(6.0, 0.35, key, 0, 3, 8)
     pb = b.metropolis_sweeps(6.0, 0.35, key, 0, 3, 8)
     assert pa == pb

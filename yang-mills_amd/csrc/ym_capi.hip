@@ -319,27 +319,31 @@ int32_t ym_su3_metropolis_sweeps(ym_lattice *l, double beta, double angle, int32
     return YM_OK;
 }
 
-int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, int32_t reps, int32_t *best_strip,
-                        float *us_per_subpass) {
-    if (!l || l->group != 3 || hits < 1 || reps < 1 || !best_strip) return YM_INVALID_ARGUMENT;
+int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, int32_t reps, int32_t *best_mode,
+                        int32_t *best_strip, float *us_per_candidate) {
+    if (!l || l->group != 3 || hits < 1 || reps < 1 || !best_mode || !best_strip) return YM_INVALID_ARGUMENT;
     if (!std::isfinite(beta) || beta < 0.0 || beta > 12.0) return YM_BETA_OUT_OF_RANGE;
     if (!std::isfinite(angle) || angle < 1.0e-8 || angle > 1.0) return YM_ANGLE_OUT_OF_RANGE;
     if (!all_even(l)) return YM_UNSUPPORTED;
-    // Candidate x1-strip widths of the tile visiting order (0 = slot order).
-    // The order permutes independent tiles, so every candidate produces the
-    // same links; the sweeps run on a scratch copy of the field.
-    const int cand[5] = {0, 2, 4, 8, 16};
+    // Candidates: row tiles with x1-strip widths {0 (slot order), 2, 4, 8,
+    // 16}, compact 2 x 2 tiles with X1-strip widths {1, 2, 4, 8}.  Every
+    // candidate permutes independent links, so all produce the same links;
+    // the sweeps run on a scratch copy of the field.
+    const int mode[9] = {0, 0, 0, 0, 0, 1, 1, 1, 1};
+    const int strip[9] = {0, 2, 4, 8, 16, 1, 2, 4, 8};
     void *scratch = nullptr;
     if (hipMalloc(&scratch, l->link_bytes) != hipSuccess) return YM_ALLOCATION_FAILED;
     int32_t st = hip_status(hipMemcpyAsync(scratch, l->links, l->link_bytes, hipMemcpyDeviceToDevice, l->stream));
     float best = 0.0f;
-    int best_t1 = l->g.t1;
-    for (int c = 0; c < 5 && !st; ++c) {
+    int bm = l->g.tmode, bs = l->g.t1;
+    for (int c = 0; c < 9 && !st; ++c) {
         ym::Geom g = l->g;
-        g.t1 = cand[c];
-        const bool valid = cand[c] == 0 || (g.tk > 0 && g.n[1] % cand[c] == 0);
+        g.tmode = mode[c];
+        g.t1 = strip[c];
+        const bool valid = mode[c] == 0 ? (strip[c] == 0 || (g.tk > 0 && g.n[1] % strip[c] == 0))
+                                        : (g.tq > 0 && (g.n[1] / 2) % strip[c] == 0);
         if (!valid) {
-            if (us_per_subpass) us_per_subpass[c] = -1.0f;
+            if (us_per_candidate) us_per_candidate[c] = -1.0f;
             continue;
         }
         // one untimed sweep, then `reps` timed sweeps
@@ -355,17 +359,20 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
         if (!st) st = hip_status(hipEventSynchronize(l->ev1));
         if (!st) st = hip_status(hipEventElapsedTime(&ms, l->ev0, l->ev1));
         const float us = ms * 1e3f / (8.0f * (float)reps);
-        if (us_per_subpass) us_per_subpass[c] = us;
+        if (us_per_candidate) us_per_candidate[c] = us;
         if (!st && (best == 0.0f || us < best)) {
             best = us;
-            best_t1 = cand[c];
+            bm = mode[c];
+            bs = strip[c];
         }
     }
     if (!st) st = finish(l);
     (void)hipFree(scratch);
     if (!st) {
-        l->g.t1 = best_t1;
-        *best_strip = best_t1;
+        l->g.tmode = bm;
+        l->g.t1 = bs;
+        *best_mode = bm;
+        *best_strip = bs;
     }
     return st;
 }

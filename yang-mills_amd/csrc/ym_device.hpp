@@ -184,6 +184,8 @@ struct Geom {
     int volume;
     int tk;    // 256-slot tiles per (x0, x1) pair, 0 when N2 * half is not a multiple of 256
     int t1;    // x1 strip width of the tile visiting order, 0 = plain order
+    int tq;    // 64-slot quarters per (x0, x1) pair for compact tiles, 0 when not available
+    int tmode; // 0: a tile is 256 consecutive slots; 1: a tile is 2 x 2 (x0, x1) x 64 slots
 };
 
 // Geometry of an N0 x N1 x N2 x N3 lattice (extents already validated).
@@ -197,6 +199,8 @@ __host__ inline Geom make_geom(const int64_t n[4]) {
     const int row_slots = g.n[2] * g.half;
     g.tk = (row_slots % 256 == 0) ? row_slots / 256 : 0;
     g.t1 = (g.tk > 0 && g.n[1] % 4 == 0) ? 4 : 0;
+    g.tq = (row_slots % 64 == 0 && g.n[0] % 2 == 0 && g.n[1] % 2 == 0) ? row_slots / 64 : 0;
+    g.tmode = 0;
     return g;
 }
 
@@ -252,6 +256,27 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     int xcd = b & 7, idx = b >> 3;
     int q = nb >> 3, r = nb & 7;
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// Slot of thread tid (0..255) of tile t.  tmode 0: tile_order(t) * 256 +
+// tid.  tmode 1 (compact tiles): tile t covers x0 in {2 X0, 2 X0 + 1},
+// x1 in {2 X1, 2 X1 + 1} and one 64-slot quarter j of the (x0, x1) rows, so
+// half of its x0- and x1-neighbours are inside the tile; tiles are visited
+// in strips of t1 values of X1 (t1 = 0 means 1) with X0 running inside a
+// strip.  Both are permutations of the Vh slots.
+__host__ __device__ __forceinline__ int tile_slot(const Geom &g, int t, int tid) {
+    if (g.tmode == 1) {
+        const int j = t % g.tq;
+        int r = t / g.tq;
+        const int strip = g.t1 > 0 ? g.t1 : 1;
+        const int x1i = r % strip;
+        r /= strip;
+        const int h0 = g.n[0] >> 1;
+        const int X0 = r % h0, X1 = (r / h0) * strip + x1i;
+        const int a = tid >> 7, b = (tid >> 6) & 1;
+        return ((2 * X0 + a) * g.n[1] + 2 * X1 + b) * (g.n[2] * g.half) + 64 * j + (tid & 63);
+    }
+    return tile_order(g, t) * 256 + tid;
 }
 
 // ------------------------------------------------------------ SU(3) types

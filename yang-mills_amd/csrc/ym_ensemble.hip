@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_ens_metropolis(typename Vec2<Real>::typ
     const int c = chain_of(begin, nch, b);
     const Geom g = cd[c].g;
     typename Vec2<Real>::type *L = links + cd[c].link_off;
-    const int s = tile_order(g, b - begin[c]) * 256 + threadIdx.x;
+    const int s = tile_slot(g, b - begin[c], threadIdx.x);
     unsigned long long nacc = 0;
     int x[4];
     if (s < g.vh && site_of(g, p, s, x)) {

@@ -317,8 +317,7 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
     else
         for (int i = threadIdx.x; i < kParams; i += 256) sh.P[i] = params[i];
     __syncthreads();
-    const int tile = tile_order(g, xcd_tile(blockIdx.x, gridDim.x));
-    const int s = tile * 256 + threadIdx.x;
+    const int s = tile_slot(g, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
     unsigned long long tried = 0, accepted = 0, macc = 0;
     int x[4] = {0, 0, 0, 0};
     const bool live = s < g.vh && site_of(g, p, s, x);

@@ -27,8 +27,7 @@ __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::typ
                                                         uint64_t sweep, int reunit,
                                                         unsigned long long *__restrict__ acc_out) {
     __shared__ unsigned long long red[4];
-    const int tile = tile_order(g, xcd_tile(blockIdx.x, gridDim.x));
-    const int s = tile * 256 + threadIdx.x;
+    const int s = tile_slot(g, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
     unsigned long long nacc = 0;
     int x[4];
     if (s < g.vh && site_of(g, p, s, x)) {
@@ -40,6 +39,71 @@ __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::typ
     if (threadIdx.x == 0 && tot) atomicAdd(acc_out, tot);
 }
 
+// Wave-specialised sub-pass for lattices of at least 256 tiles: one block of
+// 8 wavefronts per CU walks a sequence of 256-slot tiles; waves 0-3 compute
+// the staples of tile i+1 into LDS while waves 4-7 run the FP64 hits of tile
+// i from the staples of the previous step, so the neighbour loads of one
+// tile overlap the accept/reject arithmetic of the other.  Per link the
+// arithmetic is that of k_su3_metropolis, so results are identical.  Tiles
+// are split into 8 contiguous ranges, one per XCD (blocks are dealt
+// round-robin over the XCDs), and the blocks of an XCD advance through its
+// range side by side, so the tiles in flight on an XCD are neighbours.  The
+// producers also stage the link being updated, so the hit waves touch global
+// memory only for the final store.
+template <typename Real, int MU>
+__global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::type *__restrict__ L, Geom g, int p,
+                                                           double nbt, double angle, int hits, uint64_t key,
+                                                           uint64_t sweep, int reunit, int ntiles,
+                                                           unsigned long long *__restrict__ acc_out) {
+    __shared__ Real abuf[2][18][256];
+    __shared__ Real ubuf[2][18][256];
+    __shared__ unsigned long long red[8];
+    const bool producer = threadIdx.x < 256;
+    const int lane = threadIdx.x & 255;
+    const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+    const int nbx = ((int)gridDim.x - 1 - xcd) / 8 + 1;
+    const int t0 = (int)(((int64_t)ntiles * xcd) / 8), t1 = (int)(((int64_t)ntiles * (xcd + 1)) / 8);
+    const int nsteps = (t1 - t0 - j + nbx - 1) / nbx;
+    unsigned long long nacc = 0;
+    // producer: staple of the link in slot `lane` of tile i into buffer i & 1
+    auto produce = [&](int i) {
+        const int s = tile_slot(g, t0 + j + i * nbx, lane);
+        int x[4];
+        if (s < g.vh && site_of(g, p, s, x)) {
+            M3<Real> A;
+            su3_staple_at<Real, MU>(L, g, p, s, x, A);
+            const M3<Real> U = load3<Real>(L, g.vh, MU, p, s);
+#pragma unroll
+            for (int c = 0; c < 18; ++c) {
+                abuf[i & 1][c][lane] = A.v[c];
+                ubuf[i & 1][c][lane] = U.v[c];
+            }
+        }
+    };
+    if (producer && nsteps > 0) produce(0);
+    __syncthreads();
+    for (int i = 0; i < nsteps; ++i) {
+        if (producer) {
+            if (i + 1 < nsteps) produce(i + 1);
+        } else {
+            const int s = tile_slot(g, t0 + j + i * nbx, lane);
+            int x[4];
+            if (s < g.vh && site_of(g, p, s, x)) {
+                M3<Real> A, U;
+#pragma unroll
+                for (int c = 0; c < 18; ++c) {
+                    A.v[c] = abuf[i & 1][c][lane];
+                    U.v[c] = ubuf[i & 1][c][lane];
+                }
+                nacc += su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, key, sweep, reunit);
+            }
+        }
+        __syncthreads();
+    }
+    const unsigned long long tot = block_sum_u64<512>(nacc, red);
+    if (threadIdx.x == 0 && tot) atomicAdd(acc_out, tot);
+}
+
 template <typename Real>
 static void launch_metro_t(void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
                            uint64_t key, uint64_t sweep, int reunit, unsigned long long *acc, hipStream_t st) {
@@ -47,6 +111,17 @@ static void launch_metro_t(void *links, const Geom &g, int mu, int p, double bet
     const double nbt = -(beta / 3.0);
     const int nb = (g.vh + 255) / 256;
     V2 *L = (V2 *)links;
+    if (nb >= 256) {
+        // one 8-wave block per CU
+        const int grid = 256;
+        switch (mu) {
+        case 0: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 0>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, acc); break;
+        case 1: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 1>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, acc); break;
+        case 2: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 2>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, acc); break;
+        default: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 3>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, acc); break;
+        }
+        return;
+    }
     switch (mu) {
     case 0: hipLaunchKernelGGL((k_su3_metropolis<Real, 0>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
     case 1: hipLaunchKernelGGL((k_su3_metropolis<Real, 1>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;

@@ -106,17 +106,32 @@ __device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *_
 // proposals, optional FP64 reunitarisation, one rounding to storage.
 // Returns the number of accepted proposals.
 template <typename Real, int MU>
+__device__ __forceinline__ unsigned long long su3_update_from(typename Vec2<Real>::type *__restrict__ L,
+                                                              const Geom &g, int p, int s, const int x[4],
+                                                              const M3<Real> &Uf, const M3<Real> &A, double nbt,
+                                                              double angle, int hits, uint64_t key, uint64_t sweep,
+                                                              int reunit);
+
+template <typename Real, int MU>
 __device__ __forceinline__ unsigned long long su3_update_at(typename Vec2<Real>::type *__restrict__ L, const Geom &g,
                                                             int p, int s, const int x[4], const M3<Real> &A,
                                                             double nbt, double angle, int hits, uint64_t key,
                                                             uint64_t sweep, int reunit) {
+    const M3<Real> Uf = load3<Real>(L, g.vh, MU, p, s);
+    return su3_update_from<Real, MU>(L, g, p, s, x, Uf, A, nbt, angle, hits, key, sweep, reunit);
+}
+
+// As su3_update_at with the stored link Uf already loaded.
+template <typename Real, int MU>
+__device__ __forceinline__ unsigned long long su3_update_from(typename Vec2<Real>::type *__restrict__ L,
+                                                              const Geom &g, int p, int s, const int x[4],
+                                                              const M3<Real> &Uf, const M3<Real> &A, double nbt,
+                                                              double angle, int hits, uint64_t key, uint64_t sweep,
+                                                              int reunit) {
     unsigned long long nacc = 0;
     M3<double> u, M;
-    {
-        const M3<Real> Uf = load3<Real>(L, g.vh, MU, p, s);
 #pragma unroll
-        for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
-    }
+    for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -62,7 +62,7 @@ SIGNATURES = {
     "ym_su3_metropolis_sweeps": (i32, [C.c_void_p, dbl, dbl, i32, u64, u64, i32, i32, P(u64), P(u64)]),
     "ym_su3_metropolis_sweeps_profiled": (i32, [C.c_void_p, dbl, dbl, i32, u64, u64, i32, i32, P(u64), P(u64),
                                                 P(flt), P(flt), P(i32)]),
-    "ym_su3_autotune": (i32, [C.c_void_p, dbl, dbl, i32, i32, P(i32), P(flt)]),
+    "ym_su3_autotune": (i32, [C.c_void_p, dbl, dbl, i32, i32, P(i32), P(i32), P(flt)]),
     "ym_su2_heatbath_sweeps": (i32, [C.c_void_p, dbl, u64, u64, i32, i32, P(u64), P(u64)]),
     "ym_su2_overrelax_sweeps": (i32, [C.c_void_p, i32]),
     "ym_average_plaquette": (i32, [C.c_void_p, P(dbl)]),

@@ -178,14 +178,19 @@ class Su3Lattice4(_Lattice):
         check(st, "ym_su3_metropolis_sweeps")
         return p.value, a.value
 
+    AUTOTUNE_CANDIDATES = (("row", 0), ("row", 2), ("row", 4), ("row", 8), ("row", 16),
+                           ("compact", 1), ("compact", 2), ("compact", 4), ("compact", 8))
+
     def autotune(self, beta=6.0, angle=0.35, hits=8, reps=3):
-        """Pick the fastest tile visiting order for this lattice.  Returns
-        (best x1-strip width, {width: us per sub-pass})."""
-        best = C.c_int32()
-        us = (C.c_float * 5)()
-        check(lib().ym_su3_autotune(self._h, float(beta), float(angle), int(hits), int(reps), C.byref(best), us),
-              "ym_su3_autotune")
-        return best.value, {w: us[i] for i, w in enumerate((0, 2, 4, 8, 16)) if us[i] >= 0}
+        """Pick the fastest thread-to-link mapping of the Metropolis sub-pass
+        for this lattice.  Returns ((mode, strip), {(mode, strip): us per
+        sub-pass})."""
+        mode, strip = C.c_int32(), C.c_int32()
+        us = (C.c_float * 9)()
+        check(lib().ym_su3_autotune(self._h, float(beta), float(angle), int(hits), int(reps), C.byref(mode),
+                                    C.byref(strip), us), "ym_su3_autotune")
+        best = (("row", "compact")[mode.value], strip.value)
+        return best, {c: us[i] for i, c in enumerate(self.AUTOTUNE_CANDIDATES) if us[i] >= 0}
 
     def metropolis_sweeps_profiled(self, beta, angle, key, first_sweep, n_sweeps, hits=8, reunitarize=True):
         """Same sweeps with HIP events around every sub-pass launch.
