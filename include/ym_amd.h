@@ -41,6 +41,8 @@ enum {
     YM_NON_FINITE_BETA = 6,
     YM_NEGATIVE_BETA = 7,
     YM_NON_FINITE_ACTION = 8,
+    YM_EMPTY_PATH = 9,                /* PathHolonomyError::EmptyPath */
+    YM_PATH_TOO_LONG = 10,            /* PathHolonomyError::PathTooLong */
     YM_ZERO_RECTANGLE_EXTENT = 12,
     YM_COORDINATE_OUT_OF_RANGE = 17,  /* ReferenceError::CoordinateOutOfRange */
     YM_BETA_OUT_OF_RANGE = 30,        /* Su3EnsembleError / EnsembleError::BetaOutsideReferenceRange */
@@ -166,6 +168,14 @@ int32_t ym_staple(const ym_lattice *lat, const int64_t x[4], int32_t mu, double 
  * nu.  plane_set 0: all 12 ordered (mu, nu); 1: mu spatial, nu = 3 (time);
  * 2: the 6 ordered spatial pairs.  re/im: rmax*tmax doubles, index
  * (R-1)*tmax + (T-1); im is 0 for SU(2). */
+/* path_holonomy.rs:132-170 (SU(3)), lib.rs:384-436 (SU(2)): ordered product
+ * along a path from `start`; steps[k] = +(d+1) multiplies by U_d(x) and
+ * moves to x + d, -(d+1) moves to x - d and multiplies by U_d(x-d)^dagger.
+ * FP64 product of the stored links; out gets 18 (SU(3)) or 4 (SU(2))
+ * doubles in the host layout, end (may be NULL) the end point.
+ * 1 <= nsteps <= 2^20. */
+int32_t ym_path_holonomy(const ym_lattice *lat, const int64_t start[4], const int32_t *steps, int64_t nsteps,
+                         double *out, int64_t end[4]);
 int32_t ym_wilson_loops(const ym_lattice *lat, int32_t rmax, int32_t tmax, int32_t plane_set, double *re, double *im);
 /* Polyakov loop averaged over all starting sites of the hyperplane
  * x_dir = 0 (lib.rs:483-505 SU(2) (1/2)Re Tr, su3_lattice.rs:185-212 SU(3)

@@ -271,6 +271,74 @@ def test_wilson_loops_fp64_vs_reference(ym, group):
 
 
 @pytest.mark.parametrize("group", [3, 2])
+def test_path_holonomy_fp64_vs_reference(ym, group):
+    """Open and closed paths from several starts against the oracle's
+    ordered product (path_holonomy.rs:132-157, lib.rs:384-416); the closed
+    r x t rectangle must reproduce the Wilson loop of the same plane, and a
+    path that walks out and back must give the identity."""
+    n = (4, 3, 5, 6)
+    lat, aos = hot_fp64(ym, n, 47, group)
+    rng = np.random.default_rng(5)
+    ncomp = 18 if group == 3 else 4
+    fn = O.lib().ymo_su3_path_holonomy if group == 3 else O.lib().ymo_su2_path_holonomy
+    cases = [(0, 0, 0, 0), (3, 2, 4, 5), (1, 0, 2, 3)]
+    for start in cases:
+        for length in (1, 7, 40):
+            steps = rng.choice([1, 2, 3, 4, -1, -2, -3, -4], size=length).astype(np.int32)
+            got, end = lat.path_holonomy(start, steps)
+            ref = np.zeros(ncomp)
+            rend = (C.c_int64 * 4)()
+            assert fn(O.ext(n), O.dptr(aos), (C.c_int64 * 4)(*start),
+                      steps.ctypes.data_as(C.POINTER(C.c_int32)), length, O.dptr(ref), rend) == 0
+            assert_rel(got, ref)
+            assert end == tuple(rend)
+            # closed-form end point: start + net displacement, periodic
+            disp = np.zeros(4, np.int64)
+            for s in steps:
+                disp[abs(s) - 1] += 1 if s > 0 else -1
+            assert end == tuple(int(v) for v in (np.asarray(start) + disp) % np.asarray(n))
+    # out and back: P followed by reverse(-P) is the identity
+    steps = np.array([1, 2, 2, 4, -3, 1], np.int32)
+    back = np.concatenate([steps, -steps[::-1]])
+    got, end = lat.path_holonomy((2, 1, 3, 0), back)
+    ident = np.zeros(ncomp)
+    if group == 3:
+        ident[[0, 8, 16]] = 1.0
+    else:
+        ident[0] = 1.0
+    assert_rel(got, ident, tol=1e-12)
+    assert end == (2, 1, 3, 0)
+    # closed 2x1 rectangle in plane (0, 3) from one site = that site's Wilson loop
+    s = (1, 2, 0, 4)
+    rect = np.array([1, 1, 4, -1, -1, -4], np.int32)
+    got, _ = lat.path_holonomy(s, rect)
+    if group == 3:
+        a, b = C.c_double(), C.c_double()
+        assert O.lib().ymo_su3_rect_wilson_loop(O.ext(n), O.dptr(aos), (C.c_int64 * 4)(*s), 0, 3, 2, 1,
+                                                C.byref(a), C.byref(b)) == 0
+        tr = complex(got[0] + got[8] + got[16], got[1] + got[9] + got[17]) / 3
+        assert abs(tr - complex(a.value, b.value)) <= REL
+    else:
+        a = C.c_double()
+        assert O.lib().ymo_su2_rect_wilson_loop(O.ext(n), O.dptr(aos), (C.c_int64 * 4)(*s), 0, 3, 2, 1,
+                                                C.byref(a)) == 0
+        assert abs(got[0] - a.value) <= REL
+    # the reference's error variants
+    with pytest.raises(ym.YmError) as e:
+        lat.path_holonomy((0, 0, 0, 0), np.zeros(0, np.int32))
+    assert e.value.status == 9
+    with pytest.raises(ym.YmError) as e:
+        lat.path_holonomy((0, 0, 0, 0), np.array([1, 5], np.int32))
+    assert e.value.status == 4
+    with pytest.raises(ym.YmError) as e:
+        lat.path_holonomy((4, 0, 0, 0), np.array([1], np.int32))
+    assert e.value.status == 17
+    with pytest.raises(ym.YmError) as e:
+        lat.path_holonomy((0, 0, 0, 0), np.ones((1 << 20) + 1, np.int32))
+    assert e.value.status == 10
+
+
+@pytest.mark.parametrize("group", [3, 2])
 def test_polyakov_fp64_vs_reference(ym, group):
     n = (4, 2, 4, 6)
     lat, aos = hot_fp64(ym, n, 12, group)

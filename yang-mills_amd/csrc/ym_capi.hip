@@ -126,6 +126,8 @@ const char *ym_status_string(int32_t s) {
     case YM_NEGATIVE_BETA: return "beta must be non-negative";
     case YM_NON_FINITE_ACTION: return "Wilson action evaluation produced a non-finite result";
     case YM_ZERO_RECTANGLE_EXTENT: return "rectangular Wilson-loop extents must be positive";
+    case YM_EMPTY_PATH: return "path is empty";
+    case YM_PATH_TOO_LONG: return "path exceeds the bounded length";
     case YM_COORDINATE_OUT_OF_RANGE: return "lattice coordinate is outside its periodic fundamental domain";
     case YM_BETA_OUT_OF_RANGE: return "beta is outside the bounded reference range";
     case YM_ANGLE_OUT_OF_RANGE: return "Metropolis proposal angle is outside the bounded reference range";
@@ -502,6 +504,43 @@ int32_t ym_staple(const ym_lattice *lc, const int64_t x[4], int32_t mu, double *
     YM_HIP(hipGetLastError());
     YM_HIP(hipMemcpyAsync(out, l->dres, sizeof(double) * l->ncomp, hipMemcpyDeviceToHost, l->stream));
     return finish(l);
+}
+
+int32_t ym_path_holonomy(const ym_lattice *lc, const int64_t start[4], const int32_t *steps, int64_t nsteps,
+                         double *out, int64_t end[4]) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !start || !out || (nsteps > 0 && !steps)) return YM_INVALID_ARGUMENT;
+    if (nsteps <= 0) return YM_EMPTY_PATH;
+    if (nsteps > (1 << 20)) return YM_PATH_TOO_LONG;
+    int xi[4];
+    for (int d = 0; d < 4; ++d) {
+        if (start[d] < 0 || start[d] >= l->n[d]) return YM_COORDINATE_OUT_OF_RANGE;
+        xi[d] = (int)start[d];
+    }
+    for (int64_t k = 0; k < nsteps; ++k)
+        if (steps[k] == 0 || steps[k] > 4 || steps[k] < -4) return YM_DIRECTION_OUT_OF_RANGE;
+    int *dsteps = nullptr;
+    double *dout = nullptr;
+    if (hipMalloc((void **)&dsteps, sizeof(int) * nsteps) != hipSuccess) return YM_ALLOCATION_FAILED;
+    if (hipMalloc((void **)&dout, sizeof(double) * 22) != hipSuccess) {
+        (void)hipFree(dsteps);
+        return YM_ALLOCATION_FAILED;
+    }
+    int32_t st = hip_status(hipMemcpyAsync(dsteps, steps, sizeof(int) * nsteps, hipMemcpyHostToDevice, l->stream));
+    if (!st) {
+        ym::launch_path_holonomy(l->group, l->prec, l->links, l->g, xi, dsteps, (int)nsteps, dout, l->stream);
+        st = hip_status(hipGetLastError());
+    }
+    double h[22];
+    if (!st) st = hip_status(hipMemcpyAsync(h, dout, sizeof(h), hipMemcpyDeviceToHost, l->stream));
+    if (!st) st = finish(l);
+    (void)hipFree(dsteps);
+    (void)hipFree(dout);
+    if (st) return st;
+    for (int e = 0; e < l->ncomp; ++e) out[e] = h[e];
+    if (end)
+        for (int d = 0; d < 4; ++d) end[d] = (int64_t)h[l->ncomp + d];
+    return YM_OK;
 }
 
 int32_t ym_wilson_loops(const ym_lattice *lc, int32_t rmax, int32_t tmax, int32_t plane_set, double *re, double *im) {

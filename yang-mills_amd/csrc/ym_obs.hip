@@ -154,6 +154,76 @@ __global__ __launch_bounds__(256) void k_polyakov(const typename G::Store *__res
         part[2 * gridDim.x + blockIdx.x] = a2;
     }
 }
+// ------------------------------------------------------- path holonomy
+// One thread walks the path (path_holonomy.rs:132-170, lib.rs:384-436): step
+// +(d+1) multiplies by U_d(x) and moves to x + d; step -(d+1) moves to
+// x - d and multiplies by U_d(x - d)^dagger; ordered left to right in FP64.
+// out: 18 (SU(3)) or 4 (SU(2)) doubles, then the end point (4 doubles).
+template <typename Real, int GROUP>
+__global__ void k_path_holonomy(const void *__restrict__ links, Geom g, int x0, int x1, int x2, int x3,
+                                const int *__restrict__ steps, int nsteps, double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int c[4] = {x0, x1, x2, x3}, y[4];
+    if constexpr (GROUP == 3) {
+        const auto *L = (const typename Vec2<Real>::type *)links;
+        M3<double> h = identity3<double>();
+        for (int k = 0; k < nsteps; ++k) {
+            const int code = steps[k];
+            const int d = (code > 0 ? code : -code) - 1;
+            M3<double> seg;
+            if (code > 0) {
+                const M3<Real> u = load3<Real>(L, g.vh, d, parity_of(c), slot_of(g, c));
+                for (int e = 0; e < 18; ++e) seg.v[e] = (double)u.v[e];
+                shifted(g, c, d, +1, y);
+            } else {
+                shifted(g, c, d, -1, y);
+                const M3<Real> u = load3<Real>(L, g.vh, d, parity_of(y), slot_of(g, y));
+                M3<double> ud;
+                for (int e = 0; e < 18; ++e) ud.v[e] = (double)u.v[e];
+                seg = dagger3(ud);
+            }
+            h = mul_nn(h, seg);
+            for (int e = 0; e < 4; ++e) c[e] = y[e];
+        }
+        for (int e = 0; e < 18; ++e) out[e] = h.v[e];
+        for (int e = 0; e < 4; ++e) out[18 + e] = c[e];
+    } else {
+        const auto *L = (const typename Vec4<Real>::type *)links;
+        Q4<double> h;
+        h.c[0] = 1.0; h.c[1] = 0.0; h.c[2] = 0.0; h.c[3] = 0.0;
+        for (int k = 0; k < nsteps; ++k) {
+            const int code = steps[k];
+            const int d = (code > 0 ? code : -code) - 1;
+            Q4<double> seg;
+            if (code > 0) {
+                const Q4<Real> u = load2<Real>(L, g.vh, d, parity_of(c), slot_of(g, c));
+                for (int e = 0; e < 4; ++e) seg.c[e] = (double)u.c[e];
+                shifted(g, c, d, +1, y);
+            } else {
+                shifted(g, c, d, -1, y);
+                const Q4<Real> u = load2<Real>(L, g.vh, d, parity_of(y), slot_of(g, y));
+                Q4<double> ud;
+                for (int e = 0; e < 4; ++e) ud.c[e] = (double)u.c[e];
+                seg = qdag(ud);
+            }
+            h = qmul(h, seg);
+            for (int e = 0; e < 4; ++e) c[e] = y[e];
+        }
+        for (int e = 0; e < 4; ++e) out[e] = h.c[e];
+        for (int e = 0; e < 4; ++e) out[4 + e] = c[e];
+    }
+}
+void launch_path_holonomy(int group, int prec, const void *links, const Geom &g, const int x[4], const int *steps,
+                          int nsteps, double *out, hipStream_t st) {
+    if (group == 3) {
+        if (prec == 32) hipLaunchKernelGGL((k_path_holonomy<float, 3>), dim3(1), dim3(64), 0, st, links, g, x[0], x[1], x[2], x[3], steps, nsteps, out);
+        else hipLaunchKernelGGL((k_path_holonomy<double, 3>), dim3(1), dim3(64), 0, st, links, g, x[0], x[1], x[2], x[3], steps, nsteps, out);
+    } else {
+        if (prec == 32) hipLaunchKernelGGL((k_path_holonomy<float, 2>), dim3(1), dim3(64), 0, st, links, g, x[0], x[1], x[2], x[3], steps, nsteps, out);
+        else hipLaunchKernelGGL((k_path_holonomy<double, 2>), dim3(1), dim3(64), 0, st, links, g, x[0], x[1], x[2], x[3], steps, nsteps, out);
+    }
+}
+
 int launch_polyakov(int group, int prec, const void *links, const Geom &g, int dir, double *part, hipStream_t st) {
     const int nbase = g.volume / g.n[dir];
     dim3 grid((nbase + 255) / 256);

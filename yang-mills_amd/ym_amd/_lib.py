@@ -68,6 +68,7 @@ SIGNATURES = {
     "ym_average_plaquette": (i32, [C.c_void_p, P(dbl)]),
     "ym_wilson_action": (i32, [C.c_void_p, dbl, P(dbl)]),
     "ym_staple": (i32, [C.c_void_p, P(i64), i32, P(dbl)]),
+    "ym_path_holonomy": (i32, [C.c_void_p, P(i64), P(i32), i64, P(dbl), P(i64)]),
     "ym_wilson_loops": (i32, [C.c_void_p, i32, i32, i32, P(dbl), P(dbl)]),
     "ym_polyakov_loop": (i32, [C.c_void_p, i32, P(dbl), P(dbl), P(dbl)]),
     "ym_su3_timeslice_a1pp": (i32, [C.c_void_p, P(dbl), P(dbl)]),

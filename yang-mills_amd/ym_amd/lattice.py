@@ -107,6 +107,16 @@ class _Lattice:
         check(lib().ym_staple(self._h, _ext(x), int(mu), _dp(out)), "ym_staple")
         return out
 
+    def path_holonomy(self, start, steps):
+        """Ordered product along `steps` (+-(d+1)) from `start`; returns
+        (matrix as NCOMP doubles, end point)."""
+        st = np.ascontiguousarray(np.asarray(steps, dtype=np.int32))
+        out = np.zeros(self.NCOMP)
+        end = (C.c_int64 * 4)()
+        check(lib().ym_path_holonomy(self._h, _ext(start), st.ctypes.data_as(C.POINTER(C.c_int32)), st.size,
+                                     _dp(out), end), "ym_path_holonomy")
+        return out, tuple(end)
+
     def wilson_loops(self, rmax, tmax, plane_set=0):
         re = np.zeros(rmax * tmax)
         im = np.zeros(rmax * tmax)
