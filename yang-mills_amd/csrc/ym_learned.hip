@@ -121,8 +121,10 @@ __device__ __forceinline__ M3<double> from_c3(const la::C3 &c) {
 // Staple-aligned frame W(A) and the 12 features of A.  Returns false when A
 // is singular (no frame; the learned step leaves such a link unchanged).
 __device__ __forceinline__ bool staple_frame(const la::C3 &A, la::C3 &W, float feat[kIn]) {
-    la::C3 Ad = la::c3_dag(A), Z;
-    if (!la::c3_polar_unitary_fast(Ad, Z)) return false;
+    la::C3 Z;
+    // ev: eigenvalues of A A^dagger, the squared singular values of A
+    double ev[3];
+    if (!la::c3_polar_unitary_fast(la::c3_dag(A), Z, ev)) return false;
     const la::cx dz = la::c3_det(Z);
     const double phz = atan2(dz.i, dz.r) / 3.0;
     const la::cx rot = {cos(phz), -sin(phz)};
@@ -130,10 +132,6 @@ __device__ __forceinline__ bool staple_frame(const la::C3 &A, la::C3 &W, float f
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) W.m[i][j] = la::cmul(rot, Z.m[i][j]);
-    // singular values of A from the eigenvalues of A A^dagger; det phase of A
-    const la::C3 AAd = la::c3_mul(A, Ad);
-    double ev[3];
-    la::herm_eigvals(AAd, ev);
     double s[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) s[k] = sqrt(ev[k] > 0.0 ? ev[k] : 0.0) / 3.0;
@@ -155,10 +153,11 @@ __device__ __forceinline__ bool staple_frame(const la::C3 &A, la::C3 &W, float f
 }
 
 // Chart coordinates of U in the frame W: U W^dagger = exp(i X(eps)).
-__device__ __forceinline__ bool chart_coords(const la::C3 &U, const la::C3 &W, double eps[8], double ph[3]) {
+__device__ __forceinline__ bool chart_coords(const la::C3 &U, const la::C3 &W, double eps[8], double ph[3],
+                                             la::cx z[3]) {
     const la::C3 Y = la::c3_mul(U, la::c3_dag(W));
     la::C3 X;
-    if (!la::su3_log_fast(Y, X, ph)) return false;
+    if (!la::su3_log_fast(Y, X, ph, z)) return false;
     la::su3_coords(X, eps);
     return true;
 }
@@ -173,15 +172,15 @@ __device__ __forceinline__ double re_tr_mul(const la::C3 &U, const la::C3 &A) {
 }
 
 // log q up to the constant shared by every state of one link.
-__device__ __forceinline__ double log_q(const double eps[8], const double ph[3], const double mu[8],
-                                       const double inv_sig[8]) {
+__device__ __forceinline__ double log_q(const double eps[8], const double ph[3], const la::cx z[3],
+                                       const double mu[8], const double inv_sig[8]) {
     double acc = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         const double d = (eps[a] - mu[a]) * inv_sig[a];
         acc -= 0.5 * d * d;
     }
-    return acc - la::haar_log_jacobian(ph);
+    return acc - la::haar_log_jacobian(ph, z);
 }
 
 __device__ __forceinline__ double unit_open(uint32_t lo, uint32_t hi) {
@@ -196,66 +195,100 @@ __device__ __forceinline__ double unit_open(uint32_t lo, uint32_t hi) {
 // accumulation).  Lane layouts (gfx950): A, lane l holds row l % 32, k =
 // 8 (l / 32) .. +7; B, lane l holds column l % 32, the same k range; D, lane
 // l register r holds row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
-// Activations pass between layers through two 64 x 64 bf16 LDS tiles per
-// wave (row-chunk XOR swizzle against bank conflicts); weights are bf16 in
-// LDS, biases FP32; inputs, weights and hidden activations are rounded to
-// bf16 (nearest even), as in the FP32-accumulating restatement.
+// Weights are packed once per call (k_mlp_pack) into the B-operand
+// fragments of the 14 MFMAs of one 32-column block: fragment f, lane l holds
+// 8 bf16 at packed[f * 64 + l] (f = Cb for layer 1, 2 + 4 Cb + ks for layer
+// 2, 10 + ks for layer 3), so each wave loads its weights straight from
+// global memory, 16 contiguous bytes per lane, with no block-wide staging or
+// barrier.  Activations pass between layers through the wave's own two
+// 64 x 64 bf16 LDS tiles (row-chunk XOR swizzle against bank conflicts),
+// ordered by wave-local waits only.  Inputs, weights and hidden activations
+// are rounded to bf16 (nearest even), biases stay FP32, as in the
+// FP32-accumulating restatement.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int kFrags = 14;
+
+// tanh for the BF16 path, whose hidden activations are rounded to bfloat16
+// (8 significant bits) right after: odd Taylor polynomial to x^5 for
+// |x| < 1/16 (relative error < 4e-9), 1 - 2 / (2^(2 |x| log2 e) + 1) above,
+// with the hardware exp2 and reciprocal (absolute error a few FP32 ulps).
+__device__ __forceinline__ float tanh_bf16_path(float x) {
+    const float ax = fabsf(x);
+    const float x2 = x * x;
+    const float small = x * (1.0f + x2 * (-0.333333333f + x2 * 0.133333333f));
+    const float e = __builtin_amdgcn_exp2f(ax * 2.88539008f);
+    const float big = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+    return ax < 0.0625f ? small : __builtin_copysignf(big, x);
+}
 
 __device__ __forceinline__ int swz(int row, int col) {
     return row * 64 + ((((col >> 3) ^ (row & 7)) << 3) | (col & 7));
 }
 
-struct MlpLds {
-    __bf16 w1[64 * 16];  // W1 [64][16] row-major, k >= 12 zero
-    __bf16 w2[64 * 64];  // W2 swizzled
-    __bf16 w3[32 * 64];  // W3 [32][64] swizzled, rows >= 16 zero
-    float b1[64], b2[64], b3[16];
-};
+// One block of kFrags x 64 threads... launched as kFrags blocks of 64.
+__global__ __launch_bounds__(64) void k_mlp_pack(const float *__restrict__ P, bf16x8 *__restrict__ packed) {
+    const int f = blockIdx.x, l = threadIdx.x, lr = l & 31, lh = l >> 5;
+    bf16x8 v;
+    if (f < 2) {
+        const int row = 32 * f + lr;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = 8 * lh + i;
+            v[i] = (__bf16)(k < kIn ? P[oW1 + row * kIn + k] : 0.0f);
+        }
+    } else if (f < 10) {
+        const int cb = (f - 2) >> 2, ks = (f - 2) & 3, row = 32 * cb + lr;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (__bf16)P[oW2 + row * kHid + 16 * ks + 8 * lh + i];
+    } else {
+        const int ks = f - 10, row = lr;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (__bf16)(row < kOut ? P[oW3 + row * kHid + 16 * ks + 8 * lh + i] : 0.0f);
+    }
+    packed[f * 64 + l] = v;
+}
 
-__device__ __forceinline__ void mlp_lds_fill(MlpLds &S, const float *__restrict__ P) {
-    for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
-        const int r = i >> 6, c = i & 63;
-        S.w2[swz(r, c)] = (__bf16)P[oW2 + r * kHid + c];
-        if (r < 32) S.w3[swz(r, c)] = (__bf16)((r < kOut) ? P[oW3 + r * kHid + c] : 0.0f);
-        if (c < 16) S.w1[r * 16 + c] = (__bf16)((c < kIn) ? P[oW1 + r * kIn + c] : 0.0f);
-    }
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
-        S.b1[i] = P[oB1 + i];
-        S.b2[i] = P[oB2 + i];
-        if (i < kOut) S.b3[i] = P[oB3 + i];
-    }
+// Orders this wave's LDS writes before its later LDS reads.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ bf16x8 lds_chunk(const __bf16 *__restrict__ t, int row, int chunk) {
     return *reinterpret_cast<const bf16x8 *>(t + row * 64 + ((chunk ^ (row & 7)) << 3));
 }
 
-// Must be reached by every thread of the block.  bufA/bufB: this wave's two
+// Must be reached by every lane of the wave.  bufA/bufB: this wave's two
 // 64 x 64 bf16 tiles.  Rows of lanes without a link carry zero features.
-__device__ __forceinline__ void mlp_wave_bf16(const MlpLds &S, __bf16 *__restrict__ bufA, __bf16 *__restrict__ bufB,
+__device__ __forceinline__ void mlp_wave_bf16(const bf16x8 *__restrict__ packed, const float *__restrict__ P,
+                                              __bf16 *__restrict__ bufA, __bf16 *__restrict__ bufB,
                                               const float feat[kIn], float out[kOut]) {
     const int l = threadIdx.x & 63, lr = l & 31, lh = l >> 5;
+    bf16x8 wf[kFrags];
+#pragma unroll
+    for (int f = 0; f < kFrags; ++f) wf[f] = packed[f * 64 + l];
+    const float b1[2] = {P[oB1 + lr], P[oB1 + 32 + lr]};
+    const float b2[2] = {P[oB2 + lr], P[oB2 + 32 + lr]};
+    const float b3 = lr < kOut ? P[oB3 + lr] : 0.0f;
 #pragma unroll
     for (int c = 0; c < 16; ++c) bufB[swz(l, c)] = (__bf16)(c < kIn ? feat[c] : 0.0f);
-    __syncthreads();
+    wave_lds_sync();
     // layer 1: H1 = tanh(F W1^T + b1) -> bufA
 #pragma unroll
     for (int R = 0; R < 2; ++R)
 #pragma unroll
         for (int Cb = 0; Cb < 2; ++Cb) {
             f32x16 acc = {};
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                lds_chunk(bufB, 32 * R + lr, lh),
-                *reinterpret_cast<const bf16x8 *>(S.w1 + (32 * Cb + lr) * 16 + 8 * lh), acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_chunk(bufB, 32 * R + lr, lh), wf[Cb], acc, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = 32 * R + 8 * (r >> 2) + 4 * lh + (r & 3), col = 32 * Cb + lr;
-                bufA[swz(row, col)] = (__bf16)tanhf(acc[r] + S.b1[col]);
+                bufA[swz(row, col)] = (__bf16)tanh_bf16_path(acc[r] + b1[Cb]);
             }
         }
-    __syncthreads();
+    wave_lds_sync();
     // layer 2: H2 = tanh(H1 W2^T + b2) -> bufB
 #pragma unroll
     for (int R = 0; R < 2; ++R)
@@ -265,14 +298,14 @@ __device__ __forceinline__ void mlp_wave_bf16(const MlpLds &S, __bf16 *__restric
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks)
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_chunk(bufA, 32 * R + lr, 2 * ks + lh),
-                                                              lds_chunk(S.w2, 32 * Cb + lr, 2 * ks + lh), acc, 0, 0, 0);
+                                                              wf[2 + 4 * Cb + ks], acc, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = 32 * R + 8 * (r >> 2) + 4 * lh + (r & 3), col = 32 * Cb + lr;
-                bufB[swz(row, col)] = (__bf16)tanhf(acc[r] + S.b2[col]);
+                bufB[swz(row, col)] = (__bf16)tanh_bf16_path(acc[r] + b2[Cb]);
             }
         }
-    __syncthreads();
+    wave_lds_sync();
     // layer 3: OUT = H2 W3^T + b3 (columns 16..31 of the block are padding);
     // FP32 results staged in bufA (64 x 16 floats)
     float *ob = reinterpret_cast<float *>(bufA);
@@ -281,20 +314,20 @@ __device__ __forceinline__ void mlp_wave_bf16(const MlpLds &S, __bf16 *__restric
         f32x16 acc = {};
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_chunk(bufB, 32 * R + lr, 2 * ks + lh),
-                                                          lds_chunk(S.w3, lr, 2 * ks + lh), acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_chunk(bufB, 32 * R + lr, 2 * ks + lh), wf[10 + ks], acc,
+                                                          0, 0, 0);
         if (lr < kOut) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = 32 * R + 8 * (r >> 2) + 4 * lh + (r & 3);
-                ob[row * kOut + lr] = acc[r] + S.b3[lr];
+                ob[row * kOut + lr] = acc[r] + b3;
             }
         }
     }
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int c = 0; c < kOut; ++c) out[c] = ob[l * kOut + c];
-    __syncthreads();
+    wave_lds_sync();
 }
 
 template <bool BF16> struct LearnedLds;
@@ -302,32 +335,33 @@ template <> struct LearnedLds<false> {
     float P[kParams];
 };
 template <> struct LearnedLds<true> {
-    MlpLds S;
     __bf16 tiles[4][2][64 * 64];  // per wave of a 256-thread block
 };
 
 template <int MU, bool BF16>
 __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geom g, int p,
-                                                     const float *__restrict__ params, double beta, double angle,
+                                                     const float *__restrict__ params,
+                                                     const bf16x8 *__restrict__ packed, double beta, double angle,
                                                      int metro_hits, uint64_t key, uint64_t sweep,
                                                      unsigned long long *__restrict__ counts) {
     __shared__ LearnedLds<BF16> sh;
     __shared__ unsigned long long red[4];
-    if constexpr (BF16) mlp_lds_fill(sh.S, params);
-    else
+    if constexpr (!BF16) {
         for (int i = threadIdx.x; i < kParams; i += 256) sh.P[i] = params[i];
-    __syncthreads();
+        __syncthreads();
+    }
     const int s = tile_slot(g, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
     unsigned long long tried = 0, accepted = 0, macc = 0;
     int x[4] = {0, 0, 0, 0};
     const bool live = s < g.vh && site_of(g, p, s, x);
     // phase 1 (per link): staple, projected link, frame, features, chart
-    M3<double> u, Ad;
+    M3<double> u;
     la::C3 A, W;
     float feat[kIn];
 #pragma unroll
     for (int i = 0; i < kIn; ++i) feat[i] = 0.0f;
     double eps0[8], ph0[3];
+    la::cx z0[3];
     bool ok = false;
     uint32_t site = 0;
     if (live) {
@@ -335,16 +369,16 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
         su3_staple_at<float, MU>(L, g, p, s, x, Af);
         const M3<float> Uf = load3<float>(L, g.vh, MU, p, s);
 #pragma unroll
-        for (int c = 0; c < 18; ++c) {
-            u.v[c] = (double)Uf.v[c];
-            Ad.v[c] = (double)Af.v[c];
-        }
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) A.m[i][j] = {(double)YM_RE(Af, i, j), (double)YM_IM(Af, i, j)};
+#pragma unroll
+        for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
         // the chart needs an exactly special-unitary link: the FP32 link is
         // first projected back with the FP64 Gram-Schmidt of the update
         reunitarize3(u);
-        A = to_c3(Ad);
         site = (uint32_t)lex_of(g, x);
-        if (staple_frame(A, W, feat)) ok = chart_coords(to_c3(u), W, eps0, ph0);
+        if (staple_frame(A, W, feat)) ok = chart_coords(to_c3(u), W, eps0, ph0, z0);
         else {
 #pragma unroll
             for (int i = 0; i < kIn; ++i) feat[i] = 0.0f;
@@ -354,7 +388,7 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
     float out[kOut];
     if constexpr (BF16) {
         const int w = threadIdx.x >> 6;
-        mlp_wave_bf16(sh.S, sh.tiles[w][0], sh.tiles[w][1], feat, out);
+        mlp_wave_bf16(packed, params, sh.tiles[w][0], sh.tiles[w][1], feat, out);
     } else {
         mlp_forward<false>(sh.P, feat, out);
     }
@@ -366,7 +400,9 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
             for (int a = 0; a < 8; ++a) {
                 const float ls = fminf(fmaxf(out[8 + a], kLogSigmaMin), kLogSigmaMax);
                 mu[a] = (double)out[a];
-                sig[a] = exp((double)ls);
+                // sigma is whatever FP32 exp gives; the same value draws
+                // eps' and enters both densities of the MH ratio
+                sig[a] = (double)expf(ls);
                 isig[a] = 1.0 / sig[a];
             }
             double epsn[8];
@@ -389,7 +425,8 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
             // eps -> U' is one to one, so the chart coordinates of U' are
             // eps' and its phases the eigenvalues x
             double ph1[3];
-            const la::C3 E = la::expi_hermitian_fast(la::su3_from_coords(epsn), ph1);
+            la::cx z1[3];
+            const la::C3 E = la::expi_hermitian_fast(la::su3_from_coords(epsn), ph1, z1);
             bool in_chart = true;
 #pragma unroll
             for (int k = 0; k < 3; ++k) in_chart = in_chart && fabs(ph1[k]) < 3.14159165358979;
@@ -397,7 +434,7 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
             if (in_chart) {
                 const la::C3 Un = la::c3_mul(E, W);
                 const double lr = (beta / 3.0) * (re_tr_mul(Un, A) - re_tr_mul(to_c3(u), A)) +
-                                  log_q(eps0, ph0, mu, isig) - log_q(epsn, ph1, mu, isig);
+                                  log_q(eps0, ph0, z0, mu, isig) - log_q(epsn, ph1, z1, mu, isig);
                 if (lr >= 0.0 || uacc < exp(lr)) {
                     accepted = 1;
                     u = from_c3(Un);
@@ -405,7 +442,7 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
             }
         }
         if (metro_hits > 0) {
-            M3<double> M = mul_nn(u, Ad);
+            M3<double> M = mul_nn(u, from_c3(A));
             const double nbt = -(beta / 3.0);
             for (int h = 0; h < metro_hits; ++h) {
                 switch (h & 7) {
@@ -462,8 +499,9 @@ __global__ __launch_bounds__(256) void k_learn_pairs(const float2 *__restrict__ 
         }
         reunitarize3(u);
         la::C3 W;
+        la::cx z[3];
         double ph[3];
-        if (staple_frame(to_c3(Ad), W, feat) && chart_coords(to_c3(u), W, eps, ph)) ok = 1;
+        if (staple_frame(to_c3(Ad), W, feat) && chart_coords(to_c3(u), W, eps, ph, z)) ok = 1;
     }
 #pragma unroll
     for (int i = 0; i < kIn; ++i) feats[r * kIn + i] = feat[i];
@@ -490,8 +528,9 @@ __global__ void k_learn_probe(const float2 *__restrict__ L, Geom g, int p, int s
     la::C3 W;
     float feat[kIn] = {0};
     double eps[8] = {0}, ph[3] = {0};
+    la::cx z[3];
     const int okf = staple_frame(to_c3(Ad), W, feat) ? 1 : 0;
-    const int okc = okf && chart_coords(to_c3(u), W, eps, ph) ? 1 : 0;
+    const int okc = okf && chart_coords(to_c3(u), W, eps, ph, z) ? 1 : 0;
     const M3<double> Wm = from_c3(W);
     for (int c = 0; c < 18; ++c) { out[c] = Ad.v[c]; out[18 + c] = Wm.v[c]; }
     for (int a = 0; a < 8; ++a) out[36 + a] = eps[a];
@@ -499,7 +538,7 @@ __global__ void k_learn_probe(const float2 *__restrict__ L, Geom g, int p, int s
     for (int i = 0; i < kIn; ++i) out[47 + i] = feat[i];
     out[59] = okf;
     out[60] = okc;
-    out[61] = okc ? la::haar_log_jacobian(ph) : 0.0;
+    out[61] = okc ? la::haar_log_jacobian(ph, z) : 0.0;
 }
 
 // ------------------------------------------------------------ training
@@ -660,20 +699,22 @@ __global__ void k_mlp_rows(int64_t *__restrict__ rows, int B, int64_t first, uin
 }
 
 template <bool BF16>
-__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ params, const float *__restrict__ feats,
-                                                   int64_t n, float *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ params,
+                                                   const bf16x8 *__restrict__ packed,
+                                                   const float *__restrict__ feats, int64_t n,
+                                                   float *__restrict__ out) {
     __shared__ LearnedLds<BF16> sh;
-    if constexpr (BF16) mlp_lds_fill(sh.S, params);
-    else
+    if constexpr (!BF16) {
         for (int i = threadIdx.x; i < kParams; i += 256) sh.P[i] = params[i];
-    __syncthreads();
+        __syncthreads();
+    }
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     float f[kIn], o[kOut];
 #pragma unroll
     for (int i = 0; i < kIn; ++i) f[i] = r < n ? feats[r * kIn + i] : 0.0f;
     if constexpr (BF16) {
         const int w = threadIdx.x >> 6;
-        mlp_wave_bf16(sh.S, sh.tiles[w][0], sh.tiles[w][1], f, o);
+        mlp_wave_bf16(packed, params, sh.tiles[w][0], sh.tiles[w][1], f, o);
     } else {
         mlp_forward<false>(sh.P, f, o);
     }
@@ -691,6 +732,11 @@ struct ym_mlp {
     float *feats = nullptr, *targets = nullptr;
     int *valid = nullptr;
     hipStream_t stream = nullptr;
+    // per-call state of ym_su3_learned_sweeps / ym_mlp_forward, kept so
+    // that a call allocates nothing: BF16 weight fragments, counters, events
+    ym::bf16x8 *packed = nullptr;
+    unsigned long long *cnt = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
 namespace {
@@ -796,6 +842,9 @@ int32_t ym_mlp_create(uint64_t seed, int64_t capacity, ym_mlp **out) {
     A((void **)&m->feats, sizeof(float) * ym::kIn * capacity);
     A((void **)&m->targets, sizeof(float) * 8 * capacity);
     A((void **)&m->valid, sizeof(int) * capacity);
+    A((void **)&m->packed, sizeof(ym::bf16x8) * ym::kFrags * 64);
+    A((void **)&m->cnt, sizeof(unsigned long long) * 3);
+    if (!st && (hipEventCreate(&m->e0) != hipSuccess || hipEventCreate(&m->e1) != hipSuccess)) st = YM_HIP_ERROR;
     if (!st) {
         std::vector<float> p;
         init_params(p, seed);
@@ -812,9 +861,11 @@ int32_t ym_mlp_create(uint64_t seed, int64_t capacity, ym_mlp **out) {
 int32_t ym_mlp_destroy(ym_mlp *m) {
     if (!m) return YM_INVALID_ARGUMENT;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    void *b[] = {m->params, m->m1, m->m2, m->grad, m->feats, m->targets, m->valid};
+    void *b[] = {m->params, m->m1, m->m2, m->grad, m->feats, m->targets, m->valid, m->packed, m->cnt};
     for (void *x : b)
         if (x) (void)hipFree(x);
+    if (m->e0) (void)hipEventDestroy(m->e0);
+    if (m->e1) (void)hipEventDestroy(m->e1);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return YM_OK;
@@ -921,8 +972,13 @@ int32_t ym_mlp_forward(const ym_mlp *mc, const float *feats, int64_t n, int32_t 
     int32_t st = hs(hipMemcpyAsync(df, feats, sizeof(float) * ym::kIn * n, hipMemcpyHostToDevice, m->stream));
     const dim3 grid((unsigned)((n + 255) / 256));
     if (!st) {
-        if (bf16) hipLaunchKernelGGL(ym::k_mlp_infer<true>, grid, dim3(256), 0, m->stream, m->params, df, n, dout);
-        else hipLaunchKernelGGL(ym::k_mlp_infer<false>, grid, dim3(256), 0, m->stream, m->params, df, n, dout);
+        hipLaunchKernelGGL(ym::k_mlp_pack, dim3(ym::kFrags), dim3(64), 0, m->stream, m->params, m->packed);
+        if (bf16)
+            hipLaunchKernelGGL(ym::k_mlp_infer<true>, grid, dim3(256), 0, m->stream, m->params, m->packed, df, n,
+                               dout);
+        else
+            hipLaunchKernelGGL(ym::k_mlp_infer<false>, grid, dim3(256), 0, m->stream, m->params, m->packed, df, n,
+                               dout);
         st = hs(hipGetLastError());
     }
     if (!st) st = hs(hipMemcpyAsync(out, dout, sizeof(float) * ym::kOut * n, hipMemcpyDeviceToHost, m->stream));
@@ -1006,21 +1062,22 @@ int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *m, double beta, uin
     const ym::Geom g = ym::make_geom(ext);
     float2 *L = (float2 *)ym::lattice_links(lat);
     hipStream_t ls = ym::lattice_stream(lat);
-    unsigned long long *cnt = nullptr;
-    if (hipMalloc((void **)&cnt, sizeof(unsigned long long) * 3) != hipSuccess) return YM_ALLOCATION_FAILED;
+    unsigned long long *cnt = m->cnt;
+    hipEvent_t e0 = m->e0, e1 = m->e1;
     st = hs(hipStreamSynchronize(m->stream));
-    ym::launch_fill_zero_u64(cnt, 3, ls);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = YM_HIP_ERROR;
-    if (!st) st = hs(hipEventRecord(e0, ls));
+    if (!st) {
+        ym::launch_fill_zero_u64(cnt, 3, ls);
+        hipLaunchKernelGGL(ym::k_mlp_pack, dim3(ym::kFrags), dim3(64), 0, ls, m->params, m->packed);
+        st = hs(hipEventRecord(e0, ls));
+    }
     const int nb = (g.vh + 255) / 256;
     for (int k = 0; k < n_sweeps && !st; ++k)
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p) {
                 const uint64_t sw = first_sweep + (uint64_t)k;
 #define YM_LEARNED_LAUNCH(MU, BF)                                                                                   \
-    hipLaunchKernelGGL((ym::k_su3_learned<MU, BF>), dim3(nb), dim3(256), 0, ls, L, g, p, m->params, beta, angle, \
-                       metro_hits, key, sw, cnt)
+    hipLaunchKernelGGL((ym::k_su3_learned<MU, BF>), dim3(nb), dim3(256), 0, ls, L, g, p, m->params, m->packed, \
+                       beta, angle, metro_hits, key, sw, cnt)
                 if (bf16) {
                     switch (mu) {
                     case 0: YM_LEARNED_LAUNCH(0, true); break;
@@ -1049,9 +1106,6 @@ int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *m, double beta, uin
         if (learned_accepts) *learned_accepts += h[1];
         if (metro_accepts) *metro_accepts += h[2];
     }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(cnt);
     return st;
 }
 

@@ -3,11 +3,15 @@
 // eigenvectors (cyclic complex Jacobi), the principal logarithm of a unitary
 // matrix, the Gell-Mann coordinates of su(3), and the Haar-measure Jacobian
 // of the exponential map.  Written for host and device so that the same
-// source can be checked on the CPU.
+// source can be checked on the CPU.  Multiply-adds may be fused here:
+// nothing in this file is replayed bit for bit, its callers are checked
+// against tolerances.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#pragma clang fp contract(fast)
 
 namespace ym {
 namespace la {
@@ -21,6 +25,10 @@ __host__ __device__ __forceinline__ cx csub(cx a, cx b) { return {a.r - b.r, a.i
 __host__ __device__ __forceinline__ cx cconj(cx a) { return {a.r, -a.i}; }
 __host__ __device__ __forceinline__ cx cscale(cx a, double s) { return {a.r * s, a.i * s}; }
 __host__ __device__ __forceinline__ double cabs2(cx a) { return a.r * a.r + a.i * a.i; }
+// s + a b as four fused multiply-adds
+__host__ __device__ __forceinline__ cx cmadd(cx s, cx a, cx b) {
+    return {__builtin_fma(-a.i, b.i, __builtin_fma(a.r, b.r, s.r)), __builtin_fma(a.i, b.r, __builtin_fma(a.r, b.i, s.i))};
+}
 
 struct C3 {
     cx m[3][3];
@@ -33,8 +41,8 @@ __host__ __device__ __forceinline__ C3 c3_mul(const C3 &a, const C3 &b) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             cx s = cmul(a.m[i][0], b.m[0][j]);
-            s = cadd(s, cmul(a.m[i][1], b.m[1][j]));
-            s = cadd(s, cmul(a.m[i][2], b.m[2][j]));
+            s = cmadd(s, a.m[i][1], b.m[1][j]);
+            s = cmadd(s, a.m[i][2], b.m[2][j]);
             c.m[i][j] = s;
         }
     return c;
@@ -230,7 +238,7 @@ __host__ __device__ __forceinline__ C3 expi_hermitian(const C3 &x) {
             for (int k = 0; k < 3; ++k) {
                 const double ph = h.m[k][k].r;
                 const cx e = {cos(ph), sin(ph)};
-                s = cadd(s, cmul(cmul(v.m[i][k], e), cconj(v.m[j][k])));
+                s = cmadd(s, cmul(v.m[i][k], e), cconj(v.m[j][k]));
             }
             out.m[i][j] = s;
         }
@@ -267,12 +275,12 @@ __host__ __device__ __forceinline__ bool su3_log(const C3 &y, C3 &x, double phas
         for (int i = 0; i < 3; ++i) {
             cx s = {0.0, 0.0};
 #pragma unroll
-            for (int j = 0; j < 3; ++j) s = cadd(s, cmul(y.m[i][j], v.m[j][k]));
+            for (int j = 0; j < 3; ++j) s = cmadd(s, y.m[i][j], v.m[j][k]);
             yv[i] = s;
         }
         cx z = {0.0, 0.0};
 #pragma unroll
-        for (int i = 0; i < 3; ++i) z = cadd(z, cmul(cconj(v.m[i][k]), yv[i]));
+        for (int i = 0; i < 3; ++i) z = cmadd(z, cconj(v.m[i][k]), yv[i]);
         double res = 0.0;
 #pragma unroll
         for (int i = 0; i < 3; ++i) res += cabs2(csub(yv[i], cmul(z, v.m[i][k])));
@@ -297,108 +305,142 @@ __host__ __device__ __forceinline__ bool su3_log(const C3 &y, C3 &x, double phas
     return true;
 }
 
-// Spectral projectors of a Hermitian matrix with eigenvalues ev (descending,
-// from herm_eigvals): P_k = prod_{j != k} (h - ev_j) / (ev_k - ev_j).
-// Returns false when two eigenvalues are closer than 1e-3 of the spread
-// (the interpolation loses accuracy there; callers fall back to Jacobi).
-__host__ __device__ __forceinline__ bool herm_projectors(const C3 &h, const double ev[3], C3 P[3]) {
+// Functions of a Hermitian 3x3 h by Lagrange interpolation on its spectrum.
+// With m the mean eigenvalue, K = h - m I and k = ev - m,
+//   f(h) = sum_k f(ev_k) P_k,  P_k = (K - k_j)(K - k_l) w_k,
+//   w_k = 1 / ((k_k - k_j)(k_k - k_l))   ({j, l} the other two indices),
+// so f(h) = c2 K^2 + c1 K + c0 I with c2 = sum f_k w_k,
+// c1 = -sum f_k (k_j + k_l) w_k, c0 = sum f_k k_j k_l w_k: one matrix
+// product (K^2) for any f.  The shift by m keeps K^2 free of cancellation.
+// herm_spectral returns false when two eigenvalues are closer than 1e-3 of
+// the spread (the weights lose accuracy there; callers fall back to Jacobi
+// or Newton).
+struct Spectral {
+    C3 K, K2;
+    double w[3], s[3], p[3];  // w_k, k_j + k_l, k_j k_l
+};
+__host__ __device__ __forceinline__ bool herm_spectral(const C3 &h, const double ev[3], Spectral &S) {
     const double spread = ev[0] - ev[2];
     const double g01 = ev[0] - ev[1], g12 = ev[1] - ev[2];
     if (!(spread > 1e-300) || !(g01 > 1e-3 * spread) || !(g12 > 1e-3 * spread)) return false;
-    C3 M[3];
+    const double m = (h.m[0][0].r + h.m[1][1].r + h.m[2][2].r) / 3.0;
+    S.K = h;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        M[k] = h;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) M[k].m[d][d].r -= ev[k];
-    }
-    const double inv0 = 1.0 / (g01 * (ev[0] - ev[2]));
-    const double inv1 = 1.0 / ((ev[1] - ev[0]) * g12);
-    const double inv2 = 1.0 / ((ev[2] - ev[0]) * (ev[2] - ev[1]));
-    P[0] = c3_mul(M[1], M[2]);
-    P[1] = c3_mul(M[0], M[2]);
-    P[2] = c3_mul(M[0], M[1]);
-    const double inv[3] = {inv0, inv1, inv2};
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) P[k].m[i][j] = cscale(P[k].m[i][j], inv[k]);
+    for (int d = 0; d < 3; ++d) S.K.m[d][d].r -= m;
+    S.K2 = c3_mul(S.K, S.K);
+    const double k0 = ev[0] - m, k1 = ev[1] - m, k2 = ev[2] - m;
+    S.w[0] = 1.0 / (g01 * spread);
+    S.w[1] = -1.0 / (g01 * g12);
+    S.w[2] = 1.0 / (spread * g12);
+    S.s[0] = k1 + k2;
+    S.p[0] = k1 * k2;
+    S.s[1] = k0 + k2;
+    S.p[1] = k0 * k2;
+    S.s[2] = k0 + k1;
+    S.p[2] = k0 * k1;
     return true;
 }
+__host__ __device__ __forceinline__ C3 spectral_real(const Spectral &S, const double f[3]) {
+    double c2 = 0.0, c1 = 0.0, c0 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double fw = f[k] * S.w[k];
+        c2 += fw;
+        c1 -= fw * S.s[k];
+        c0 += fw * S.p[k];
+    }
+    C3 o;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            o.m[i][j] = {c2 * S.K2.m[i][j].r + c1 * S.K.m[i][j].r + (i == j ? c0 : 0.0),
+                         c2 * S.K2.m[i][j].i + c1 * S.K.m[i][j].i};
+    return o;
+}
+__host__ __device__ __forceinline__ C3 spectral_cx(const Spectral &S, const cx f[3]) {
+    cx c2 = {0.0, 0.0}, c1 = {0.0, 0.0}, c0 = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const cx fw = cscale(f[k], S.w[k]);
+        c2 = cadd(c2, fw);
+        c1 = csub(c1, cscale(fw, S.s[k]));
+        c0 = cadd(c0, cscale(fw, S.p[k]));
+    }
+    C3 o;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            cx t = cmadd(cmul(c2, S.K2.m[i][j]), c1, S.K.m[i][j]);
+            if (i == j) t = cadd(t, c0);
+            o.m[i][j] = t;
+        }
+    return o;
+}
+// Tr(a b)
+__host__ __device__ __forceinline__ cx c3_trace_mul(const C3 &a, const C3 &b) {
+    cx t = {0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t = cmadd(t, a.m[i][j], b.m[j][i]);
+    return t;
+}
 
-// Unitary polar factor of a through the spectrum of a^dagger a = Q diag(s^2)
-// Q^dagger: w = a sum_k P_k / s_k; falls back to the Newton iteration when
-// two singular values nearly coincide.
-__host__ __device__ __forceinline__ bool c3_polar_unitary_fast(const C3 &a, C3 &w) {
+// Unitary polar factor of a through the spectrum of h = a^dagger a =
+// Q diag(s^2) Q^dagger: w = a h^{-1/2}; falls back to the Newton iteration
+// when two singular values nearly coincide or a is nearly singular.  ev: the
+// eigenvalues s_k^2 of h (descending), for either path.
+__host__ __device__ __forceinline__ bool c3_polar_unitary_fast(const C3 &a, C3 &w, double ev[3]) {
     const C3 h = c3_mul(c3_dag(a), a);
-    double ev[3];
     herm_eigvals(h, ev);
-    C3 P[3];
-    if (ev[2] > 1e-4 * ev[0] && herm_projectors(h, ev, P)) {
-        C3 S;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                cx t = {0.0, 0.0};
-#pragma unroll
-                for (int k = 0; k < 3; ++k) t = cadd(t, cscale(P[k].m[i][j], 1.0 / sqrt(ev[k])));
-                S.m[i][j] = t;
-            }
-        w = c3_mul(a, S);
+    Spectral S;
+    if (ev[2] > 1e-4 * ev[0] && herm_spectral(h, ev, S)) {
+        const double f[3] = {1.0 / sqrt(ev[0]), 1.0 / sqrt(ev[1]), 1.0 / sqrt(ev[2])};
+        w = c3_mul(a, spectral_real(S, f));
         return true;
     }
     return c3_polar_unitary(a, w);
 }
 
-// exp(i X) for Hermitian X, with its eigenvalues x (descending): through the
-// spectral projectors, or Jacobi when two eigenvalues nearly coincide.
-__host__ __device__ __forceinline__ C3 expi_hermitian_fast(const C3 &x, double ev[3]) {
+// exp(i X) for Hermitian X, with its eigenvalues x (descending when the
+// interpolation applies) and e_k = exp(i x_k); Jacobi when two eigenvalues
+// nearly coincide.
+__host__ __device__ __forceinline__ C3 expi_hermitian_fast(const C3 &x, double ev[3], cx e[3]) {
     herm_eigvals(x, ev);
-    C3 P[3];
-    if (herm_projectors(x, ev, P)) {
-        C3 out;
-        cx e[3];
+    Spectral S;
+    if (herm_spectral(x, ev, S)) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) e[k] = {cos(ev[k]), sin(ev[k])};
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                cx t = {0.0, 0.0};
-#pragma unroll
-                for (int k = 0; k < 3; ++k) t = cadd(t, cmul(e[k], P[k].m[i][j]));
-                out.m[i][j] = t;
-            }
-        return out;
+        return spectral_cx(S, e);
     }
     C3 h = x, v;
     herm_jacobi(h, v);
     C3 out;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) ev[k] = h.m[k][k].r;
+    for (int k = 0; k < 3; ++k) {
+        ev[k] = h.m[k][k].r;
+        e[k] = {cos(ev[k]), sin(ev[k])};
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             cx t = {0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const cx e = {cos(ev[k]), sin(ev[k])};
-                t = cadd(t, cmul(cmul(v.m[i][k], e), cconj(v.m[j][k])));
-            }
+            for (int k = 0; k < 3; ++k) t = cmadd(t, cmul(v.m[i][k], e[k]), cconj(v.m[j][k]));
             out.m[i][j] = t;
         }
     return out;
 }
 
-// Principal logarithm as su3_log, first through the spectral projectors of
+// Principal logarithm as su3_log, first by interpolation on the spectrum of
 // the Hermitian H = (y - y^dagger)/(2i) + c (y + y^dagger)/2 (which commutes
-// with y): phases from tr(y P_k); falls back to su3_log when two eigenvalues
-// of H nearly coincide or a projector is not rank one (|tr(y P_k)| != 1).
-__host__ __device__ __forceinline__ bool su3_log_fast(const C3 &y, C3 &x, double phases[3]) {
+// with y): z_k = Tr(y P_k) = exp(i phi_k); falls back to su3_log when two
+// eigenvalues of H nearly coincide or a projector is not rank one
+// (|z_k| != 1).  z (unit modulus) receives exp(i phases).
+__host__ __device__ __forceinline__ bool su3_log_fast(const C3 &y, C3 &x, double phases[3], cx z[3]) {
     const double c = 0.3819660112501051518;
     C3 h;
 #pragma unroll
@@ -410,19 +452,18 @@ __host__ __device__ __forceinline__ bool su3_log_fast(const C3 &y, C3 &x, double
         }
     double ev[3];
     herm_eigvals(h, ev);
-    C3 P[3];
-    if (herm_projectors(h, ev, P)) {
+    Spectral S;
+    if (herm_spectral(h, ev, S)) {
+        const cx t2 = c3_trace_mul(y, S.K2), t1 = c3_trace_mul(y, S.K), t0 = c3_trace(y);
         double ph[3], sum = 0.0;
         bool good = true;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            cx z = {0.0, 0.0};
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) z = cadd(z, cmul(y.m[i][j], P[k].m[j][i]));
-            good = good && fabs(cabs2(z) - 1.0) < 1e-9;
-            ph[k] = atan2(z.i, z.r);
+            const cx zk = cscale(cadd(csub(t2, cscale(t1, S.s[k])), cscale(t0, S.p[k])), S.w[k]);
+            const double n2 = cabs2(zk);
+            good = good && fabs(n2 - 1.0) < 1e-9;
+            ph[k] = atan2(zk.i, zk.r);
+            z[k] = cscale(zk, 1.0 / sqrt(n2));
             sum += ph[k];
         }
         if (good) {
@@ -430,46 +471,47 @@ __host__ __device__ __forceinline__ bool su3_log_fast(const C3 &y, C3 &x, double
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 if (!(fabs(ph[k]) < 3.14159165358979)) return false;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    cx t = {0.0, 0.0};
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) t = cadd(t, cscale(P[k].m[i][j], ph[k]));
-                    x.m[i][j] = t;
-                }
+            x = spectral_real(S, ph);
             if (phases)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) phases[k] = ph[k];
             return true;
         }
     }
-    return su3_log(y, x, phases);
+    double ph[3];
+    if (!su3_log(y, x, ph)) return false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        z[k] = {cos(ph[k]), sin(ph[k])};
+        if (phases) phases[k] = ph[k];
+    }
+    return true;
 }
 
 // log of the Haar density of exp(i X) in the coordinates eps (X = eps.lambda/2)
 // relative to Lebesgue measure, up to a constant:
-// sum_{j<k} 2 log sinc((x_j - x_k) / 2), x the eigenvalues of X.
-__host__ __device__ __forceinline__ double haar_log_jacobian(const double x[3]) {
-    double acc = 0.0;
+// sum_{j<k} 2 log sinc((x_j - x_k) / 2), x the eigenvalues of X, from x and
+// z_k = exp(i x_k):
+// sinc^2((x_j - x_k)/2) = |z_j - z_k|^2 / (x_j - x_k)^2, one logarithm for
+// the three pairs (1 - d^2/12 for |d| < 1e-6, exact to 1e-25 there).
+__host__ __device__ __forceinline__ double haar_log_jacobian(const double x[3], const cx z[3]) {
+    double num = 1.0, den = 1.0;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
         for (int k = j + 1; k < 3; ++k) {
-            const double d = 0.5 * (x[j] - x[k]);
-            const double ad = fabs(d);
-            double sinc;
-            if (ad < 1e-4) {
-                const double d2 = d * d;
-                sinc = 1.0 - d2 / 6.0 + d2 * d2 / 120.0;
+            const double d = x[j] - x[k];
+            if (fabs(d) < 1e-6) {
+                num *= 1.0 - d * d / 12.0;
             } else {
-                sinc = sin(ad) / ad;
+                num *= cabs2(csub(z[j], z[k]));
+                den *= d * d;
             }
-            acc += 2.0 * log(sinc);
         }
-    return acc;
+    return log(num / den);
 }
 
 }  // namespace la
 }  // namespace ym
+
+#pragma clang fp contract(off)
