@@ -88,17 +88,25 @@ def cpu_baseline(seconds_target):
     }
 
 
-def learned_vs_plain(chains=4, sweeps=1500, discard=300, therm=2000, configs=20, epochs=10):
+def learned_vs_plain(chains=8, sweeps=2500, discard=500, therm=2000, equil=1000, configs=20, epochs=10):
     """Plaquette ESS/s of the learned proposal (one MH-corrected independence
     proposal per link per sweep, BF16 MFMA inference, no Metropolis hits)
     against plain 8-hit Metropolis on SU(3) 16^3x32 at beta = 6.0.  Chain c
     is thermalised on its own (cold start, `therm` plain sweeps, its own key);
-    the plain and the learned chain c both continue from that configuration
-    for `sweeps` sweeps, measuring the plaquette after every sweep.  ESS from
+    the network is trained on pairs from chain 0 after that; then every chain
+    runs `equil` learned sweeps, because plain Metropolis from a cold start is
+    still drifting after 10^4 sweeps at this volume (its plaquette approaches
+    the equilibrium value from above for that long) while the learned chain
+    equilibrates within ~10^3.  The plain and the learned chain c both
+    continue from that equilibrated configuration for `sweeps` sweeps,
+    measuring the plaquette after every sweep.  ESS from
     the reference diagnostics (positive-window tau_int, su2_ensemble.rs:
-    375-463) summed over chains (su3_confirmation.rs:214); seconds are the
-    wall time of the sweeps and measurements.  Training (pair collection from
-    one thermalised chain + Adam) is reported separately as setup cost."""
+    375-463) summed over chains (su3_confirmation.rs:214).  Two clocks:
+    `seconds` is the wall time of the sweeps and the per-sweep measurements
+    (host calls included), `device_seconds` the HIP-event time of the sweep
+    kernels alone.  The plain chains use the autotuned tile mapping.
+    Training (pair collection from one thermalised chain + Adam) is reported
+    separately as setup cost."""
     import numpy as np
 
     import ym_amd
@@ -119,23 +127,28 @@ def learned_vs_plain(chains=4, sweeps=1500, discard=300, therm=2000, configs=20,
     pairs = mlp.dataset_size()[0]
     losses, train_ms = mlp.train(epochs=epochs, batch=16384, lr=2e-3)
     setup_s = time.perf_counter() - t0
+    for c in range(chains):
+        ym_amd.learned.learned_sweeps(starts[c], mlp, 6.0, keys[c], 50000, equil, metro_hits=0, bf16=True)
     out = {"workload": "SU(3) Wilson action 16^3x32, beta=6.0, plaquette after every sweep",
            "training_pairs": pairs, "training_epochs": epochs, "final_training_loss": float(losses[-1]),
            "setup_seconds": setup_s, "chains": chains, "sweeps_per_chain": sweeps, "discarded": discard,
-           "thermalisation_sweeps_per_chain": therm}
+           "thermalisation_sweeps_per_chain": f"{therm} plain + {equil} learned"}
     for kind in ("plain", "learned"):
-        rows, acc, wall = [], [], 0.0
+        rows, acc, wall, dev_ms = [], [], 0.0, 0.0
         for c in range(chains):
             l = starts[c].clone()
+            if kind == "plain":
+                l.autotune(6.0, 0.35, 8)
             vals = []
             t1 = time.perf_counter()
             for s in range(sweeps):
                 sw = 100000 + s
                 if kind == "plain":
-                    l.metropolis_sweeps(6.0, 0.35, keys[c], sw, 1, 8)
+                    dev_ms += l.metropolis_sweeps_profiled(6.0, 0.35, keys[c], sw, 1, 8)[2]
                 else:
                     r = ym_amd.learned.learned_sweeps(l, mlp, 6.0, keys[c], sw, 1, metro_hits=0, bf16=True)
                     acc.append(r.learned_accepts / max(1, r.learned_proposals))
+                    dev_ms += r.device_ms
                 vals.append(l.average_plaquette())
             wall += time.perf_counter() - t1
             rows.append(np.array(vals[discard:]))
@@ -145,11 +158,15 @@ def learned_vs_plain(chains=4, sweeps=1500, discard=300, therm=2000, configs=20,
         means = rows.mean(axis=1)
         out[kind] = {"plaquette": pool.pooled_mean, "plaquette_se": pool.pooled_se,
                      "between_chain_se": float(means.std(ddof=1) / np.sqrt(chains)),
-                     "ess": pool.aggregate_ess, "tau_int": taus, "seconds": wall,
-                     "ess_per_s": pool.aggregate_ess / wall}
+                     "ess": pool.aggregate_ess, "tau_int": taus, "mean_tau_int": float(np.mean(taus)),
+                     "seconds": wall, "device_seconds": dev_ms * 1e-3,
+                     "device_us_per_sweep": dev_ms * 1e3 / (chains * sweeps),
+                     "ess_per_s": pool.aggregate_ess / wall,
+                     "ess_per_device_s": pool.aggregate_ess / (dev_ms * 1e-3)}
         if acc:
             out[kind]["acceptance"] = float(np.mean(acc))
     out["ess_per_s_ratio"] = out["learned"]["ess_per_s"] / out["plain"]["ess_per_s"]
+    out["ess_per_device_s_ratio"] = out["learned"]["ess_per_device_s"] / out["plain"]["ess_per_device_s"]
     d = out["learned"]["plaquette"] - out["plain"]["plaquette"]
     se = max(out["learned"]["plaquette_se"], out["learned"]["between_chain_se"]) ** 2 + \
         max(out["plain"]["plaquette_se"], out["plain"]["between_chain_se"]) ** 2
@@ -277,7 +294,8 @@ def main():
         log("learned vs plain plaquette ESS/s (16^3x32)")
         learned = learned_vs_plain()
         log(f"ESS/s learned {learned['learned']['ess_per_s']:.1f} vs plain {learned['plain']['ess_per_s']:.1f} "
-            f"(x{learned['ess_per_s_ratio']:.2f}), acceptance {learned['learned'].get('acceptance', 0):.3f}")
+            f"(x{learned['ess_per_s_ratio']:.2f}; device time x{learned['ess_per_device_s_ratio']:.2f}), "
+            f"acceptance {learned['learned'].get('acceptance', 0):.3f}")
 
     cpu = None
     if not args.no_cpu_baseline:
