@@ -144,7 +144,7 @@ def learned_vs_plain(chains=8, sweeps=2500, discard=500, therm=2000, equil=1000,
             for s in range(sweeps):
                 sw = 100000 + s
                 if kind == "plain":
-                    dev_ms += l.metropolis_sweeps_profiled(6.0, 0.35, keys[c], sw, 1, 8)[2]
+                    dev_ms += l.metropolis_sweeps_timed(6.0, 0.35, keys[c], sw, 1, 8)[2]
                 else:
                     r = ym_amd.learned.learned_sweeps(l, mlp, 6.0, keys[c], sw, 1, metro_hits=0, bf16=True)
                     acc.append(r.learned_accepts / max(1, r.learned_proposals))

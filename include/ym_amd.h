@@ -135,8 +135,9 @@ int32_t ym_su3_autotune(ym_lattice *lat, double beta, double proposal_angle, int
 /* Profiling variant of ym_su3_metropolis_sweeps (same arithmetic and
  * results): HIP events on the handle's stream around every sub-pass launch.
  * *total_ms = first start to last stop; *kernel_ms_sum = sum of the
- * per-launch durations; *launches = 8 * n_sweeps.  No reference
- * counterpart (the reference has no timers). */
+ * per-launch durations; *launches = 8 * n_sweeps.  With kernel_ms_sum and
+ * launches both NULL only the span is timed (two events, none between
+ * launches).  No reference counterpart (the reference has no timers). */
 int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *lat, double beta, double proposal_angle, int32_t hits,
                                           uint64_t key, uint64_t first_sweep, int32_t n_sweeps, int32_t reunitarize,
                                           uint64_t *proposals, uint64_t *accepts, float *total_ms,

@@ -456,3 +456,11 @@ def test_autotune_keeps_results_identical(ym):
     pb = b.metropolis_sweeps(6.0, 0.35, key, 0, 3, 8)
     assert pa == pb
     assert np.array_equal(a.download(), b.download())
+    # the timed variants run the same sweeps
+    c = a.clone()
+    pa = a.metropolis_sweeps(6.0, 0.35, key, 3, 2, 8)
+    p1, a1, tot, ksum, nl = b.metropolis_sweeps_profiled(6.0, 0.35, key, 3, 2, 8)
+    p2, a2, ms = c.metropolis_sweeps_timed(6.0, 0.35, key, 3, 2, 8)
+    assert pa == (p1, a1) == (p2, a2)
+    assert nl == 16 and 0 < ksum <= tot and ms > 0
+    assert np.array_equal(a.download(), b.download()) and np.array_equal(a.download(), c.download())

@@ -383,14 +383,17 @@ int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *l, double beta, double ang
                                           uint64_t first_sweep, int32_t n_sweeps, int32_t reunitarize,
                                           uint64_t *proposals, uint64_t *accepts, float *total_ms,
                                           float *kernel_ms_sum, int32_t *launches) {
-    if (!l || l->group != 3 || hits < 1 || n_sweeps < 1 || !total_ms || !kernel_ms_sum || !launches)
+    if (!l || l->group != 3 || hits < 1 || n_sweeps < 1 || !total_ms || (!kernel_ms_sum != !launches))
         return YM_INVALID_ARGUMENT;
     if (!std::isfinite(beta) || beta < 0.0 || beta > 12.0) return YM_BETA_OUT_OF_RANGE;
     if (!std::isfinite(angle) || angle < 1.0e-8 || angle > 1.0) return YM_ANGLE_OUT_OF_RANGE;
     if (!all_even(l)) return YM_UNSUPPORTED;
     const int reunit = (l->prec == 32) ? 1 : (reunitarize != 0);
     const int nl = 8 * n_sweeps;
-    std::vector<hipEvent_t> ev(2 * (size_t)nl, nullptr);
+    // per-launch events only when per-launch times are asked for: an event
+    // between two launches delays the second one
+    const bool per_launch = kernel_ms_sum != nullptr;
+    std::vector<hipEvent_t> ev(per_launch ? 2 * (size_t)nl : 2, nullptr);
     int32_t st = YM_OK;
     for (auto &e : ev)
         if (hipEventCreate(&e) != hipSuccess) st = YM_HIP_ERROR;
@@ -400,10 +403,10 @@ int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *l, double beta, double ang
         for (int s = 0; s < n_sweeps; ++s)
             for (int mu = 0; mu < 4; ++mu)
                 for (int p = 0; p < 2; ++p, ++k) {
-                    (void)hipEventRecord(ev[2 * k], l->stream);
+                    if (per_launch || k == 0) (void)hipEventRecord(ev[per_launch ? 2 * k : 0], l->stream);
                     ym::launch_su3_metropolis(l->prec, l->links, l->g, mu, p, beta, angle, hits, key,
                                               first_sweep + (uint64_t)s, reunit, l->dctr, l->stream);
-                    (void)hipEventRecord(ev[2 * k + 1], l->stream);
+                    if (per_launch || k == nl - 1) (void)hipEventRecord(ev[per_launch ? 2 * k + 1 : 1], l->stream);
                 }
         unsigned long long acc = 0;
         st = hip_status(hipGetLastError());
@@ -411,13 +414,15 @@ int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *l, double beta, double ang
         if (!st) st = finish(l);
         if (!st) {
             float sum = 0.f, ms = 0.f;
-            for (int i = 0; i < nl && !st; ++i) {
+            for (int i = 0; per_launch && i < nl && !st; ++i) {
                 st = hip_status(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
                 sum += ms;
             }
-            if (!st) st = hip_status(hipEventElapsedTime(total_ms, ev[0], ev[2 * nl - 1]));
-            *kernel_ms_sum = sum;
-            *launches = nl;
+            if (!st) st = hip_status(hipEventElapsedTime(total_ms, ev[0], ev[ev.size() - 1]));
+            if (per_launch) {
+                *kernel_ms_sum = sum;
+                *launches = nl;
+            }
             if (proposals) *proposals += (uint64_t)n_sweeps * (uint64_t)hits * 4ull * (uint64_t)l->g.volume;
             if (accepts) *accepts += acc;
         }

@@ -215,6 +215,18 @@ class Su3Lattice4(_Lattice):
         check(st, "ym_su3_metropolis_sweeps_profiled")
         return p.value, a.value, tot.value, ksum.value, nl.value
 
+    def metropolis_sweeps_timed(self, beta, angle, key, first_sweep, n_sweeps, hits=8, reunitarize=True):
+        """Same sweeps with one HIP event pair around all launches.
+
+        Returns (proposals, accepts, device_ms)."""
+        p, a = C.c_uint64(0), C.c_uint64(0)
+        tot = C.c_float()
+        st = lib().ym_su3_metropolis_sweeps_profiled(self._h, float(beta), float(angle), int(hits), int(key),
+                                                     int(first_sweep), int(n_sweeps), 1 if reunitarize else 0,
+                                                     C.byref(p), C.byref(a), C.byref(tot), None, None)
+        check(st, "ym_su3_metropolis_sweeps_profiled")
+        return p.value, a.value, tot.value
+
     def timeslice_a1pp(self):
         T = self.extents[3]
         pl, rc = np.zeros(T), np.zeros(T)
