@@ -153,8 +153,9 @@ def learned_vs_plain(chains=8, sweeps=2500, discard=500, therm=2000, equil=1000,
             wall += time.perf_counter() - t1
             rows.append(np.array(vals[discard:]))
         rows = np.array(rows)
-        pool = ym_amd.stats.pool_chains(rows, 100)
-        taus = [ym_amd.diagnostics(r, 100)[0].tau_int for r in rows]
+        lag = min(100, rows.shape[1] // 2)
+        pool = ym_amd.stats.pool_chains(rows, lag)
+        taus = [ym_amd.diagnostics(r, lag)[0].tau_int for r in rows]
         means = rows.mean(axis=1)
         out[kind] = {"plaquette": pool.pooled_mean, "plaquette_se": pool.pooled_se,
                      "between_chain_se": float(means.std(ddof=1) / np.sqrt(chains)),
@@ -237,6 +238,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain ESS/s measurement")
+    ap.add_argument("--learned-chains", type=int, default=8)
+    ap.add_argument("--learned-sweeps", type=int, default=2500, help="measured sweeps per chain (a fifth, at "
+                    "most 500, discarded)")
+    ap.add_argument("--learned-therm", type=int, default=2000, help="plain thermalisation sweeps per chain")
+    ap.add_argument("--learned-equil", type=int, default=1000, help="learned equilibration sweeps per chain")
     ap.add_argument("--hot-start", action="store_true",
                     help="Haar-random start instead of the reference's cold (identity) start")
     ap.add_argument("--traffic-calibration", action="store_true",
@@ -292,7 +298,9 @@ def main():
     learned = None
     if not args.no_learned:
         log("learned vs plain plaquette ESS/s (16^3x32)")
-        learned = learned_vs_plain()
+        learned = learned_vs_plain(chains=max(2, args.learned_chains), sweeps=args.learned_sweeps,
+                                   discard=min(500, args.learned_sweeps // 5), therm=args.learned_therm,
+                                   equil=args.learned_equil)
         log(f"ESS/s learned {learned['learned']['ess_per_s']:.1f} vs plain {learned['plain']['ess_per_s']:.1f} "
             f"(x{learned['ess_per_s_ratio']:.2f}; device time x{learned['ess_per_device_s_ratio']:.2f}), "
             f"acceptance {learned['learned'].get('acceptance', 0):.3f}")
