@@ -257,6 +257,15 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     int q = nb >> 3, r = nb & 7;
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
+// The same ranges, each walked backwards (block b takes the tile that
+// xcd_tile gives to the last-dispatched block of its XCD, and so on).
+__device__ __forceinline__ int xcd_tile_rev(int b, int nb) {
+    int xcd = b & 7, idx = b >> 3;
+    int q = nb >> 3, r = nb & 7;
+    const int cnt = q + (xcd < r ? 1 : 0);
+    const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+    return base + cnt - 1 - idx;
+}
 
 // Slot of thread tid (0..255) of tile t.  tmode 0: tile_order(t) * 256 +
 // tid.  tmode 1 (compact tiles): tile t covers x0 in {2 X0, 2 X0 + 1},

@@ -60,7 +60,10 @@ __global__ __launch_bounds__(256) void k_ens_metropolis(typename Vec2<Real>::typ
                                                         int sweep_k, int reunit,
                                                         unsigned long long *__restrict__ acc) {
     __shared__ unsigned long long red[4];
-    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    // parity-1 sub-passes walk the tiles backwards: each sub-pass reads every
+    // chain's whole field, and starting where the previous one ended finds
+    // its last lines still cached
+    const int b = (p & 1) ? xcd_tile_rev(blockIdx.x, gridDim.x) : xcd_tile(blockIdx.x, gridDim.x);
     const int c = chain_of(begin, nch, b);
     const Geom g = cd[c].g;
     typename Vec2<Real>::type *L = links + cd[c].link_off;

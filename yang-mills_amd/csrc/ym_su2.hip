@@ -60,7 +60,9 @@ __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type 
                                                       unsigned long long *__restrict__ counters,
                                                       int *__restrict__ status) {
     __shared__ unsigned long long red[4];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    // parity-1 sub-passes walk the tiles backwards (the field of a 48^4
+    // lattice is larger than the memory-side cache; see k_su3_metropolis_ws)
+    const int tile = (p & 1) ? xcd_tile_rev(blockIdx.x, gridDim.x) : xcd_tile(blockIdx.x, gridDim.x);
     const int s = tile * 256 + threadIdx.x;
     unsigned long long nprop = 0, nupd = 0;
     int x[4];
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type 
 
 template <typename Real, int MU>
 __global__ __launch_bounds__(256) void k_su2_overrelax(typename Vec4<Real>::type *__restrict__ L, Geom g, int p) {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = (p & 1) ? xcd_tile_rev(blockIdx.x, gridDim.x) : xcd_tile(blockIdx.x, gridDim.x);
     const int s = tile * 256 + threadIdx.x;
     int x[4];
     if (s >= g.vh || !site_of(g, p, s, x)) return;

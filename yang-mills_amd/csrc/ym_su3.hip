@@ -49,7 +49,13 @@ __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::typ
 // round-robin over the XCDs), and the blocks of an XCD advance through its
 // range side by side, so the tiles in flight on an XCD are neighbours.  The
 // producers also stage the link being updated, so the hit waves touch global
-// memory only for the final store.
+// memory only for the final store.  Every sub-pass reads the whole link
+// field; sub-passes of parity 1 walk each XCD's range backwards, so a
+// sub-pass starts on the tiles its predecessor touched last, which are still
+// in that XCD's L2 and in the memory-side cache (a 302 MB field at 32^4 is
+// larger than the 256 MB cache: the same direction every time evicts each
+// line just before it is needed again).  Tiles are independent, so the
+// direction changes timing only.
 template <typename Real, int MU>
 __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::type *__restrict__ L, Geom g, int p,
                                                            double nbt, double angle, int hits, uint64_t key,
@@ -66,8 +72,10 @@ __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::
     const int nsteps = (t1 - t0 - j + nbx - 1) / nbx;
     unsigned long long nacc = 0;
     // producer: staple of the link in slot `lane` of tile i into buffer i & 1
+    const bool rev = (p & 1) != 0;
+    auto tile_at = [&](int i) { return rev ? t1 - 1 - (j + i * nbx) : t0 + j + i * nbx; };
     auto produce = [&](int i) {
-        const int s = tile_slot(g, t0 + j + i * nbx, lane);
+        const int s = tile_slot(g, tile_at(i), lane);
         int x[4];
         if (s < g.vh && site_of(g, p, s, x)) {
             M3<Real> A;
@@ -86,7 +94,7 @@ __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::
         if (producer) {
             if (i + 1 < nsteps) produce(i + 1);
         } else {
-            const int s = tile_slot(g, t0 + j + i * nbx, lane);
+            const int s = tile_slot(g, tile_at(i), lane);
             int x[4];
             if (s < g.vh && site_of(g, p, s, x)) {
                 M3<Real> A, U;
