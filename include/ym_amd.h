@@ -58,6 +58,7 @@ enum {
     YM_TIME_SLICE_OUT_OF_RANGE = 50,
     YM_SPATIAL_EXTENT_TOO_SMALL = 51,
     YM_SPATIAL_NOT_CUBIC = 52,
+    YM_SPATIAL_EXTENT_NOT_EVEN = 55,  /* GlueballBlockingError::SpatialExtentNotEven */
     YM_NON_FINITE_RHO = 60,
     YM_RHO_OUT_OF_RANGE = 61,
     YM_TOO_MANY_LEVELS = 62,
@@ -74,6 +75,36 @@ enum {
     YM_EFF_INVALID_DIAGNOSTIC = 84,
     YM_EFF_DEGENERATE = 85,
     YM_EFF_NON_FINITE = 86,
+    YM_EM_TIME_EXTENT_TOO_SMALL = 100, /* EffectiveMassError::TimeExtentTooSmall (T < 4) */
+    YM_EM_TIME_EXTENT_TOO_LARGE = 101, /* EffectiveMassError::TimeExtentTooLarge (T > 64) */
+    YM_EM_TOO_FEW_REPLICAS = 102,      /* EffectiveMassError::TooFewJackknifeReplicates (< 3) */
+    YM_EM_TOO_MANY_REPLICAS = 103,     /* EffectiveMassError::TooManyJackknifeReplicates (> 64) */
+    YM_EM_EMPTY_WINDOW = 104,          /* EffectiveMassError::EmptyAnalysisWindow */
+    YM_EM_INVALID_LAG = 105,           /* EffectiveMassError::InvalidAnalysisLag */
+    YM_EM_SHAPE_MISMATCH = 106,        /* EffectiveMassError::ShapeMismatch */
+    YM_EM_NON_FINITE_CORRELATOR = 107, /* EffectiveMassError::NonFiniteCorrelator */
+    YM_EM_NON_POSITIVE_CENTRE = 108,   /* EffectiveMassError::NonPositiveCentralCorrelator */
+    YM_EM_ACOSH_DOMAIN = 109,          /* EffectiveMassError::AcoshDomainViolation */
+    YM_EM_NON_FINITE_STATISTIC = 110,  /* EffectiveMassError::NonFiniteStatistic */
+    YM_EM_INVALID_FIXTURE = 111,       /* EffectiveMassError::InvalidFixtureParameter */
+    YM_FIT_EMPTY_CANDIDATES = 120,     /* ScalarFitError::EmptyCandidateWindows */
+    YM_FIT_TOO_MANY_CANDIDATES = 121,  /* ScalarFitError::TooManyCandidateWindows */
+    YM_FIT_INVALID_CANDIDATE = 122,    /* ScalarFitError::InvalidCandidateWindow */
+    YM_FIT_DUPLICATE_CANDIDATE = 123,  /* ScalarFitError::DuplicateCandidateWindow */
+    YM_FIT_DEGENERATE_PLATEAU = 124,   /* ScalarFitError::DegeneratePlateauUncertainty */
+    YM_FIT_NO_PLATEAU = 125,           /* ScalarFitError::NoAcceptedPlateauWindow */
+    YM_FIT_INSUFFICIENT_REPLICAS = 126, /* ScalarFitError::InsufficientReplicatesForCovariance */
+    YM_FIT_COVARIANCE_NOT_PD = 127,    /* ScalarFitError::CovarianceNotPositiveDefinite */
+    YM_FIT_SINGULAR_COVARIANCE = 128,  /* ScalarFitError::SingularCovariance */
+    YM_FIT_NON_FINITE_STATISTIC = 129, /* ScalarFitError::NonFiniteStatistic */
+    YM_FIT_NO_GRID_FIT = 130,          /* ScalarFitError::NoFiniteGridFit */
+    YM_VAR_TIME_EXTENT_TOO_SMALL = 140, /* VariationalError::TimeExtentTooSmall (T < 3) */
+    YM_VAR_MATRIX_NOT_SYMMETRIC = 142, /* VariationalError::MatrixNotSymmetric */
+    YM_VAR_INVALID_REFERENCE_TIMES = 143, /* VariationalError::InvalidReferenceTimes */
+    YM_VAR_REFERENCE_NOT_PD = 144,     /* VariationalError::ReferenceMatrixNotPositiveDefinite */
+    YM_VAR_DEGENERATE_EIGENVECTOR = 145, /* VariationalError::DegeneratePrincipalEigenvector */
+    YM_VAR_NON_POSITIVE_EIGENVALUE = 146, /* VariationalError::NonPositivePrincipalEigenvalue */
+    YM_VAR_PROJECTION_NORMALIZATION = 147, /* VariationalError::ProjectionNormalizationFailure */
     YM_WRONG_CONFIGURATION = 90,      /* benchmark gate: schedule does not allow the split */
     YM_STATIONARITY_FAILED = 91,      /* benchmark gate: constant half or zero spread */
     YM_INVALID_ARGUMENT = 96,         /* null pointer, wrong group/precision, size mismatch */
@@ -192,6 +223,17 @@ int32_t ym_su3_timeslice_a1pp(const ym_lattice *lat, double *plaq_t, double *rec
  * dst (same shape; src is not modified; temporal links copied).  The
  * reference's 8-level cap is lifted to 64. */
 int32_t ym_su3_stout_spatial(ym_lattice *dst, const ym_lattice *src, double rho, int32_t levels);
+
+/* glueball_blocking.rs:71-94 blocked_extents: coarse extents (N/2, N/2, N/2,
+ * N3) of src; fails with YM_SPATIAL_NOT_CUBIC, YM_SPATIAL_EXTENT_TOO_SMALL
+ * (N < 6) or YM_SPATIAL_EXTENT_NOT_EVEN, in the reference's order. */
+int32_t ym_su3_blocked_extents(const ym_lattice *src, int64_t coarse[4]);
+/* glueball_blocking.rs:105-136 spatial_stride2_block: dst (created with the
+ * blocked extents and src's precision) receives V_mu(y) = U_mu(x) U_mu(x+mu)
+ * for spatial mu and V_3(y) = U_3(x), x = (2y0, 2y1, 2y2, y3).  The product
+ * is formed in the storage precision; the reference's per-link group check
+ * is left to ym_link_residuals. */
+int32_t ym_su3_spatial_block(ym_lattice *dst, const ym_lattice *src);
 /* su3_lattice.rs:285-297 max_link_{unitarity,determinant}_residual (SU(3));
  * lib.rs:696-712 max_link_unit_residual in *unit for SU(2) (*det = 0). */
 int32_t ym_link_residuals(const ym_lattice *lat, double *unit, double *det);
@@ -385,6 +427,62 @@ int32_t ym_connected_correlator(const double *samples, int64_t nconf, int64_t T,
  * per lag.  replicas (nconf x T) may be NULL. */
 int32_t ym_jackknife_correlator(const double *samples, int64_t nconf, int64_t T, double *full, double *replicas,
                                 double *jackknife_mean, double *jackknife_error);
+
+
+/* ------------------------------------------- host FP64 correlator analysis
+ * Replica correlators are passed as nrep row pointers with their lengths
+ * (the reference's &[Vec<f64>]); every row must hold T values. */
+typedef struct {
+    int64_t tau;
+    double full_estimate, jackknife_mean, standard_error;
+} ym_effective_mass_point;
+/* glueball_effective_mass.rs:161-232 cosh_effective_mass_with_jackknife:
+ * m(tau) = acosh((C(tau-1) + C(tau+1)) / (2 C(tau))) at strictly increasing
+ * lags 0 < tau < T/2 on the full correlator and on each delete-one replica;
+ * standard error sqrt((N-1)/N sum_i (m_i - m_bar)^2).  4 <= T <= 64,
+ * 3 <= nrep <= 64.  out holds nlags points. */
+int32_t ym_cosh_effective_mass(const double *full, int64_t T, const double *const *replicas,
+                               const int64_t *replica_lengths, int64_t nrep, const int64_t *lags, int64_t nlags,
+                               ym_effective_mass_point *out);
+/* glueball_effective_mass.rs:240-268 analytic_periodic_single_state:
+ * out[tau] = A (exp(-m tau) + exp(-m (T - tau))), tau = 0..T-1. */
+int32_t ym_analytic_periodic_single_state(int64_t T, double mass, double amplitude, double *out);
+
+typedef struct {
+    int64_t window_start, window_end;
+    double plateau_weighted_effective_mass, plateau_maximum_z;
+    double fitted_mass, fitted_amplitude, chi_square;
+    int64_t degrees_of_freedom;
+    double chi_square_per_dof;
+    double mass_jackknife_mean, mass_standard_error;
+    double amplitude_jackknife_mean, amplitude_standard_error;
+    double maximum_off_diagonal_correlation;
+} ym_scalar_fit_result;
+/* glueball_scalar_fit.rs:156-650 correlated_scalar_fit: among nwin
+ * predeclared windows (pairs start, end; 3..8 points, start >= 1,
+ * 2 end < T, unique, at most 16) the longest whose effective masses stay
+ * within 2.5 jackknife errors of their inverse-variance mean (ties: later
+ * start) is fitted with C(tau) = A (exp(-m tau) + exp(-m (T - tau))) by
+ * correlated chi^2 (jackknife covariance of the window, Gauss-Jordan
+ * inverse) profiled over the mass grid 0.05:0.001:2.0; each replica is
+ * refitted with the same inverse for the jackknife errors. */
+int32_t ym_correlated_scalar_fit(const double *full, int64_t T, const double *const *replicas,
+                                 const int64_t *replica_lengths, int64_t nrep, const int64_t *windows, int64_t nwin,
+                                 ym_scalar_fit_result *out);
+/* glueball_scalar_variational.rs:628-744 scalar_variational_connected_correlator
+ * from the A1++ operator samples (as ym_su3_timeslice_a1pp returns them):
+ * samples is [nconf][T][2] (plaquette, rectangle).  matrices receives T
+ * symmetric 2 x 2 matrices (row-major, 4 doubles each),
+ * C_ij(tau) = 1/2 [<O_i(t) O_j(t+tau)> + <O_j(t) O_i(t+tau)>] - <O_i><O_j>;
+ * vacuum_means receives <O_0>, <O_1>.  1 <= nconf <= 64, 3 <= T <= 64. */
+int32_t ym_variational_connected_correlator(const double *samples, int64_t nconf, int64_t T, double *vacuum_means,
+                                            double *matrices);
+/* glueball_scalar_variational.rs:746-925 principal_generalized_projection:
+ * largest lambda of C(t1) v = lambda C(t0) v with (t0, t1) = (1, 2), v
+ * normalised to v^T C(t0) v = 1 (first, else second, component positive);
+ * projected[t] = v^T C(t) v for t = 0..T-1. */
+int32_t ym_principal_generalized_projection(const double *matrices, int64_t T, int64_t t0, int64_t t1,
+                                            double *eigenvalue, double *coefficients, double *projected);
 
 /* su2_ensemble.rs:46-55 mix64 / derive_chain_seed and su3_ensemble.rs:189-192. */
 uint64_t ym_derive_chain_seed(uint64_t master_seed, uint64_t chain_id);

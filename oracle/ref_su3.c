@@ -866,6 +866,40 @@ int ymo_su3_stout_smear(const int64_t n[4], const double *in, double *out, doubl
     free(tmp);
     return st;
 }
+/* ----------------------------------------------- spatial stride-2 blocking */
+/* glueball_blocking.rs:71-94 -- coarse extents; checks in the reference's order. */
+int ymo_su3_blocked_extents(const int64_t n[4], int64_t coarse[4]) {
+    if (n[0] != n[1] || n[1] != n[2]) return YMO_SPATIAL_NOT_CUBIC;
+    if (n[0] < 6) return YMO_SPATIAL_EXTENT_TOO_SMALL;
+    if (n[0] % 2 != 0) return YMO_SPATIAL_EXTENT_NOT_EVEN;
+    for (int d = 0; d < 3; ++d) coarse[d] = n[d] / 2;
+    coarse[3] = n[3];
+    return YMO_OK;
+}
+/* glueball_blocking.rs:105-136 -- V_mu(y) = U_mu(x) U_mu(x+mu) (checked
+ * product, then the 5e-12 group check), V_3(y) = U_3(x), x = (2y0,2y1,2y2,y3). */
+int ymo_su3_spatial_block(const int64_t n[4], const double *in, double *out) {
+    int64_t c[4];
+    int st = ymo_su3_blocked_extents(n, c);
+    if (st) return st;
+    lat3 f = {{n[0], n[1], n[2], n[3]}, (double *)in};
+    lat3 g = {{c[0], c[1], c[2], c[3]}, out};
+    int64_t y[4];
+    FOR_SITES(&g, y) {
+        const int64_t x[4] = {2 * y[0], 2 * y[1], 2 * y[2], y[3]};
+        for (int mu = 0; mu < 3; ++mu) {
+            int64_t xp[4];
+            nb_plus(&f, x, mu, xp);
+            mat3 v;
+            if ((st = su3_mul(link_get(&f, x, mu), link_get(&f, xp, mu), &v))) return st;
+            if (m_unitarity_residual(v) > SU3_ACCEPTANCE_TOLERANCE || m_determinant_residual(v) > SU3_ACCEPTANCE_TOLERANCE)
+                return YMO_SU3_NON_UNITARY;
+            link_set(&g, y, mu, v);
+        }
+        link_set(&g, y, 3, link_get(&f, x, 3));
+    }
+    return YMO_OK;
+}
 /* glueball_smearing.rs:270-300 */
 double ymo_su3_max_link_residual(const int64_t n[4], const double *a, const double *b) {
     int64_t count = lat_sites(n) * 4;

@@ -62,6 +62,7 @@ enum {
     YMO_SPATIAL_NOT_CUBIC = 52,
     YMO_COUNT_OVERFLOW = 53,
     YMO_NON_FINITE_OBSERVABLE = 54,
+    YMO_SPATIAL_EXTENT_NOT_EVEN = 55,
     YMO_NON_FINITE_RHO = 60,
     YMO_RHO_OUT_OF_RANGE = 61,
     YMO_TOO_MANY_LEVELS = 62,
@@ -174,6 +175,8 @@ int ymo_su3_rect_wilson_loop(const int64_t n[4], const double *links, const int6
 int ymo_su3_a1pp_basis(const int64_t n[4], const double *links, int64_t t, double *plaq, double *rect, int64_t *plaq_count, int64_t *rect_count);
 int ymo_su3_stout_smear(const int64_t n[4], const double *in, double *out, double rho, int64_t levels, int64_t max_levels);
 double ymo_su3_max_link_residual(const int64_t n[4], const double *a, const double *b);
+int ymo_su3_blocked_extents(const int64_t n[4], int64_t coarse[4]);
+int ymo_su3_spatial_block(const int64_t n[4], const double *in, double *out);
 
 /* ---- reference-exact: SU(2) ---- */
 int ymo_su2_from_axis_angle(const double axis[3], double angle, double out[4]);

@@ -124,6 +124,8 @@ def lib():
         "ymo_su3_a1pp_basis": (C.c_int, [ip, dp, i64, dp, dp, ip, ip]),
         "ymo_su3_stout_smear": (C.c_int, [ip, dp, dp, dbl, i64, i64]),
         "ymo_su3_max_link_residual": (dbl, [ip, dp, dp]),
+        "ymo_su3_blocked_extents": (C.c_int, [ip, ip]),
+        "ymo_su3_spatial_block": (C.c_int, [ip, dp, dp]),
         "ymo_su2_from_axis_angle": (C.c_int, [dp, dbl, dp]),
         "ymo_su2_mul": (None, [dp, dp, dp]),
         "ymo_su2_identity": (C.c_int, [ip, dp]),

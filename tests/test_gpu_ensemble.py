@@ -89,6 +89,41 @@ def test_production_pipeline_small(ym):
         assert np.all(np.isfinite(ev.jackknife_error))
         # C(tau) is symmetric under tau -> T - tau (periodic estimator)
         assert np.allclose(ev.full[1:], ev.full[1:][::-1], rtol=0, atol=1e-14)
+    # host analysis of each (geometry, level): results or the reference's error status,
+    # the same as the analysis of the same samples run on their own
+    assert set(rep.analysis) == {(g, lev) for g in rep.geometries for lev in range(3)}
+    for (g, lev), an in rep.analysis.items():
+        var = an["variational"]
+        assert isinstance(var, ym.stats.VariationalCorrelator) and var.matrices.shape == (g[3], 2, 2)
+        assert np.allclose(var.matrices[:, 0, 0], rep.correlators[(g, lev, 0)].full, rtol=0, atol=2e-12)
+        for key in ("effective_mass", "fit", "projection"):
+            r = an[key]
+            assert not isinstance(r, dict) or r["status"] >= 100 or r["status"] in (54, 74, 75), (key, r)
+
+
+def test_correlator_analysis_on_device_samples(ym):
+    """A1++ samples of 64 small chains measured on the device feed the host
+    analysis: the variational diagonal replays the jackknife correlators and
+    the GEVP, when admitted, returns a normalised projection."""
+    ens = ym.Ensemble.from_chain_ids([(4, 4, 4, 8)] * 64, list(range(64)), 0x5EED)
+    try:
+        res = ens.run_cycles(5.7, 0.35, sweeps_per_cycle=20, n_cycles=1, stout_levels=1)
+    finally:
+        ens.close()
+    pl = np.array([res.ops[0][c][1, :, 0] for c in range(64)])
+    rc = np.array([res.ops[0][c][1, :, 1] for c in range(64)])
+    an = ym.ensemble.correlator_analysis(pl, rc)
+    var = an["variational"]
+    for k, ev in ((0, an["plaquette"]), (1, an["rectangle"])):
+        assert np.max(np.abs(var.matrices[:, k, k] - ev.full)) <= 2e-12
+    pr = an["projection"]
+    if isinstance(pr, ym.stats.PrincipalProjection):
+        assert pr.projected_correlator[1] == 1.0 and pr.principal_eigenvalue > 0
+        c = pr.coefficients
+        m1 = var.matrices[1]
+        assert abs(c @ (0.5 * (m1 + m1.T)) @ c - 1.0) <= 1e-10
+    else:
+        assert pr["status"] in (144, 145, 146, 147)
 
 
 def test_ensemble_rejects_bad_arguments(ym):

@@ -410,6 +410,56 @@ def test_stout_fp64_vs_reference_hot(ym, levels):
     assert float(np.max(np.abs(out - ref))) <= 1e-13
 
 
+def test_spatial_block_frozen_fixture(ym):
+    """glueball-spatial-block-phase3b2-v1.json on the device: identity blocks
+    to identity, the cycle fixture gives the frozen A1++ outputs (1e-12), the
+    temporal link is sampled exactly, geometries are rejected in order."""
+    n = (6, 6, 6, 3)
+    fine = ym.Su3Lattice4(n, 64)
+    coarse = ym.spatial_stride2_block(fine)
+    assert coarse.extents == (3, 3, 3, 3)
+    assert np.array_equal(coarse.download(), O.su3_identity((3, 3, 3, 3)))
+    cyc = np.zeros(18)
+    O.lib().ymo_su3_traceless_cycle_link(O.dptr(cyc))
+    aos = O.su3_identity(n)
+    aos[0:18] = cyc
+    off3 = O.link_offset(n, (2, 4, 0, 1), 3, 18)
+    aos[off3:off3 + 18] = cyc
+    fine.upload(aos)
+    coarse = ym.spatial_stride2_block(fine)
+    pl, rc = coarse.timeslice_a1pp()
+    assert abs(pl[0] - 0.9506172839506173) <= 1e-12 and abs(rc[0] - 0.9259259259259259) <= 1e-12
+    got = coarse.download()
+    assert np.array_equal(got[O.link_offset((3, 3, 3, 3), (1, 2, 0, 1), 3, 18):][:18], cyc)
+    for bad, code in (((4, 4, 4, 3), 51), ((7, 7, 7, 3), 55), ((6, 8, 6, 3), 52)):
+        with pytest.raises(ym.YmError) as e:
+            ym.spatial_stride2_block(ym.Su3Lattice4(bad, 64))
+        assert e.value.status == code
+
+
+@pytest.mark.parametrize("prec,n", [(64, (8, 8, 8, 6)), (64, (6, 6, 6, 5)), (32, (12, 12, 12, 8))])
+def test_spatial_block_vs_reference(ym, prec, n):
+    """Hot-start configurations (odd N3 included): FP64 storage matches the
+    reference-exact blocking to 1e-13; FP32 storage to its rounding."""
+    lat = ym.Su3Lattice4(n, prec)
+    lat.hot_start(ym.derive_su3_chain_seed(0xB10C, prec))
+    aos = lat.download()
+    ref_aos = aos.copy()
+    if prec == 32:  # the reference's 5e-12 group check needs FP64-unitary links
+        r64 = ym.Su3Lattice4(n, 64)
+        r64.upload(aos)
+        r64.reunitarize()
+        ref_aos = r64.download()
+    nc = (n[0] // 2, n[1] // 2, n[2] // 2, n[3])
+    ref = np.zeros(int(np.prod(nc)) * 4 * 18)
+    assert O.lib().ymo_su3_spatial_block(O.ext(n), O.dptr(ref_aos), O.dptr(ref)) == 0
+    got = ym.spatial_stride2_block(lat).download()
+    assert float(np.max(np.abs(got - ref))) <= (1e-13 if prec == 64 else 2e-6)
+    if prec == 64:
+        u, d = ym.spatial_stride2_block(lat).link_residuals()
+        assert u <= 1e-13 and d <= 1e-13
+
+
 def test_residuals_fp64_vs_reference(ym):
     n = (4, 4, 2, 4)
     lat, aos = hot_fp64(ym, n, 2)

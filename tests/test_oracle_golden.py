@@ -485,6 +485,86 @@ def test_stout_frozen_vectors():
     assert L.ymo_su3_stout_smear(O.ext(n), O.dptr(links), O.dptr(out), 0.1, 9, 0) == 62
 
 
+def _block(n, links):
+    c = (C.c_int64 * 4)()
+    assert L.ymo_su3_blocked_extents(O.ext(n), c) == 0
+    nc = tuple(c)
+    out = np.zeros(int(np.prod(nc)) * 4 * 18)
+    assert L.ymo_su3_spatial_block(O.ext(n), O.dptr(links), O.dptr(out)) == 0
+    return nc, out
+
+
+def test_spatial_block_frozen_vectors():
+    """glueball-spatial-block-phase3b2-v1.json (glueball_blocking.rs:105-136)."""
+    n = (6, 6, 6, 3)
+    nc, out = _block(n, O.su3_identity(n))
+    assert nc == (3, 3, 3, 3)
+    ident = O.su3_identity(nc)
+    assert L.ymo_su3_max_link_residual(O.ext(nc), O.dptr(out), O.dptr(ident)) == 0.0
+    # canonical output: a cycle link at the fine origin, direction 0
+    v = vec("glueball-spatial-block-phase3b2-v1", "canonical-coarse-operator-output")
+    fine = O.su3_identity(n)
+    fine[0:18] = cycle()
+    nc, out = _block(n, fine)
+    p, r, pc, rc = C.c_double(), C.c_double(), C.c_int64(), C.c_int64()
+    assert L.ymo_su3_a1pp_basis(O.ext(nc), O.dptr(out), 0, C.byref(p), C.byref(r), C.byref(pc), C.byref(rc)) == 0
+    e = v["expected"]
+    assert abs(p.value - e["spatial_plaquette_1x1"]) <= v["absolute_tolerance"]
+    assert abs(r.value - e["spatial_rectangle_2x1"]) <= v["absolute_tolerance"]
+    # the coarse link at the origin is the cycle times the identity
+    assert np.array_equal(out[0:18], cycle())
+    u, d = C.c_double(), C.c_double()
+    L.ymo_su3_max_residuals(O.ext(nc), O.dptr(out), C.byref(u), C.byref(d))
+    assert u.value <= 5e-12 and d.value <= 5e-12
+    # temporal link sampled exactly
+    ft = O.su3_identity(n)
+    off = O.link_offset(n, (0, 0, 0, 0), 3, 18)
+    ft[off:off + 18] = cycle()
+    nc, out = _block(n, ft)
+    assert np.array_equal(out[O.link_offset(nc, (0, 0, 0, 0), 3, 18):][:18], cycle())
+    # gauge covariance under the coarse restriction y -> (2y0, 2y1, 2y2, y3)
+    g = su3_gauge_field(n)
+    t = np.zeros_like(fine)
+    L.ymo_su3_gauge_transform(O.ext(n), O.dptr(fine), O.dptr(g), O.dptr(t))
+    _, bt = _block(n, t)
+    gc = np.zeros(int(np.prod(nc)) * 18)
+    for i, y in enumerate(np.ndindex(*nc)):
+        x = (2 * y[0], 2 * y[1], 2 * y[2], y[3])
+        assert L.ymo_su3_deterministic_site_gauge((C.c_int64 * 4)(*x), O.dptr(gc[i * 18:(i + 1) * 18])) == 0
+    _, b = _block(n, fine)
+    tb = np.zeros_like(b)
+    L.ymo_su3_gauge_transform(O.ext(nc), O.dptr(b), O.dptr(gc), O.dptr(tb))
+    assert L.ymo_su3_max_link_residual(O.ext(nc), O.dptr(bt), O.dptr(tb)) <= 5e-11
+    # rejected geometries, in the reference's order
+    c = (C.c_int64 * 4)()
+    for bad, code in (((4, 4, 4, 3), 51), ((7, 7, 7, 3), 55), ((6, 8, 6, 3), 52)):
+        assert L.ymo_su3_blocked_extents(O.ext(bad), c) == code
+        lb = O.su3_identity(bad)
+        assert L.ymo_su3_spatial_block(O.ext(bad), O.dptr(lb), O.dptr(np.zeros(18))) == code
+
+
+def test_spatial_block_is_two_step_holonomy():
+    """A coarse spatial link equals the fine two-step path holonomy from the
+    embedded site (path_holonomy.rs:132-170), on a random SU(3) field."""
+    n = (6, 6, 6, 4)
+    rng = np.random.default_rng(11)
+    links = O.su3_identity(n)
+    for i in range(links.size // 18):
+        links[i * 18:(i + 1) * 18] = O.su3_from_algebra(rng.uniform(-0.4, 0.4, 8))
+    nc, out = _block(n, links)
+    h = np.zeros(18)
+    endp = (C.c_int64 * 4)()
+    for y in [(0, 0, 0, 0), (1, 2, 0, 3), (2, 2, 2, 1)]:
+        x = (2 * y[0], 2 * y[1], 2 * y[2], y[3])
+        for mu in range(3):
+            steps = (C.c_int32 * 2)(mu + 1, mu + 1)
+            assert L.ymo_su3_path_holonomy(O.ext(n), O.dptr(links), (C.c_int64 * 4)(*x), steps, 2, O.dptr(h), endp) == 0
+            off = O.link_offset(nc, y, mu, 18)
+            assert np.max(np.abs(out[off:off + 18] - h)) <= 1e-15
+        off = O.link_offset(nc, y, 3, 18)
+        assert np.array_equal(out[off:off + 18], links[O.link_offset(n, x, 3, 18):][:18])
+
+
 def test_connected_correlator_vectors():
     v = vec("glueball-connected-correlator-phase3c-v1", "shifted-defect-analytic-correlators")
     n = (3, 3, 3, 4)

@@ -142,6 +142,7 @@ const char *ym_status_string(int32_t s) {
     case YM_TIME_SLICE_OUT_OF_RANGE: return "Euclidean time slice is outside the lattice";
     case YM_SPATIAL_EXTENT_TOO_SMALL: return "every spatial extent must be at least 3";
     case YM_SPATIAL_NOT_CUBIC: return "spatial extents must be equal";
+    case YM_SPATIAL_EXTENT_NOT_EVEN: return "stride-2 blocking requires even spatial extents";
     case YM_NON_FINITE_RHO: return "stout rho must be finite";
     case YM_RHO_OUT_OF_RANGE: return "stout rho must satisfy 0 < rho <= 0.25";
     case YM_TOO_MANY_LEVELS: return "stout level count exceeds the limit";
@@ -158,6 +159,36 @@ const char *ym_status_string(int32_t s) {
     case YM_EFF_INVALID_DIAGNOSTIC: return "chain diagnostics rejected the series";
     case YM_EFF_DEGENERATE: return "series is constant";
     case YM_EFF_NON_FINITE: return "efficiency report is non-finite or non-positive";
+    case YM_EM_TIME_EXTENT_TOO_SMALL: return "effective mass requires time extent >= 4";
+    case YM_EM_TIME_EXTENT_TOO_LARGE: return "effective-mass time extent exceeds 64";
+    case YM_EM_TOO_FEW_REPLICAS: return "at least three delete-one jackknife replicas are required";
+    case YM_EM_TOO_MANY_REPLICAS: return "jackknife replica count exceeds 64";
+    case YM_EM_EMPTY_WINDOW: return "analysis lag set must not be empty";
+    case YM_EM_INVALID_LAG: return "lags must be strictly increasing and lie before the temporal midpoint";
+    case YM_EM_SHAPE_MISMATCH: return "full and jackknife correlators must share one time extent";
+    case YM_EM_NON_FINITE_CORRELATOR: return "correlator input must be finite";
+    case YM_EM_NON_POSITIVE_CENTRE: return "cosh effective mass requires C(tau) > 0";
+    case YM_EM_ACOSH_DOMAIN: return "cosh effective-mass acosh argument must be >= 1";
+    case YM_EM_NON_FINITE_STATISTIC: return "effective-mass statistic became non-finite";
+    case YM_EM_INVALID_FIXTURE: return "analytic fixture parameters are invalid";
+    case YM_FIT_EMPTY_CANDIDATES: return "at least one candidate window is required";
+    case YM_FIT_TOO_MANY_CANDIDATES: return "candidate-window count exceeds 16";
+    case YM_FIT_INVALID_CANDIDATE: return "candidate windows must hold 3..8 points, start >= 1 and end before the midpoint";
+    case YM_FIT_DUPLICATE_CANDIDATE: return "candidate windows must be unique";
+    case YM_FIT_DEGENERATE_PLATEAU: return "plateau selection requires positive finite jackknife errors";
+    case YM_FIT_NO_PLATEAU: return "no candidate window passed the plateau criterion";
+    case YM_FIT_INSUFFICIENT_REPLICAS: return "more jackknife replicas than fitted time points are required";
+    case YM_FIT_COVARIANCE_NOT_PD: return "jackknife covariance must be positive definite";
+    case YM_FIT_SINGULAR_COVARIANCE: return "jackknife covariance inversion failed";
+    case YM_FIT_NON_FINITE_STATISTIC: return "fit statistic became non-finite";
+    case YM_FIT_NO_GRID_FIT: return "mass grid produced no admissible fit";
+    case YM_VAR_TIME_EXTENT_TOO_SMALL: return "variational correlator requires at least three time slices";
+    case YM_VAR_MATRIX_NOT_SYMMETRIC: return "correlator matrix violates the symmetry tolerance";
+    case YM_VAR_INVALID_REFERENCE_TIMES: return "reference and diagonalization times must be 1 and 2, inside the series";
+    case YM_VAR_REFERENCE_NOT_PD: return "reference correlator matrix is not positive definite";
+    case YM_VAR_DEGENERATE_EIGENVECTOR: return "principal generalized eigenvector is not unique";
+    case YM_VAR_NON_POSITIVE_EIGENVALUE: return "principal generalized eigenvalue is not positive";
+    case YM_VAR_PROJECTION_NORMALIZATION: return "projected reference correlator is not normalized";
     case YM_WRONG_CONFIGURATION: return "schedule does not admit the half split";
     case YM_STATIONARITY_FAILED: return "stationarity gate failed (constant half or zero spread)";
     case YM_INVALID_ARGUMENT: return "invalid argument";
@@ -665,6 +696,32 @@ int32_t ym_su3_stout_spatial(ym_lattice *dst, const ym_lattice *src, double rho,
     int32_t st = finish(dst);
     if (tmp) (void)hipFree(tmp);
     return st;
+}
+
+int32_t ym_su3_blocked_extents(const ym_lattice *src, int64_t coarse[4]) {
+    if (!src || !coarse) return YM_INVALID_ARGUMENT;
+    if (src->group != 3) return YM_UNSUPPORTED;
+    // glueball_blocking.rs:71-94, checks in the reference's order.
+    if (src->n[0] != src->n[1] || src->n[1] != src->n[2]) return YM_SPATIAL_NOT_CUBIC;
+    if (src->n[0] < 6) return YM_SPATIAL_EXTENT_TOO_SMALL;
+    if (src->n[0] % 2 != 0) return YM_SPATIAL_EXTENT_NOT_EVEN;
+    for (int d = 0; d < 3; ++d) coarse[d] = src->n[d] / 2;
+    coarse[3] = src->n[3];
+    return YM_OK;
+}
+
+int32_t ym_su3_spatial_block(ym_lattice *dst, const ym_lattice *src) {
+    if (!dst || !src || dst == src) return YM_INVALID_ARGUMENT;
+    int64_t c[4];
+    int32_t st = ym_su3_blocked_extents(src, c);
+    if (st) return st;
+    if (dst->group != 3 || dst->prec != src->prec) return YM_INVALID_ARGUMENT;
+    for (int d = 0; d < 4; ++d)
+        if (dst->n[d] != c[d]) return YM_INVALID_ARGUMENT;
+    YM_HIP(hipStreamSynchronize(src->stream));
+    ym::launch_block2(src->prec, src->links, dst->links, src->g, dst->g, dst->stream);
+    YM_HIP(hipGetLastError());
+    return finish(dst);
 }
 
 int32_t ym_link_residuals(const ym_lattice *lc, double *unit, double *det) {

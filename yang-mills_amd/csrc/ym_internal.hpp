@@ -39,6 +39,7 @@ int launch_wilson_loops(int group, int prec, const void *links, const Geom &g, i
 int launch_polyakov(int group, int prec, const void *links, const Geom &g, int dir, double *part, hipStream_t st);
 int launch_a1pp(int prec, const void *links, const Geom &g, double *part, hipStream_t st);
 void launch_stout(int prec, const void *in, void *out, const Geom &g, double rho, hipStream_t st);
+void launch_block2(int prec, const void *in, void *out, const Geom &gf, const Geom &gc, hipStream_t st);
 void launch_path_holonomy(int group, int prec, const void *links, const Geom &g, const int x[4], const int *steps,
                           int nsteps, double *out, hipStream_t st);
 void launch_sum_partials(const double *part, int n, int ncols, int stride, double *out, hipStream_t st);

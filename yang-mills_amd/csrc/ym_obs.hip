@@ -286,6 +286,37 @@ void launch_stout(int prec, const void *in, void *out, const Geom &g, double rho
     else hipLaunchKernelGGL((k_stout<double>), grid, dim3(128), 0, st, (const double2 *)in, (double2 *)out, g, rho);
 }
 
+// ------------------------------------------------ spatial stride-2 blocking
+// glueball_blocking.rs:105-136: coarse site y embeds at x = (2y0, 2y1, 2y2,
+// y3); V_mu(y) = U_mu(x) U_mu(x + mu) for spatial mu, V_3(y) = U_3(x).  One
+// thread per coarse site of parity p (grid.y); the fine lattice gf, the
+// coarse gc.  The product runs in the storage precision.  Every coarse slot
+// is written once; the padding slot of an odd N3 is left alone.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_block2(const typename Vec2<Real>::type *__restrict__ in,
+                                                typename Vec2<Real>::type *__restrict__ out, Geom gf, Geom gc) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    int y[4];
+    if (s >= gc.vh || !site_of(gc, p, s, y)) return;
+    const int x[4] = {2 * y[0], 2 * y[1], 2 * y[2], y[3]};
+    const int px = parity_of(x), sx = slot_of(gf, x);
+#pragma unroll 1
+    for (int mu = 0; mu < 3; ++mu) {
+        int xp[4];
+        shifted(gf, x, mu, 1, xp);
+        const M3<Real> a = load3<Real>(in, gf.vh, mu, px, sx);
+        const M3<Real> b = load3<Real>(in, gf.vh, mu, px ^ 1, slot_of(gf, xp));
+        store3<Real>(out, gc.vh, mu, p, s, mul_nn(a, b));
+    }
+    store3<Real>(out, gc.vh, 3, p, s, load3<Real>(in, gf.vh, 3, px, sx));
+}
+void launch_block2(int prec, const void *in, void *out, const Geom &gf, const Geom &gc, hipStream_t st) {
+    dim3 grid((gc.vh + 255) / 256, 2);
+    if (prec == 32) hipLaunchKernelGGL((k_block2<float>), grid, dim3(256), 0, st, (const float2 *)in, (float2 *)out, gf, gc);
+    else hipLaunchKernelGGL((k_block2<double>), grid, dim3(256), 0, st, (const double2 *)in, (double2 *)out, gf, gc);
+}
+
 // ------------------------------------------------------- SU(2) residual
 template <typename Real>
 __global__ __launch_bounds__(256) void k_su2_residual(const typename Vec4<Real>::type *__restrict__ L, Geom g,

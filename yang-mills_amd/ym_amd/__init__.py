@@ -5,7 +5,7 @@ The names follow the reference's Rust module functions (reference/rust/src/):
 ``metropolis_sweep`` (su3_ensemble.rs:342), ``heatbath_sweep`` /
 ``overrelaxation_sweep`` (su2_ensemble.rs:321-346), ``average_plaquette``
 (su2_ensemble.rs:348, su3_ensemble.rs:228), ``spatial_stout_smear``
-(glueball_smearing.rs:257), ``scalar_a1pp_operator_basis``
+(glueball_smearing.rs:257), ``spatial_stride2_block`` (glueball_blocking.rs:110), ``scalar_a1pp_operator_basis``
 (glueball_operators.rs:143), ``run_chain`` (su3_ensemble.rs:408) and
 ``diagnostics`` (su2_ensemble.rs:375).  Every call goes through the C ABI
 declared in include/ym_amd.h; there is no CPU fallback: importing this package
@@ -23,6 +23,7 @@ from .lattice import (  # noqa: F401
     Su3Lattice4,
     ChainReport,
     average_plaquette,
+    blocked_extents,
     derive_chain_seed,
     derive_su3_chain_seed,
     diagnostics,
@@ -32,4 +33,5 @@ from .lattice import (  # noqa: F401
     run_chain,
     scalar_a1pp_operator_basis,
     spatial_stout_smear,
+    spatial_stride2_block,
 )

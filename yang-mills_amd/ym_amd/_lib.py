@@ -44,6 +44,18 @@ class StationarityReport(C.Structure):
                 ("second_half_se", dbl), ("drift_z", dbl)]
 
 
+class EffectiveMassPoint(C.Structure):
+    _fields_ = [("tau", i64), ("full_estimate", dbl), ("jackknife_mean", dbl), ("standard_error", dbl)]
+
+
+class ScalarFitResult(C.Structure):
+    _fields_ = [("window_start", i64), ("window_end", i64), ("plateau_weighted_effective_mass", dbl),
+                ("plateau_maximum_z", dbl), ("fitted_mass", dbl), ("fitted_amplitude", dbl), ("chi_square", dbl),
+                ("degrees_of_freedom", i64), ("chi_square_per_dof", dbl), ("mass_jackknife_mean", dbl),
+                ("mass_standard_error", dbl), ("amplitude_jackknife_mean", dbl),
+                ("amplitude_standard_error", dbl), ("maximum_off_diagonal_correlation", dbl)]
+
+
 class PoolSummary(C.Structure):
     _fields_ = [("pooled_mean", dbl), ("pooled_se", dbl), ("aggregate_ess", dbl), ("max_pairwise_z", dbl),
                 ("max_stationarity_z", dbl), ("min_ess", dbl)]
@@ -73,6 +85,8 @@ SIGNATURES = {
     "ym_polyakov_loop": (i32, [C.c_void_p, i32, P(dbl), P(dbl), P(dbl)]),
     "ym_su3_timeslice_a1pp": (i32, [C.c_void_p, P(dbl), P(dbl)]),
     "ym_su3_stout_spatial": (i32, [C.c_void_p, C.c_void_p, dbl, i32]),
+    "ym_su3_blocked_extents": (i32, [C.c_void_p, P(i64)]),
+    "ym_su3_spatial_block": (i32, [C.c_void_p, C.c_void_p]),
     "ym_link_residuals": (i32, [C.c_void_p, P(dbl), P(dbl)]),
     "ym_reunitarize": (i32, [C.c_void_p]),
     "ym_su3_run_chain": (i32, [P(Su3ChainConfig), P(dbl), P(Su3ChainReport)]),
@@ -109,6 +123,11 @@ SIGNATURES = {
     "ym_periodic_connected": (i32, [P(dbl), i64, i64, P(dbl), P(dbl), P(dbl)]),
     "ym_connected_correlator": (i32, [P(dbl), i64, i64, i64, P(dbl)]),
     "ym_jackknife_correlator": (i32, [P(dbl), i64, i64, P(dbl), P(dbl), P(dbl), P(dbl)]),
+    "ym_cosh_effective_mass": (i32, [P(dbl), i64, P(P(dbl)), P(i64), i64, P(i64), i64, P(EffectiveMassPoint)]),
+    "ym_analytic_periodic_single_state": (i32, [i64, dbl, dbl, P(dbl)]),
+    "ym_correlated_scalar_fit": (i32, [P(dbl), i64, P(P(dbl)), P(i64), i64, P(i64), i64, P(ScalarFitResult)]),
+    "ym_variational_connected_correlator": (i32, [P(dbl), i64, i64, P(dbl), P(dbl)]),
+    "ym_principal_generalized_projection": (i32, [P(dbl), i64, i64, i64, P(dbl), P(dbl), P(dbl)]),
     "ym_derive_chain_seed": (u64, [u64, u64]),
     "ym_derive_su3_chain_seed": (u64, [u64, u64]),
     "ym_timer_begin": (i32, [C.c_void_p]),

@@ -136,6 +136,42 @@ class ProductionReport:
     correlators: dict              # (geometry, level, kind) -> stats.CorrelatorEvidence
     device_ms: float
     sweeps: int
+    analysis: dict = None          # (geometry, level) -> correlator_analysis(...)
+
+
+def _attempt(fn, *a):
+    """Result of an analysis step, or the status of the reference error it fails with."""
+    from ._lib import YmError
+    try:
+        return fn(*a)
+    except YmError as e:
+        return {"status": e.status, "error": str(e)}
+
+
+def correlator_analysis(plaquette, rectangle, candidate_windows=None):
+    """The host FP64 analysis of one ensemble's A1++ samples ((nconf, T)
+    arrays of per-timeslice plaquette and rectangle operators): connected
+    correlators with delete-one jackknife (glueball_scalar_production.rs:383-470),
+    cosh effective masses of the plaquette channel (glueball_effective_mass.rs:161-232),
+    the correlated one-state fit over predeclared windows
+    (glueball_scalar_fit.rs:595-650), and the 2 x 2 variational correlator with
+    its principal generalized projection (glueball_scalar_variational.rs:628-925).
+    Steps that the data do not admit report the reference's error status."""
+    plaquette = np.asarray(plaquette, dtype=np.float64)
+    rectangle = np.asarray(rectangle, dtype=np.float64)
+    T = plaquette.shape[1]
+    ev = stats.correlator_evidence(plaquette)
+    lags = [t for t in range(1, T) if 2 * t < T and t < T - 1]
+    if candidate_windows is None:
+        candidate_windows = [(a, a + n - 1) for n in (3, 4) for a in (1, 2) if 2 * (a + n - 1) < T]
+    out = {"plaquette": ev, "rectangle": stats.correlator_evidence(rectangle)}
+    out["effective_mass"] = _attempt(stats.cosh_effective_mass_with_jackknife, ev.full, ev.jackknife, lags)
+    out["fit"] = _attempt(stats.correlated_scalar_fit, ev.full, ev.jackknife, candidate_windows)
+    var = _attempt(stats.scalar_variational_connected_correlator, plaquette, rectangle)
+    out["variational"] = var
+    out["projection"] = _attempt(stats.principal_generalized_projection, var.matrices) \
+        if isinstance(var, stats.VariationalCorrelator) else var
+    return out
 
 
 def production(chains_per_geometry=64, geometries=PRODUCTION_GEOMETRIES, stout_levels=4, rho=CANONICAL_RHO,
@@ -157,14 +193,16 @@ def production(chains_per_geometry=64, geometries=PRODUCTION_GEOMETRIES, stout_l
                            rho=rho, use_graph=use_graph)
     finally:
         ens.close()
-    corr = {}
+    corr, analysis = {}, {}
     for gi, g in enumerate(geometries):
         rows = range(gi * chains_per_geometry, (gi + 1) * chains_per_geometry)
         for lev in range(stout_levels + 1):
+            samples = [np.array([r.ops[0][c][lev, :, kind] for c in rows]) for kind in range(2)]
             for kind in range(2):
-                samples = np.array([r.ops[0][c][lev, :, kind] for c in rows])
-                corr[(g, lev, kind)] = stats.correlator_evidence(samples)
+                corr[(g, lev, kind)] = stats.correlator_evidence(samples[kind])
+            if chains_per_geometry >= 3 and g[3] >= 4:
+                analysis[(g, lev)] = correlator_analysis(samples[0], samples[1])
     links_per_sweep = np.array([4 * np.prod(e) for e in extents], dtype=np.float64)
     acceptance = r.accepts.astype(np.float64) / (links_per_sweep * hits * sweeps)
     return ProductionReport(tuple(geometries), chains_per_geometry, r.plaquette[0], acceptance, corr, r.device_ms,
-                            sweeps)
+                            sweeps, analysis)

@@ -262,6 +262,21 @@ def spatial_stout_smear(lattice, rho, levels):
     return out
 
 
+def blocked_extents(lattice):
+    """glueball_blocking.rs:85-94: coarse extents (N/2, N/2, N/2, N3)."""
+    c = (C.c_int64 * 4)()
+    check(lib().ym_su3_blocked_extents(lattice.handle, c), "ym_su3_blocked_extents")
+    return tuple(int(v) for v in c)
+
+
+def spatial_stride2_block(lattice):
+    """glueball_blocking.rs:105-136: returns a new coarse lattice with
+    V_mu(y) = U_mu(x) U_mu(x+mu) (spatial mu), V_3(y) = U_3(x), x = (2y0, 2y1, 2y2, y3)."""
+    out = Su3Lattice4(blocked_extents(lattice), lattice.precision)
+    check(lib().ym_su3_spatial_block(out.handle, lattice.handle), "ym_su3_spatial_block")
+    return out
+
+
 def scalar_a1pp_operator_basis(lattice, time_slice):
     """glueball_operators.rs:143-160 for one slice: (plaquette_1x1, rectangle_2x1)."""
     pl, rc = lattice.timeslice_a1pp()

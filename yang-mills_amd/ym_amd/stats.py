@@ -7,14 +7,18 @@ Mirrors the reference's analysis functions: ``evaluate``
 ``connected_from_samples`` (glueball_scalar_production.rs:383-443) and the
 delete-one jackknife of ``correlator_evidence``
 (glueball_scalar_production.rs:445-470) with the error convention of
-glueball_effective_mass.rs:161-215.
+glueball_effective_mass.rs:161-215; the correlator analysis that consumes it:
+``cosh_effective_mass_with_jackknife`` (glueball_effective_mass.rs:161-232),
+``correlated_scalar_fit`` (glueball_scalar_fit.rs:595-650),
+``scalar_variational_connected_correlator`` and
+``principal_generalized_projection`` (glueball_scalar_variational.rs:628-925).
 """
 import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import EfficiencyReport, PoolSummary, StationarityReport, check, lib
+from ._lib import EffectiveMassPoint, EfficiencyReport, PoolSummary, ScalarFitResult, StationarityReport, check, lib
 
 
 def _arr(x):
@@ -105,3 +109,103 @@ def correlator_evidence(samples):
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
     check(lib().ym_jackknife_correlator(sp, nconf, T, dp(full), dp(rep), dp(jm), dp(je)), "ym_jackknife_correlator")
     return CorrelatorEvidence(full, rep, jm, je)
+
+
+# ------------------------------------------------------------ F1 analysis
+def _rows(replicas):
+    """Replica correlators as row pointers + lengths (the reference's &[Vec<f64>])."""
+    rows = [np.ascontiguousarray(np.asarray(r, dtype=np.float64)) for r in replicas]
+    ptrs = (C.POINTER(C.c_double) * max(1, len(rows)))(*[r.ctypes.data_as(C.POINTER(C.c_double)) for r in rows])
+    lens = (C.c_int64 * max(1, len(rows)))(*[r.size for r in rows])
+    return rows, ptrs, lens
+
+
+@dataclass
+class EffectiveMassSeries:
+    time_extent: int
+    jackknife_replicates: int
+    tau: np.ndarray
+    full_estimate: np.ndarray
+    jackknife_mean: np.ndarray
+    standard_error: np.ndarray
+
+
+def cosh_effective_mass_with_jackknife(full, jackknife, lags):
+    """glueball_effective_mass.rs:161-232."""
+    f, fp = _arr(full)
+    rows, ptrs, lens = _rows(jackknife)
+    lg = np.ascontiguousarray(np.asarray(lags, dtype=np.int64))
+    out = (EffectiveMassPoint * max(1, lg.size))()
+    check(lib().ym_cosh_effective_mass(fp, f.size, ptrs, lens, len(rows), lg.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       lg.size, out), "ym_cosh_effective_mass")
+    pts = out[:lg.size]
+    return EffectiveMassSeries(f.size, len(rows), np.array([p.tau for p in pts], dtype=np.int64),
+                               np.array([p.full_estimate for p in pts]), np.array([p.jackknife_mean for p in pts]),
+                               np.array([p.standard_error for p in pts]))
+
+
+def analytic_periodic_single_state(time_extent, mass, amplitude):
+    """glueball_effective_mass.rs:240-268."""
+    out = np.zeros(max(0, int(time_extent)))
+    check(lib().ym_analytic_periodic_single_state(int(time_extent), float(mass), float(amplitude),
+                                                  out.ctypes.data_as(C.POINTER(C.c_double))),
+          "ym_analytic_periodic_single_state")
+    return out
+
+
+def correlated_scalar_fit(full, jackknife, candidate_windows):
+    """glueball_scalar_fit.rs:595-650.  candidate_windows: [(start, end), ...]."""
+    f, fp = _arr(full)
+    rows, ptrs, lens = _rows(jackknife)
+    w = np.ascontiguousarray(np.asarray(candidate_windows, dtype=np.int64).reshape(-1))
+    r = ScalarFitResult()
+    check(lib().ym_correlated_scalar_fit(fp, f.size, ptrs, lens, len(rows), w.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         w.size // 2, C.byref(r)), "ym_correlated_scalar_fit")
+    return r
+
+
+@dataclass
+class VariationalCorrelator:
+    configuration_count: int
+    time_extent: int
+    vacuum_means: np.ndarray
+    matrices: np.ndarray  # (T, 2, 2)
+
+
+def scalar_variational_connected_correlator(plaquette, rectangle):
+    """glueball_scalar_variational.rs:628-744 on A1++ samples: plaquette and
+    rectangle are (nconf, T) arrays of per-timeslice operator values."""
+    p = np.asarray(plaquette, dtype=np.float64)
+    q = np.asarray(rectangle, dtype=np.float64)
+    if p.shape != q.shape or p.ndim != 2:
+        raise ValueError("plaquette and rectangle samples must be two (nconf, T) arrays")
+    s = np.ascontiguousarray(np.stack([p, q], axis=-1))
+    nconf, T = p.shape
+    vm, mats = np.zeros(2), np.zeros((T, 2, 2))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    check(lib().ym_variational_connected_correlator(dp(s), nconf, T, dp(vm), dp(mats)),
+          "ym_variational_connected_correlator")
+    return VariationalCorrelator(nconf, T, vm, mats)
+
+
+@dataclass
+class PrincipalProjection:
+    reference_time: int
+    diagonalization_time: int
+    principal_eigenvalue: float
+    coefficients: np.ndarray
+    projected_correlator: np.ndarray
+
+
+def principal_generalized_projection(matrices, reference_time=1, diagonalization_time=2):
+    """glueball_scalar_variational.rs:746-925.  matrices: (T, 2, 2)."""
+    m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64))
+    if m.ndim != 3 or m.shape[1:] != (2, 2):
+        raise ValueError("matrices must be (T, 2, 2)")
+    T = m.shape[0]
+    lam, coef, proj = C.c_double(), np.zeros(2), np.zeros(T)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    check(lib().ym_principal_generalized_projection(dp(m), T, int(reference_time), int(diagonalization_time),
+                                                    C.byref(lam), dp(coef), dp(proj)),
+          "ym_principal_generalized_projection")
+    return PrincipalProjection(int(reference_time), int(diagonalization_time), lam.value, coef, proj)
