@@ -44,7 +44,11 @@ enum {
     YM_EMPTY_PATH = 9,                /* PathHolonomyError::EmptyPath */
     YM_PATH_TOO_LONG = 10,            /* PathHolonomyError::PathTooLong */
     YM_ZERO_RECTANGLE_EXTENT = 12,
+    YM_SU2_NON_FINITE = 13,           /* ReferenceError::NonFiniteSu2Component */
+    YM_SU2_NON_UNIT = 15,             /* ReferenceError::NonUnitSu2 */
     YM_COORDINATE_OUT_OF_RANGE = 17,  /* ReferenceError::CoordinateOutOfRange */
+    YM_INVALID_CONFIGURATION_ENCODING = 18, /* ReferenceError::InvalidConfigurationEncoding */
+    YM_LATTICE_SIZE_OVERFLOW = 19,    /* ReferenceError::LatticeSizeOverflow */
     YM_BETA_OUT_OF_RANGE = 30,        /* Su3EnsembleError / EnsembleError::BetaOutsideReferenceRange */
     YM_ANGLE_OUT_OF_RANGE = 31,       /* Su3EnsembleError::ProposalAngleOutsideReferenceRange */
     YM_MEASUREMENT_COUNT = 32,
@@ -428,6 +432,41 @@ int32_t ym_connected_correlator(const double *samples, int64_t nconf, int64_t T,
 int32_t ym_jackknife_correlator(const double *samples, int64_t nconf, int64_t T, double *full, double *replicas,
                                 double *jackknife_mean, double *jackknife_error);
 
+
+/* ------------------------------------------------ configuration formats
+ * Host-only (no device): they work on the AoS FP64 arrays that
+ * ym_lattice_download returns and ym_lattice_upload takes. */
+/* lib.rs:587-614 to_canonical_config: the canonical YM-SU2-CONFIG-V1 text of
+ * an SU(2) configuration (aos: 4 doubles per link).  Fails with
+ * YM_SU2_NON_UNIT when a link's |norm^2 - 1| exceeds 2e-12 and with
+ * YM_LATTICE_TOO_LARGE above 2^20 links (the format's bound).  Writes
+ * *length bytes (no terminating NUL) when buf holds them; buf == NULL only
+ * reports *length. */
+int32_t ym_su2_config_encode(const int64_t extents[4], const double *aos, char *buf, int64_t capacity,
+                             int64_t *length);
+/* lib.rs:616-694 from_canonical_config: extents from the header lines, then
+ * (aos != NULL, capacity >= 4 * links) the links, each kept bit for bit;
+ * errors in the reference's order (YM_INVALID_CONFIGURATION_ENCODING,
+ * YM_LATTICE_TOO_LARGE, YM_INVALID_EXTENT, YM_SU2_NON_FINITE,
+ * YM_SU2_NON_UNIT); text that is not byte-identical to the canonical
+ * encoding of its links is rejected. */
+int32_t ym_su2_config_decode(const char *text, int64_t length, int64_t extents[4], double *aos, int64_t capacity);
+/* Binary checkpoint of a chain (no reference counterpart; it serves the
+ * chain drivers su3_ensemble.rs:408-480 / su2_ensemble.rs:488-560 and the
+ * production generator glueball_scalar_production.rs:257-297 for resume):
+ * group (2, 3), storage precision (32, 64), extents, the chain key, the next
+ * sweep index and the links in the storage precision, AoS order, with an
+ * FNV-1a 64 checksum.  The sweeps draw from a counter-based stream keyed by
+ * (key, sweep, site, direction, hit), so links + key + next sweep restore a
+ * chain exactly.  ym_checkpoint_bytes gives the encoded size. */
+int32_t ym_checkpoint_bytes(int32_t group, int32_t precision, const int64_t extents[4], int64_t *bytes);
+int32_t ym_checkpoint_encode(int32_t group, int32_t precision, const int64_t extents[4], const double *aos,
+                             uint64_t key, uint64_t next_sweep, void *buf, int64_t capacity, int64_t *length);
+/* Validates magic, sizes and checksum (YM_INVALID_CONFIGURATION_ENCODING). */
+int32_t ym_checkpoint_header(const void *buf, int64_t length, int32_t *group, int32_t *precision, int64_t extents[4],
+                             uint64_t *key, uint64_t *next_sweep);
+/* Links back into AoS FP64 (count = links x 4 or 18 doubles), exactly. */
+int32_t ym_checkpoint_decode(const void *buf, int64_t length, double *aos, int64_t count);
 
 /* ------------------------------------------- host FP64 correlator analysis
  * Replica correlators are passed as nrep row pointers with their lengths

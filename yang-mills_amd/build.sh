@@ -6,7 +6,7 @@
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
 HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
-SRC="$HERE/csrc/ym_su3.hip $HERE/csrc/ym_su2.hip $HERE/csrc/ym_obs.hip $HERE/csrc/ym_capi.hip $HERE/csrc/ym_ensemble.hip $HERE/csrc/ym_learned.hip $HERE/csrc/ym_host_stats.cpp $HERE/csrc/ym_host_fit.cpp"
+SRC="$HERE/csrc/ym_su3.hip $HERE/csrc/ym_su2.hip $HERE/csrc/ym_obs.hip $HERE/csrc/ym_capi.hip $HERE/csrc/ym_ensemble.hip $HERE/csrc/ym_learned.hip $HERE/csrc/ym_host_stats.cpp $HERE/csrc/ym_host_fit.cpp $HERE/csrc/ym_codec.cpp"
 $HIPCC --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
   -ffp-contract=off -fno-gpu-flush-denormals-to-zero \
   -Xclang -target-feature -Xclang -packed-fp32-ops \

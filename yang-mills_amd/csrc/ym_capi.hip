@@ -128,6 +128,10 @@ const char *ym_status_string(int32_t s) {
     case YM_ZERO_RECTANGLE_EXTENT: return "rectangular Wilson-loop extents must be positive";
     case YM_EMPTY_PATH: return "path is empty";
     case YM_PATH_TOO_LONG: return "path exceeds the bounded length";
+    case YM_SU2_NON_FINITE: return "SU(2) components must be finite";
+    case YM_SU2_NON_UNIT: return "serialized SU(2) components must satisfy the unit invariant";
+    case YM_INVALID_CONFIGURATION_ENCODING: return "invalid configuration encoding";
+    case YM_LATTICE_SIZE_OVERFLOW: return "lattice size arithmetic overflowed";
     case YM_COORDINATE_OUT_OF_RANGE: return "lattice coordinate is outside its periodic fundamental domain";
     case YM_BETA_OUT_OF_RANGE: return "beta is outside the bounded reference range";
     case YM_ANGLE_OUT_OF_RANGE: return "Metropolis proposal angle is outside the bounded reference range";

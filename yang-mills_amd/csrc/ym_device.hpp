@@ -340,6 +340,42 @@ __device__ __forceinline__ void store3(typename Vec2<Real>::type *__restrict__ L
     }
 }
 
+// FP32 storage pairs the entries inside the same 4.5 KB tile: entries
+// (0, 1), (2, 3), (4, 5), (6, 7) as 4 runs of 64 float4 (16 bytes per slot)
+// and entry 8 as one run of 64 float2.  A link is then 4 dwordx4 + 1 dwordx2
+// loads per lane instead of 9 dwordx2, each wavefront load still one
+// contiguous block (1 KB or 512 B), so a wave keeps about twice the bytes in
+// flight for the same number of outstanding memory instructions.  FP64
+// storage keeps the 9 runs of double2 (already 16 bytes per lane).
+template <>
+__device__ __forceinline__ M3<float> load3<float>(const float2 *__restrict__ L, int vh, int mu, int par, int s) {
+    const size_t tile = (size_t)(mu * 2 + par) * ntile_of(vh) + (size_t)(s >> 6);
+    const int lane = s & 63;
+    const float4 *q = reinterpret_cast<const float4 *>(L) + tile * 288 + lane;
+    M3<float> m;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float4 t = q[k * 64];
+        m.v[4 * k] = t.x;
+        m.v[4 * k + 1] = t.y;
+        m.v[4 * k + 2] = t.z;
+        m.v[4 * k + 3] = t.w;
+    }
+    const float2 t = L[tile * 576 + 512 + lane];
+    m.v[16] = t.x;
+    m.v[17] = t.y;
+    return m;
+}
+template <>
+__device__ __forceinline__ void store3<float>(float2 *__restrict__ L, int vh, int mu, int par, int s, const M3<float> &m) {
+    const size_t tile = (size_t)(mu * 2 + par) * ntile_of(vh) + (size_t)(s >> 6);
+    const int lane = s & 63;
+    float4 *q = reinterpret_cast<float4 *>(L) + tile * 288 + lane;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k * 64] = make_float4(m.v[4 * k], m.v[4 * k + 1], m.v[4 * k + 2], m.v[4 * k + 3]);
+    L[tile * 576 + 512 + lane] = make_float2(m.v[16], m.v[17]);
+}
+
 // T = X Y^dagger
 template <typename Real>
 __device__ __forceinline__ void mul_nd(const M3<Real> &X, const M3<Real> &Y, M3<Real> &T) {

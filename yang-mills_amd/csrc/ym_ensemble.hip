@@ -190,12 +190,15 @@ __global__ void k_ens_advance(uint64_t *ctr, uint64_t sweeps, uint64_t slots) {
 }
 
 // Identity links over the concatenated buffer: every chain occupies whole
-// 576-pair tiles, so element e of pair i is (i / 64) % 9.
+// 576-pair tiles.  Entry e of pair j of a tile (load3/store3 layout): FP64
+// stores 9 runs of 64 pairs, e = j / 64; FP32 stores entries (2q, 2q+1) as
+// float4 run q (pairs 128 q .. 128 q + 127, alternating), then entry 8.
 template <typename Real>
 __global__ void k_ens_identity(typename Vec2<Real>::type *__restrict__ links, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const int e = (int)((i >> 6) % 9);
+    const int j = (int)(i % 576);
+    const int e = sizeof(Real) == 8 ? j >> 6 : (j >= 512 ? 8 : 2 * (j >> 7) + (j & 1));
     typename Vec2<Real>::type v;
     v.x = (e == 0 || e == 4 || e == 8) ? (Real)1 : (Real)0;
     v.y = (Real)0;

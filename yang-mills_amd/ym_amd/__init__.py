@@ -18,6 +18,7 @@ from . import ensemble  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
 from . import learned  # noqa: F401
 from . import precision  # noqa: F401
+from . import codec  # noqa: F401
 from .lattice import (  # noqa: F401
     Lattice4,
     Su3Lattice4,

@@ -123,6 +123,12 @@ SIGNATURES = {
     "ym_periodic_connected": (i32, [P(dbl), i64, i64, P(dbl), P(dbl), P(dbl)]),
     "ym_connected_correlator": (i32, [P(dbl), i64, i64, i64, P(dbl)]),
     "ym_jackknife_correlator": (i32, [P(dbl), i64, i64, P(dbl), P(dbl), P(dbl), P(dbl)]),
+    "ym_su2_config_encode": (i32, [P(i64), P(dbl), C.c_char_p, i64, P(i64)]),
+    "ym_su2_config_decode": (i32, [C.c_char_p, i64, P(i64), P(dbl), i64]),
+    "ym_checkpoint_bytes": (i32, [i32, i32, P(i64), P(i64)]),
+    "ym_checkpoint_encode": (i32, [i32, i32, P(i64), P(dbl), u64, u64, C.c_void_p, i64, P(i64)]),
+    "ym_checkpoint_header": (i32, [C.c_void_p, i64, P(i32), P(i32), P(i64), P(u64), P(u64)]),
+    "ym_checkpoint_decode": (i32, [C.c_void_p, i64, P(dbl), i64]),
     "ym_cosh_effective_mass": (i32, [P(dbl), i64, P(P(dbl)), P(i64), i64, P(i64), i64, P(EffectiveMassPoint)]),
     "ym_analytic_periodic_single_state": (i32, [i64, dbl, dbl, P(dbl)]),
     "ym_correlated_scalar_fit": (i32, [P(dbl), i64, P(P(dbl)), P(i64), i64, P(i64), i64, P(ScalarFitResult)]),

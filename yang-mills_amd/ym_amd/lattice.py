@@ -138,6 +138,18 @@ class _Lattice:
     def reunitarize(self):
         check(lib().ym_reunitarize(self._h), "ym_reunitarize")
 
+    # configuration formats -------------------------------------------------
+    def save_checkpoint(self, path, key, next_sweep):
+        """Binary checkpoint: links in storage precision + chain key + next sweep."""
+        from . import codec
+        return codec.save_checkpoint(path, self, key, next_sweep)
+
+    @staticmethod
+    def load_checkpoint(path):
+        """(lattice, key, next_sweep) from a checkpoint file."""
+        from . import codec
+        return codec.load_checkpoint(path)
+
     # timing ---------------------------------------------------------------
     def timer_begin(self):
         check(lib().ym_timer_begin(self._h), "ym_timer_begin")
@@ -166,6 +178,17 @@ class Lattice4(_Lattice):
 
     def overrelax_sweeps(self, n_sweeps=1):
         check(lib().ym_su2_overrelax_sweeps(self._h, int(n_sweeps)), "ym_su2_overrelax_sweeps")
+
+    def to_canonical_config(self):
+        """lib.rs:587-614: the YM-SU2-CONFIG-V1 text of this configuration."""
+        from . import codec
+        return codec.to_canonical_config(self)
+
+    @classmethod
+    def from_canonical_config(cls, text, precision=64):
+        """lib.rs:616-694: a new lattice from canonical text."""
+        from . import codec
+        return codec.from_canonical_config(text, precision)
 
 
 class Su3Lattice4(_Lattice):
