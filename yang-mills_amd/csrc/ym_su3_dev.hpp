@@ -66,7 +66,10 @@ __device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t k
 
 // Staple A(x) of the link U_MU(x), x of parity p at slot s: the six staples
 // in the storage precision with the fixed FMA chain (mul_nd/acc_nd forward,
-// mul_dd/acc_nn backward, nu ascending).
+// mul_dd/acc_nn backward, nu ascending).  The three links of term k + 1 are
+// loaded before term k is multiplied, so a wave keeps two terms' loads in
+// flight instead of waiting on each term's links in turn; the arithmetic and
+// its order are unchanged.
 template <typename Real, int MU>
 __device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *__restrict__ L, const Geom &g, int p,
                                               int s, const int x[4], M3<Real> &A) {
@@ -76,28 +79,41 @@ __device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *_
     int xpm[4];
     shifted(g, x, MU, +1, xpm);
     const int s_pm = slot_of(g, xpm);
+    // term 2k: forward staple of direction nu_k, X = U_nu(x+mu), Y = U_mu(x+nu), Z = U_nu(x);
+    // term 2k+1: backward, X = U_nu(x+mu-nu), Y = U_mu(x-nu), Z = U_nu(x-nu)
+    int dir[6], px[6], sx[6], sy[6], pz[6], sz[6];
 #pragma unroll
-    for (int nu = 0; nu < 4; ++nu) {
+    for (int k = 0, nu = 0; nu < 4; ++nu) {
         if (nu == MU) continue;
         int xpn[4], xmn[4], xpmmn[4];
         shifted(g, x, nu, +1, xpn);
         shifted(g, x, nu, -1, xmn);
         shifted(g, xpm, nu, -1, xpmmn);
         const int s_pn = slot_of(g, xpn), s_mn = slot_of(g, xmn), s_pmmn = slot_of(g, xpmmn);
-        M3<Real> T;
-        {
-            const M3<Real> X = load3<Real>(L, g.vh, nu, q, s_pm);
-            const M3<Real> Y = load3<Real>(L, g.vh, MU, q, s_pn);
-            mul_nd(X, Y, T);
-            const M3<Real> Z = load3<Real>(L, g.vh, nu, p, s);
-            acc_nd(A, T, Z);
+        dir[k] = nu; px[k] = q; sx[k] = s_pm; sy[k] = s_pn; pz[k] = p; sz[k] = s;
+        ++k;
+        dir[k] = nu; px[k] = p; sx[k] = s_pmmn; sy[k] = s_mn; pz[k] = q; sz[k] = s_mn;
+        ++k;
+    }
+    M3<Real> X[2], Y[2], Z[2];
+    X[0] = load3<Real>(L, g.vh, dir[0], px[0], sx[0]);
+    Y[0] = load3<Real>(L, g.vh, MU, q, sy[0]);
+    Z[0] = load3<Real>(L, g.vh, dir[0], pz[0], sz[0]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int b = k & 1;
+        if (k + 1 < 6) {
+            X[b ^ 1] = load3<Real>(L, g.vh, dir[k + 1], px[k + 1], sx[k + 1]);
+            Y[b ^ 1] = load3<Real>(L, g.vh, MU, q, sy[k + 1]);
+            Z[b ^ 1] = load3<Real>(L, g.vh, dir[k + 1], pz[k + 1], sz[k + 1]);
         }
-        {
-            const M3<Real> X = load3<Real>(L, g.vh, nu, p, s_pmmn);
-            const M3<Real> Y = load3<Real>(L, g.vh, MU, q, s_mn);
-            mul_dd(X, Y, T);
-            const M3<Real> Z = load3<Real>(L, g.vh, nu, q, s_mn);
-            acc_nn(A, T, Z);
+        M3<Real> T;
+        if (b == 0) {
+            mul_nd(X[b], Y[b], T);
+            acc_nd(A, T, Z[b]);
+        } else {
+            mul_dd(X[b], Y[b], T);
+            acc_nn(A, T, Z[b]);
         }
     }
 }
