@@ -154,6 +154,26 @@ def test_su2_heatbath_overrelax_match_replay_bitwise(ym, prec):
         assert np.array_equal(lat.download(), dev_to_aos(n, buf, prec, 4))
 
 
+@pytest.mark.parametrize("prec,beta", [(32, 2.3), (64, 2.5), (32, 0.5)])
+def test_su2_heatbath_hot_start_matches_replay_bitwise(ym, prec, beta):
+    """Full waves with varied staples (hot start, 8^3 x 6): every link's
+    accepted attempt, hence the proposal count and the new link, equals the
+    sequential rejection loop of the replay, whichever lanes evaluated it."""
+    n = (8, 8, 8, 6)
+    key = ym.derive_chain_seed(0x2b00201211160101, 9)
+    lat = ym.Lattice4(n, prec)
+    lat.hot_start(key)
+    buf = dev_buffer(n, 4, prec)
+    assert O.lib().ymd_su2_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, key) == 0
+    for sweep in range(2):
+        gp, gu = lat.heatbath_sweeps(beta, key, sweep, 1, 10_000)
+        cp, cu = C.c_uint64(), C.c_uint64()
+        assert O.lib().ymd_su2_heatbath_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, beta, key, sweep,
+                                              10_000, C.byref(cp), C.byref(cu)) == 0
+        assert (gp, gu) == (cp.value, cu.value) and gp > gu
+        assert np.array_equal(lat.download(), dev_to_aos(n, buf, prec, 4))
+
+
 def test_su2_heatbath_attempt_limit_reported(ym):
     # A cap of one attempt at large alpha rejects most first attempts: the
     # sweep must return HeatbathAttemptLimitExceeded (status 42), not hang.

@@ -248,6 +248,13 @@ __device__ __forceinline__ void shifted(const Geom &g, const int x[4], int d, in
     y[d] = v;
 }
 
+// Orders this wave's LDS writes before its later LDS reads (no block barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Logical block index with consecutive tiles kept on one XCD: the hardware
 // deals workgroups round-robin over the 8 XCDs (block b -> XCD b % 8), so XCD
 // k receives a contiguous range of logical tiles and its L2 sees the

@@ -249,13 +249,6 @@ __global__ __launch_bounds__(64) void k_mlp_pack(const float *__restrict__ P, bf
     packed[f * 64 + l] = v;
 }
 
-// Orders this wave's LDS writes before its later LDS reads.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ bf16x8 lds_chunk(const __bf16 *__restrict__ t, int row, int chunk) {
     return *reinterpret_cast<const bf16x8 *>(t + row * 64 + ((chunk ^ (row & 7)) << 3));
 }

@@ -66,19 +66,34 @@ __device__ __forceinline__ double plaquette_site(const typename G::Store *__rest
     double acc = 0.0;
     int x[4];
     if (!site_of(g, p, s, x)) return acc;
+    // The four links at x are loaded once; the two neighbour links of plane
+    // k + 1, U_nu(x + mu) and U_mu(x + nu), are in flight while plane k is
+    // multiplied.  Planes in the order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3).
+    constexpr int PM[6] = {0, 0, 0, 1, 1, 2}, PN[6] = {1, 2, 3, 2, 3, 3};
+    typename G::Mat U[4];
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-        int xm[4];
-        shifted(g, x, mu, +1, xm);
-        const typename G::Mat Umu = G::load(L, g, x, mu);
+    for (int mu = 0; mu < 4; ++mu) U[mu] = G::load(L, g, x, mu);
+    typename G::Mat A[2], B[2];
+    {
+        int xm[4], xn[4];
+        shifted(g, x, PM[0], +1, xm);
+        shifted(g, x, PN[0], +1, xn);
+        A[0] = G::load(L, g, xm, PN[0]);
+        B[0] = G::load(L, g, xn, PM[0]);
+    }
 #pragma unroll
-        for (int nu = mu + 1; nu < 4; ++nu) {
-            int xn[4];
-            shifted(g, x, nu, +1, xn);
-            const typename G::Mat a = G::mul(Umu, G::load(L, g, xm, nu));
-            const typename G::Mat b = G::mul(G::load(L, g, x, nu), G::load(L, g, xn, mu));
-            acc += G::retr_nd(a, b);
+    for (int k = 0; k < 6; ++k) {
+        const int b = k & 1;
+        if (k + 1 < 6) {
+            int xm[4], xn[4];
+            shifted(g, x, PM[k + 1], +1, xm);
+            shifted(g, x, PN[k + 1], +1, xn);
+            A[b ^ 1] = G::load(L, g, xm, PN[k + 1]);
+            B[b ^ 1] = G::load(L, g, xn, PM[k + 1]);
         }
+        const typename G::Mat a = G::mul(U[PM[k]], A[b]);
+        const typename G::Mat c = G::mul(U[PN[k]], B[b]);
+        acc += G::retr_nd(a, c);
     }
     return acc;
 }

@@ -54,46 +54,100 @@ __device__ __forceinline__ Q4<Real> su2_staple(const typename Vec4<Real>::type *
     return A;
 }
 
+// One heat-bath attempt: r0 from the truncated exponential by inverse CDF,
+// accepted iff u2 <= sqrt(1 - r0^2) (su2_ensemble.rs:229-250).
+__device__ __forceinline__ bool hb_attempt(uint64_t key, uint32_t site, int mu, int att, uint64_t sweep, double alpha,
+                                           double tm, double &r0, double &radial) {
+    uint32_t o[4];
+    draw(key, site, mu, KIND_HB_ATTEMPT, (uint32_t)att, sweep, o);
+    const double u1 = unit53(o[0], o[1]), u2 = unit53(o[2], o[3]);
+    if (alpha <= 1.0e-300) {
+        r0 = 2.0 * u1 - 1.0;
+    } else {
+        const double y = -log1p_dev(-u1 * tm) / alpha;
+        r0 = 1.0 - y;
+        r0 = r0 < -1.0 ? -1.0 : (r0 > 1.0 ? 1.0 : r0);
+    }
+    radial = __builtin_sqrt(fmax(__builtin_fma(-r0, r0, 1.0), 0.0));
+    return u2 <= radial;
+}
+
+// The rejection loop is run by the whole wavefront together.  Each round the
+// P links still without an accepted attempt get k = 64 / P lanes each; lane o
+// of a link evaluates its attempt next + o.  The link takes the lowest
+// accepted attempt index of the round (or moves next on by k), so every link
+// ends with exactly the attempt, r0 and proposal count of its own sequential
+// loop, while a wave needs about (total attempts) / 64 rounds instead of the
+// largest attempt count among its 64 links.
 template <typename Real, int MU>
 __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type *__restrict__ L, Geom g, int p,
                                                       double beta, uint64_t key, uint64_t sweep, int max_attempts,
                                                       unsigned long long *__restrict__ counters,
                                                       int *__restrict__ status) {
     __shared__ unsigned long long red[4];
+    __shared__ int owner[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // parity-1 sub-passes walk the tiles backwards (the field of a 48^4
     // lattice is larger than the memory-side cache; see k_su3_metropolis_ws)
     const int tile = (p & 1) ? xcd_tile_rev(blockIdx.x, gridDim.x) : xcd_tile(blockIdx.x, gridDim.x);
     const int s = tile * 256 + threadIdx.x;
     unsigned long long nprop = 0, nupd = 0;
-    int x[4];
-    if (s < g.vh && site_of(g, p, s, x)) {
+    int x[4] = {0, 0, 0, 0};
+    const bool has = s < g.vh && site_of(g, p, s, x);
+    Q4<double> a;
+    double norm = 0.0, alpha = 0.0, tm = 0.0;
+    uint32_t site = 0;
+    if (has) {
         const Q4<Real> Af = su2_staple<Real, MU>(L, g, x, p, s);
-        Q4<double> a;
 #pragma unroll
         for (int c = 0; c < 4; ++c) a.c[c] = (double)Af.c[c];
-        const double norm = __builtin_sqrt(qnorm2(a));
-        const double alpha = beta * norm;
-        const uint32_t site = (uint32_t)lex_of(g, x);
-        double r0 = 0.0, radial = 0.0;
-        bool ok = false;
-        int att = 0;
-        const double tm = alpha <= 1.0e-300 ? 0.0 : -expm1_nonpos(-2.0 * alpha);
-        while (att < max_attempts) {
-            uint32_t o[4];
-            draw(key, site, MU, KIND_HB_ATTEMPT, (uint32_t)att, sweep, o);
-            att += 1;
-            const double u1 = unit53(o[0], o[1]), u2 = unit53(o[2], o[3]);
-            if (alpha <= 1.0e-300) {
-                r0 = 2.0 * u1 - 1.0;
+        norm = __builtin_sqrt(qnorm2(a));
+        alpha = beta * norm;
+        site = (uint32_t)lex_of(g, x);
+        tm = alpha <= 1.0e-300 ? 0.0 : -expm1_nonpos(-2.0 * alpha);
+    }
+    bool pending = has, ok = false;
+    int next = 0;
+    double r0 = 0.0, radial = 0.0;
+    while (true) {
+        const unsigned long long pm = __ballot(pending);
+        if (pm == 0ull) break;
+        const int P = __popcll(pm);
+        const int k = 64 / P;
+        const int rank = __popcll(pm & ((1ull << lane) - 1ull));
+        if (pending) owner[wave][rank] = lane;
+        wave_lds_sync();
+        const int r = lane / k, o = lane - r * k;
+        const int src = r < P ? owner[wave][r] : lane;
+        const double alpha_o = __shfl(alpha, src), tm_o = __shfl(tm, src);
+        const uint32_t site_o = (uint32_t)__shfl((int)site, src);
+        const int att = __shfl(next, src) + o;
+        double tr0 = 0.0, trad = 0.0;
+        const bool hit = r < P && att < max_attempts && hb_attempt(key, site_o, MU, att, sweep, alpha_o, tm_o, tr0, trad);
+        const unsigned long long am = __ballot(hit);
+        // lanes of this link: [rank * k, rank * k + k)
+        const unsigned long long span = (k == 64) ? ~0ull : (((1ull << k) - 1ull) << (rank * k));
+        const unsigned long long mine = pending ? (am & span) : 0ull;
+        const int first = mine ? __ffsll((long long)mine) - 1 : lane;
+        const double r0_f = __shfl(tr0, first), rad_f = __shfl(trad, first);
+        if (pending) {
+            if (mine) {
+                ok = true;
+                pending = false;
+                r0 = r0_f;
+                radial = rad_f;
+                nprop = (unsigned long long)(next + (first - rank * k) + 1);
             } else {
-                const double y = -log1p_dev(-u1 * tm) / alpha;
-                r0 = 1.0 - y;
-                r0 = r0 < -1.0 ? -1.0 : (r0 > 1.0 ? 1.0 : r0);
+                next += k;
+                if (next >= max_attempts) {
+                    pending = false;
+                    nprop = (unsigned long long)max_attempts;
+                }
             }
-            radial = __builtin_sqrt(fmax(__builtin_fma(-r0, r0, 1.0), 0.0));
-            if (u2 <= radial) { ok = true; break; }
         }
-        nprop = (unsigned long long)att;
+        wave_lds_sync();
+    }
+    if (has) {
         if (!ok) {
             atomicOr(status, 1);
         } else {
