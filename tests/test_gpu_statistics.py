@@ -43,6 +43,68 @@ def test_level_c_chains_agree_with_reference_v2(ym, prec):
     assert abs(np.mean(acc) - 0.8675) < 0.005, acc
 
 
+def _jackknife_over_chains(means):
+    m = np.asarray(means, dtype=np.float64)
+    n = m.size
+    reps = (m.sum() - m) / (n - 1)
+    return float(m.mean()), float(math.sqrt((n - 1) / n * np.sum((reps - reps.mean()) ** 2)))
+
+
+def test_level_c_against_reference_chains_in_the_same_job(ym):
+    """Level (c) as stated: fresh reference-order CPU chains (the oracle's
+    restatement of run_chain, su3_ensemble.rs:408-480: lexicographic order,
+    SplitMix64 stream, checked FP64 products) run in this job beside FP32 GPU
+    checkerboard chains, same schedule (4^4, beta = 6, cold start, 128
+    thermalisation sweeps + 32 measurements 2 sweeps apart; the CPU chains
+    cost about 0.2 s per sweep, so 8 of them run in parallel threads),
+    independent seeds.  Pooled plaquettes agree within 2 sigma of the
+    jackknife over chains; GPU acceptance matches the CPU's."""
+    import ctypes as C
+    import threading
+
+    import oracle_lib as O
+
+    n, therm, meas, stride = (4, 4, 4, 4), 128, 32, 2
+    master = 0x3C0FFEE000000001
+    ncpu = 8
+    cpu = [None] * ncpu
+
+    def run(i):
+        cfg = O.Su3ChainConfig()
+        cfg.extents[:] = list(n)
+        cfg.beta, cfg.master_seed, cfg.chain_id = 6.0, master, 100 + i
+        cfg.thermalization_sweeps, cfg.measurements, cfg.sweeps_between_measurements = therm, meas, stride
+        cfg.proposal_angle, cfg.autocorrelation_max_lag = 0.35, 24
+        m = np.zeros(meas)
+        d, r = O.ChainDiagnostics(), O.Su3ChainReport()
+        assert O.lib().ymo_su3_run_chain(C.byref(cfg), O.dptr(m), C.byref(d), C.byref(r), None) == 0
+        cpu[i] = (m, r.accepts / r.proposals)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(ncpu)]
+    for t in threads:
+        t.start()
+    gpu, acc = [], []
+    for chain in range(16):
+        key = ym.derive_su3_chain_seed(master, 200 + chain)
+        lat = ym.Su3Lattice4(n, 32)
+        meas_s, tp, ta = [], 0, 0
+        for s in range(therm + meas * stride):
+            p, a = lat.metropolis_sweeps(6.0, 0.35, key, s, 1, 8)
+            tp, ta = tp + p, ta + a
+            if s >= therm and (s - therm) % stride == stride - 1:
+                meas_s.append(lat.average_plaquette())
+        gpu.append(np.mean(meas_s))
+        acc.append(ta / tp)
+    for t in threads:
+        t.join()
+    gm, gse = _jackknife_over_chains(gpu)
+    cm, cse = _jackknife_over_chains([c[0].mean() for c in cpu])
+    z = abs(gm - cm) / math.hypot(gse, cse)
+    assert z < 2.0, (gm, gse, cm, cse, z)
+    assert abs(np.mean(acc) - np.mean([c[1] for c in cpu])) < 0.004
+    assert 0.56 <= cm <= 0.63  # the reference's external envelope (su3_benchmark.rs:23-28)
+
+
 def test_su2_fp32_storage_errors(ym):
     r = ym.precision.su2_precision(8, 2.3, therm=10, chain=20, loops=4)
     assert r["plaquette_abs_err"] < 2e-6
