@@ -205,8 +205,22 @@ def side_workload(name):
         for (g, lev, kind), ev in rep.correlators.items():
             if kind == 0 and lev in (0, 2, 4):
                 corr[f"{g[0]}^3x{g[3]} stout{lev} C(1)/C(0)"] = float(ev.full[1] / ev.full[0])
+        analysis = {}
+        for (g, lev), an in rep.analysis.items():
+            if lev not in (0, 4):
+                continue
+            em, pr, fit = an["effective_mass"], an["projection"], an["fit"]
+            analysis[f"{g[0]}^3x{g[3]} stout{lev}"] = {
+                "m_eff(1)": (f"{em.full_estimate[0]:.4f} +- {em.standard_error[0]:.4f}"
+                             if isinstance(em, ym_amd.stats.EffectiveMassSeries) else f"status {em['status']}"),
+                "gevp_lambda(t1=2)": (f"{pr.principal_eigenvalue:.5f}" if isinstance(pr, ym_amd.stats.PrincipalProjection)
+                                      else f"status {pr['status']}"),
+                "fit": (f"window {fit.window_start}-{fit.window_end} m = {fit.fitted_mass:.3f} +- "
+                        f"{fit.mass_standard_error:.3f}" if not isinstance(fit, dict) else f"status {fit['status']}"),
+            }
         return {"workload": "256 SU(3) chains (64 each of 4^3x10, 6^3x12, 8^3x16, 12^3x24), beta=5.7, "
-                            "68 sweeps from cold, A1++ on raw and stout 1-4, C(t) + jackknife",
+                            "68 sweeps from cold, A1++ on raw and stout 1-4, C(t) + jackknife, host analysis",
+                "analysis": analysis,
                 "device_ms": rep.device_ms, "wall_s": wall,
                 "link_updates_per_s": links * rep.sweeps / (rep.device_ms * 1e-3),
                 "mean_plaquette_by_geometry": {f"{g[0]}^3x{g[3]}": float(np.mean(rep.plaquette[i * 64:(i + 1) * 64]))

@@ -67,9 +67,9 @@ __device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t k
 // Staple A(x) of the link U_MU(x), x of parity p at slot s: the six staples
 // in the storage precision with the fixed FMA chain (mul_nd/acc_nd forward,
 // mul_dd/acc_nn backward, nu ascending).  The three links of term k + 1 are
-// loaded before term k is multiplied, so a wave keeps two terms' loads in
-// flight instead of waiting on each term's links in turn; the arithmetic and
-// its order are unchanged.
+// loaded before term k is multiplied (FP32 storage), so a wave keeps two
+// terms' loads in flight instead of waiting on each term's links in turn; the
+// arithmetic and its order are unchanged.
 template <typename Real, int MU>
 __device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *__restrict__ L, const Geom &g, int p,
                                               int s, const int x[4], M3<Real> &A) {
@@ -94,6 +94,21 @@ __device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *_
         ++k;
         dir[k] = nu; px[k] = p; sx[k] = s_pmmn; sy[k] = s_mn; pz[k] = q; sz[k] = s_mn;
         ++k;
+    }
+    if constexpr (sizeof(Real) == 8) {
+        // FP64 storage: double-buffered FP64 matrices would not fit in registers
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            M3<Real> T;
+            const M3<Real> X = load3<Real>(L, g.vh, dir[k], px[k], sx[k]);
+            const M3<Real> Y = load3<Real>(L, g.vh, MU, q, sy[k]);
+            if ((k & 1) == 0) mul_nd(X, Y, T);
+            else mul_dd(X, Y, T);
+            const M3<Real> Z = load3<Real>(L, g.vh, dir[k], pz[k], sz[k]);
+            if ((k & 1) == 0) acc_nd(A, T, Z);
+            else acc_nn(A, T, Z);
+        }
+        return;
     }
     M3<Real> X[2], Y[2], Z[2];
     X[0] = load3<Real>(L, g.vh, dir[0], px[0], sx[0]);
