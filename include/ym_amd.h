@@ -367,8 +367,10 @@ int32_t ym_mlp_train(ym_mlp *mlp, int32_t epochs, int32_t batch, float lr, uint6
 /* n_sweeps checkerboard sweeps (SU(3), FP32 storage): per link one learned
  * independence proposal with MH correction, then metro_hits Metropolis hits
  * (generator h mod 8, proposal_angle), FP64 reunitarisation.  Counts are
- * accumulated.  The call keeps its weight fragments and counters in the MLP
- * object: calls that share one MLP must not run concurrently. */
+ * accumulated.  The network is only read (its BF16 weight fragments are
+ * repacked by ym_mlp_set_params / ym_mlp_train), and counters and events
+ * belong to the lattice handle, so calls on different lattices may share one
+ * MLP concurrently (from different host threads) while no call changes it. */
 int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *mlp, double beta, uint64_t key, uint64_t first_sweep,
                               int32_t n_sweeps, int32_t metro_hits, double proposal_angle, int32_t bf16,
                               uint64_t *learned_proposals, uint64_t *learned_accepts, uint64_t *metro_accepts,

@@ -55,6 +55,7 @@ struct ym_lattice {
     unsigned long long *dctr = nullptr;  // 4 counters
     int *dstatus = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t aux0 = nullptr, aux1 = nullptr;  // timing of the learned sweeps
 };
 
 namespace ym {
@@ -66,6 +67,11 @@ int32_t lattice_info(const ym_lattice *l, int32_t *group, int32_t *prec, int64_t
     return YM_OK;
 }
 void *lattice_links(const ym_lattice *l) { return l->links; }
+unsigned long long *lattice_counters(const ym_lattice *l) { return l->dctr; }
+void lattice_aux_events(const ym_lattice *l, hipEvent_t *e0, hipEvent_t *e1) {
+    *e0 = l->aux0;
+    *e1 = l->aux1;
+}
 hipStream_t lattice_stream(const ym_lattice *l) { return l->stream; }
 }  // namespace ym
 
@@ -243,6 +249,7 @@ int32_t ym_lattice_create(int32_t group, const int64_t extents[4], int32_t preci
     if (!st && hipMalloc((void **)&l->dctr, 4 * sizeof(unsigned long long)) != hipSuccess) st = YM_ALLOCATION_FAILED;
     if (!st && hipMalloc((void **)&l->dstatus, sizeof(int)) != hipSuccess) st = YM_ALLOCATION_FAILED;
     if (!st && (hipEventCreate(&l->ev0) != hipSuccess || hipEventCreate(&l->ev1) != hipSuccess)) st = YM_HIP_ERROR;
+    if (!st && (hipEventCreate(&l->aux0) != hipSuccess || hipEventCreate(&l->aux1) != hipSuccess)) st = YM_HIP_ERROR;
     if (!st) st = ym_lattice_set_identity(l);
     if (st) {
         ym_lattice_destroy(l);
@@ -262,6 +269,8 @@ int32_t ym_lattice_destroy(ym_lattice *l) {
     if (l->dstatus) (void)hipFree(l->dstatus);
     if (l->ev0) (void)hipEventDestroy(l->ev0);
     if (l->ev1) (void)hipEventDestroy(l->ev1);
+    if (l->aux0) (void)hipEventDestroy(l->aux0);
+    if (l->aux1) (void)hipEventDestroy(l->aux1);
     if (l->stream) (void)hipStreamDestroy(l->stream);
     delete l;
     return YM_OK;

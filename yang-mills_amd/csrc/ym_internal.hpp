@@ -13,6 +13,8 @@ struct Geom;
 // Accessors of a lattice handle for the other kernel families.
 int32_t lattice_info(const ym_lattice *lat, int32_t *group, int32_t *prec, int64_t ext[4]);
 void *lattice_links(const ym_lattice *lat);
+unsigned long long *lattice_counters(const ym_lattice *lat);  // 4 device counters of the handle
+void lattice_aux_events(const ym_lattice *lat, hipEvent_t *e0, hipEvent_t *e1);
 hipStream_t lattice_stream(const ym_lattice *lat);
 
 void launch_su3_metropolis(int prec, void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,

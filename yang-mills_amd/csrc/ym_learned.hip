@@ -725,11 +725,10 @@ struct ym_mlp {
     float *feats = nullptr, *targets = nullptr;
     int *valid = nullptr;
     hipStream_t stream = nullptr;
-    // per-call state of ym_su3_learned_sweeps / ym_mlp_forward, kept so
-    // that a call allocates nothing: BF16 weight fragments, counters, events
+    // BF16 B-operand fragments of the weights, repacked whenever the
+    // parameters change (ym_mlp_set_params, ym_mlp_train); sweeps and
+    // inference only read them
     ym::bf16x8 *packed = nullptr;
-    unsigned long long *cnt = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
 namespace {
@@ -836,8 +835,6 @@ int32_t ym_mlp_create(uint64_t seed, int64_t capacity, ym_mlp **out) {
     A((void **)&m->targets, sizeof(float) * 8 * capacity);
     A((void **)&m->valid, sizeof(int) * capacity);
     A((void **)&m->packed, sizeof(ym::bf16x8) * ym::kFrags * 64);
-    A((void **)&m->cnt, sizeof(unsigned long long) * 3);
-    if (!st && (hipEventCreate(&m->e0) != hipSuccess || hipEventCreate(&m->e1) != hipSuccess)) st = YM_HIP_ERROR;
     if (!st) {
         std::vector<float> p;
         init_params(p, seed);
@@ -854,11 +851,9 @@ int32_t ym_mlp_create(uint64_t seed, int64_t capacity, ym_mlp **out) {
 int32_t ym_mlp_destroy(ym_mlp *m) {
     if (!m) return YM_INVALID_ARGUMENT;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    void *b[] = {m->params, m->m1, m->m2, m->grad, m->feats, m->targets, m->valid, m->packed, m->cnt};
+    void *b[] = {m->params, m->m1, m->m2, m->grad, m->feats, m->targets, m->valid, m->packed};
     for (void *x : b)
         if (x) (void)hipFree(x);
-    if (m->e0) (void)hipEventDestroy(m->e0);
-    if (m->e1) (void)hipEventDestroy(m->e1);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return YM_OK;
@@ -879,6 +874,7 @@ int32_t ym_mlp_set_params(ym_mlp *m, const float *params) {
     if (!st) {
         ym::launch_fill_zero((double *)m->m1, ym::kParams / 2, m->stream);
         ym::launch_fill_zero((double *)m->m2, ym::kParams / 2, m->stream);
+        hipLaunchKernelGGL(ym::k_mlp_pack, dim3(ym::kFrags), dim3(64), 0, m->stream, m->params, m->packed);
         m->step = 0;
         st = hs(hipGetLastError());
     }
@@ -965,7 +961,6 @@ int32_t ym_mlp_forward(const ym_mlp *mc, const float *feats, int64_t n, int32_t 
     int32_t st = hs(hipMemcpyAsync(df, feats, sizeof(float) * ym::kIn * n, hipMemcpyHostToDevice, m->stream));
     const dim3 grid((unsigned)((n + 255) / 256));
     if (!st) {
-        hipLaunchKernelGGL(ym::k_mlp_pack, dim3(ym::kFrags), dim3(64), 0, m->stream, m->params, m->packed);
         if (bf16)
             hipLaunchKernelGGL(ym::k_mlp_infer<true>, grid, dim3(256), 0, m->stream, m->params, m->packed, df, n,
                                dout);
@@ -1031,6 +1026,9 @@ int32_t ym_mlp_train(ym_mlp *m, int32_t epochs, int32_t batch, float lr, uint64_
         if (epoch_loss) epoch_loss[ep] = csum > 0 ? (float)(lsum / csum) : 0.0f;
     }
     if (!st) st = hs(hipEventRecord(e1, m->stream));
+    // the BF16 fragments follow the parameters (sweeps only read them)
+    if (!st) hipLaunchKernelGGL(ym::k_mlp_pack, dim3(ym::kFrags), dim3(64), 0, m->stream, m->params, m->packed);
+    if (!st) st = hs(hipGetLastError());
     if (!st) st = hs(hipStreamSynchronize(m->stream));
     if (!st && device_ms) st = hs(hipEventElapsedTime(device_ms, e0, e1));
     if (e0) (void)hipEventDestroy(e0);
@@ -1055,12 +1053,15 @@ int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *m, double beta, uin
     const ym::Geom g = ym::make_geom(ext);
     float2 *L = (float2 *)ym::lattice_links(lat);
     hipStream_t ls = ym::lattice_stream(lat);
-    unsigned long long *cnt = m->cnt;
-    hipEvent_t e0 = m->e0, e1 = m->e1;
+    // Counters and events of the lattice handle: the network is only read, so
+    // several lattices (own streams, own host threads) may sweep with one
+    // network at the same time.
+    unsigned long long *cnt = ym::lattice_counters(lat);
+    hipEvent_t e0, e1;
+    ym::lattice_aux_events(lat, &e0, &e1);
     st = hs(hipStreamSynchronize(m->stream));
     if (!st) {
         ym::launch_fill_zero_u64(cnt, 3, ls);
-        hipLaunchKernelGGL(ym::k_mlp_pack, dim3(ym::kFrags), dim3(64), 0, ls, m->params, m->packed);
         st = hs(hipEventRecord(e0, ls));
     }
     const int nb = (g.vh + 255) / 256;
