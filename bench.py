@@ -168,6 +168,50 @@ def learned_vs_plain(chains=8, sweeps=2500, discard=500, therm=2000, equil=1000,
             out[kind]["acceptance"] = float(np.mean(acc))
     out["ess_per_s_ratio"] = out["learned"]["ess_per_s"] / out["plain"]["ess_per_s"]
     out["ess_per_device_s_ratio"] = out["learned"]["ess_per_device_s"] / out["plain"]["ess_per_device_s"]
+
+    # The same chains run concurrently: one host thread per chain, each
+    # lattice on its own HIP stream, the learned chains sharing the one
+    # network.  At 16^3x32 a single chain leaves most of the GPU idle (the
+    # learned sub-pass is one wave per SIMD), so this is how several chains
+    # are run in practice; ESS per wall second of the whole set.
+    for kind in ("plain", "learned"):
+        lats = [starts[c].clone() for c in range(chains)]
+        if kind == "plain":
+            for l in lats:
+                l.autotune(6.0, 0.35, 8)
+        vals = [[] for _ in range(chains)]
+        errors = []
+
+        def run(c, kind=kind, lats=lats, vals=vals, errors=errors):
+            try:
+                l = lats[c]
+                for s in range(sweeps):
+                    sw = 200000 + s
+                    if kind == "plain":
+                        l.metropolis_sweeps(6.0, 0.35, keys[c], sw, 1, 8)
+                    else:
+                        ym_amd.learned.learned_sweeps(l, mlp, 6.0, keys[c], sw, 1, metro_hits=0, bf16=True)
+                    vals[c].append(l.average_plaquette())
+            except Exception as e:  # reported after the join
+                errors.append(e)
+
+        th = [threading.Thread(target=run, args=(c,)) for c in range(chains)]
+        t1 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        wall = time.perf_counter() - t1
+        if errors:
+            raise errors[0]
+        rows = np.array([v[discard:] for v in vals])
+        lag = min(100, rows.shape[1] // 2)
+        pool = ym_amd.stats.pool_chains(rows, lag)
+        out[kind]["concurrent"] = {"seconds": wall, "plaquette": pool.pooled_mean, "plaquette_se": pool.pooled_se,
+                                   "ess": pool.aggregate_ess, "ess_per_s": pool.aggregate_ess / wall,
+                                   "sweeps_per_s": chains * sweeps / wall}
+    out["concurrent_ess_per_s_ratio"] = out["learned"]["concurrent"]["ess_per_s"] / \
+        out["plain"]["concurrent"]["ess_per_s"]
     d = out["learned"]["plaquette"] - out["plain"]["plaquette"]
     se = max(out["learned"]["plaquette_se"], out["learned"]["between_chain_se"]) ** 2 + \
         max(out["plain"]["plaquette_se"], out["plain"]["between_chain_se"]) ** 2
@@ -318,6 +362,8 @@ def main():
         log(f"ESS/s learned {learned['learned']['ess_per_s']:.1f} vs plain {learned['plain']['ess_per_s']:.1f} "
             f"(x{learned['ess_per_s_ratio']:.2f}; device time x{learned['ess_per_device_s_ratio']:.2f}), "
             f"acceptance {learned['learned'].get('acceptance', 0):.3f}")
+        log(f"concurrent chains: ESS/s learned {learned['learned']['concurrent']['ess_per_s']:.1f} vs plain "
+            f"{learned['plain']['concurrent']['ess_per_s']:.1f} (x{learned['concurrent_ess_per_s_ratio']:.2f})")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -132,6 +132,40 @@ def test_learned_sweeps_deterministic_and_accepting(ym, thermal):
     assert cbf.learned_proposals > 0
 
 
+def test_learned_sweeps_share_one_network_across_threads(ym, thermal):
+    """Four lattices (own streams) sweep with one trained network from four
+    host threads at once: links and counts equal the same sweeps run one
+    after the other; the network's parameters are unchanged."""
+    import threading
+
+    m = ym.learned.Mlp(seed=13, dataset_capacity=8 * 8 ** 4 * 2)
+    m.collect_pairs(thermal.clone())
+    m.train(epochs=2, batch=4096, lr=3e-3)
+    p0 = m.get_params()
+    keys = [ym.derive_su3_chain_seed(0xABC, c) for c in range(4)]
+    seq = []
+    for c in range(4):
+        lat = thermal.clone()
+        r = ym.learned.learned_sweeps(lat, m, 6.0, keys[c], 700, 3, metro_hits=1, bf16=(c % 2 == 0))
+        seq.append((lat.download(), r.learned_proposals, r.learned_accepts, r.metro_accepts))
+    lats = [thermal.clone() for _ in range(4)]
+    out = [None] * 4
+
+    def run(c):
+        r = ym.learned.learned_sweeps(lats[c], m, 6.0, keys[c], 700, 3, metro_hits=1, bf16=(c % 2 == 0))
+        out[c] = (r.learned_proposals, r.learned_accepts, r.metro_accepts)
+
+    th = [threading.Thread(target=run, args=(c,)) for c in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for c in range(4):
+        assert out[c] == seq[c][1:], c
+        assert np.array_equal(lats[c].download(), seq[c][0]), c
+    assert np.array_equal(m.get_params(), p0)
+
+
 def test_learned_chain_targets_wilson_action(ym, thermal):
     """Plaquette of learned chains (1 learned proposal + 1 Metropolis hit per
     link) against plain Metropolis chains (8 hits).  Every chain is
