@@ -386,6 +386,7 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
         ym::Geom g = l->g;
         g.tmode = mode[c];
         g.t1 = strip[c];
+        ym::set_divisors(g);
         const bool valid = mode[c] == 0 ? (strip[c] == 0 || (g.tk > 0 && g.n[1] % strip[c] == 0))
                                         : (g.tq > 0 && (g.n[1] / 2) % strip[c] == 0);
         if (!valid) {
@@ -417,6 +418,7 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
     if (!st) {
         l->g.tmode = bm;
         l->g.t1 = bs;
+        ym::set_divisors(l->g);
         *best_mode = bm;
         *best_strip = bs;
     }

@@ -176,6 +176,27 @@ __device__ __forceinline__ double expm1_nonpos(double x) {
 }
 
 // ------------------------------------------------------------- geometry
+// Division by a divisor fixed for a launch, as a multiply and a shift
+// (Granlund & Montgomery, PLDI 1994, Thm 4.2 with N = 30): for d >= 1 and
+// l = ceil(log2 d), m = floor(2^(30+l) / d) + 1 < 2^32 gives
+// floor(n * m / 2^(30+l)) = floor(n / d) for every 0 <= n < 2^30.  Slot,
+// row and tile indices stay below 2^30 (at most 2^30 sites per lattice).
+struct FastDiv {
+    uint32_t m, s, d;
+};
+__host__ inline FastDiv make_fastdiv(int d) {
+    FastDiv f{0u, 0u, (uint32_t)d};
+    if (d < 1) return f;
+    uint32_t l = 0;
+    while ((1ull << l) < (uint64_t)d) ++l;
+    f.s = 30u + l;
+    f.m = (uint32_t)(((1ull << f.s) / (uint64_t)d) + 1ull);
+    return f;
+}
+__host__ __device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
+    return (int)(((uint64_t)(uint32_t)n * (uint64_t)f.m) >> f.s);
+}
+
 struct Geom {
     int n[4];
     int n3p;   // N3 rounded up to even
@@ -186,7 +207,22 @@ struct Geom {
     int t1;    // x1 strip width of the tile visiting order, 0 = plain order
     int tq;    // 64-slot quarters per (x0, x1) pair for compact tiles, 0 when not available
     int tmode; // 0: a tile is 256 consecutive slots; 1: a tile is 2 x 2 (x0, x1) x 64 slots
+    // divisors of site_of / tile_order / tile_slot (set_divisors keeps them current)
+    FastDiv dv_half, dv_n2, dv_n1, dv_n0, dv_h0, dv_tk, dv_tq, dv_t1, dv_strip;
 };
+
+// Refresh the division constants after n, tk, tq or t1 changed.
+__host__ inline void set_divisors(Geom &g) {
+    g.dv_half = make_fastdiv(g.half);
+    g.dv_n2 = make_fastdiv(g.n[2]);
+    g.dv_n1 = make_fastdiv(g.n[1]);
+    g.dv_n0 = make_fastdiv(g.n[0]);
+    g.dv_h0 = make_fastdiv(g.n[0] >> 1);
+    g.dv_tk = make_fastdiv(g.tk);
+    g.dv_tq = make_fastdiv(g.tq);
+    g.dv_t1 = make_fastdiv(g.t1);
+    g.dv_strip = make_fastdiv(g.t1 > 0 ? g.t1 : 1);
+}
 
 // Geometry of an N0 x N1 x N2 x N3 lattice (extents already validated).
 __host__ inline Geom make_geom(const int64_t n[4]) {
@@ -201,6 +237,7 @@ __host__ inline Geom make_geom(const int64_t n[4]) {
     g.t1 = (g.tk > 0 && g.n[1] % 4 == 0) ? 4 : 0;
     g.tq = (row_slots % 64 == 0 && g.n[0] % 2 == 0 && g.n[1] % 2 == 0) ? row_slots / 64 : 0;
     g.tmode = 0;
+    set_divisors(g);
     return g;
 }
 
@@ -213,21 +250,21 @@ __host__ inline Geom make_geom(const int64_t n[4]) {
 // depend on it.
 __host__ __device__ __forceinline__ int tile_order(const Geom &g, int t) {
     if (g.t1 == 0) return t;
-    const int j = t % g.tk;
-    int r = t / g.tk;
-    const int x1i = r % g.t1;
-    r /= g.t1;
-    const int x0 = r % g.n[0], x1t = r / g.n[0];
+    int r = fdiv(t, g.dv_tk);
+    const int j = t - r * g.tk;
+    const int r1 = fdiv(r, g.dv_t1);
+    const int x1i = r - r1 * g.t1;
+    const int x1t = fdiv(r1, g.dv_n0), x0 = r1 - x1t * g.n[0];
     return (x0 * g.n[1] + x1t * g.t1 + x1i) * g.tk + j;
 }
 
 // Site of slot s in parity p; returns false for the padding slot of odd N3.
 __device__ __forceinline__ bool site_of(const Geom &g, int p, int s, int x[4]) {
-    int row = s / g.half;
+    int row = fdiv(s, g.dv_half);
     int j = s - row * g.half;
-    int r2 = row / g.n[2];
+    int r2 = fdiv(row, g.dv_n2);
     x[2] = row - r2 * g.n[2];
-    x[0] = r2 / g.n[1];
+    x[0] = fdiv(r2, g.dv_n1);
     x[1] = r2 - x[0] * g.n[1];
     x[3] = 2 * j + ((p + x[0] + x[1] + x[2]) & 1);
     return x[3] < g.n[3];
@@ -282,13 +319,14 @@ __device__ __forceinline__ int xcd_tile_rev(int b, int nb) {
 // strip.  Both are permutations of the Vh slots.
 __host__ __device__ __forceinline__ int tile_slot(const Geom &g, int t, int tid) {
     if (g.tmode == 1) {
-        const int j = t % g.tq;
-        int r = t / g.tq;
+        const int r = fdiv(t, g.dv_tq);
+        const int j = t - r * g.tq;
         const int strip = g.t1 > 0 ? g.t1 : 1;
-        const int x1i = r % strip;
-        r /= strip;
+        const int r1 = fdiv(r, g.dv_strip);
+        const int x1i = r - r1 * strip;
         const int h0 = g.n[0] >> 1;
-        const int X0 = r % h0, X1 = (r / h0) * strip + x1i;
+        const int r2 = fdiv(r1, g.dv_h0);
+        const int X0 = r1 - r2 * h0, X1 = r2 * strip + x1i;
         const int a = tid >> 7, b = (tid >> 6) & 1;
         return ((2 * X0 + a) * g.n[1] + 2 * X1 + b) * (g.n[2] * g.half) + 64 * j + (tid & 63);
     }

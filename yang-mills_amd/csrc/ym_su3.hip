@@ -217,35 +217,31 @@ void launch_su3_hot(int prec, void *links, const Geom &g, uint64_t key, hipStrea
 }
 
 // --------------------------------------------------------- staple probe
-// Staple at one site, same arithmetic as the sweep, for parity tests.
-template <typename Real>
+// Staple at one site, with the sweep's own routine (su3_staple_at), for
+// parity tests.
+template <typename Real, int MU>
 __global__ void k_su3_staple_probe(const typename Vec2<Real>::type *__restrict__ L, Geom g, int x0, int x1, int x2,
-                                   int x3, int mu, double *out) {
+                                   int x3, double *out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int x[4] = {x0, x1, x2, x3};
-    const int p = parity_of(x), q = p ^ 1, s = slot_of(g, x);
+    const int x[4] = {x0, x1, x2, x3};
     M3<Real> A;
-    for (int c = 0; c < 18; ++c) A.v[c] = 0;
-    int xpm[4];
-    shifted(g, x, mu, +1, xpm);
-    for (int nu = 0; nu < 4; ++nu) {
-        if (nu == mu) continue;
-        int xpn[4], xmn[4], xpmmn[4];
-        shifted(g, x, nu, +1, xpn);
-        shifted(g, x, nu, -1, xmn);
-        shifted(g, xpm, nu, -1, xpmmn);
-        M3<Real> T;
-        mul_nd(load3<Real>(L, g.vh, nu, parity_of(xpm), slot_of(g, xpm)), load3<Real>(L, g.vh, mu, parity_of(xpn), slot_of(g, xpn)), T);
-        acc_nd(A, T, load3<Real>(L, g.vh, nu, p, s));
-        mul_dd(load3<Real>(L, g.vh, nu, parity_of(xpmmn), slot_of(g, xpmmn)), load3<Real>(L, g.vh, mu, parity_of(xmn), slot_of(g, xmn)), T);
-        acc_nn(A, T, load3<Real>(L, g.vh, nu, parity_of(xmn), slot_of(g, xmn)));
-    }
-    (void)q;
+    su3_staple_at<Real, MU>(L, g, parity_of(x), slot_of(g, x), x, A);
     for (int c = 0; c < 18; ++c) out[c] = (double)A.v[c];
 }
+template <typename Real>
+static void staple_probe_t(const void *links, const Geom &g, const int x[4], int mu, double *out, hipStream_t st) {
+    using V2 = typename Vec2<Real>::type;
+    const V2 *L = (const V2 *)links;
+    switch (mu) {
+    case 0: hipLaunchKernelGGL((k_su3_staple_probe<Real, 0>), dim3(1), dim3(64), 0, st, L, g, x[0], x[1], x[2], x[3], out); break;
+    case 1: hipLaunchKernelGGL((k_su3_staple_probe<Real, 1>), dim3(1), dim3(64), 0, st, L, g, x[0], x[1], x[2], x[3], out); break;
+    case 2: hipLaunchKernelGGL((k_su3_staple_probe<Real, 2>), dim3(1), dim3(64), 0, st, L, g, x[0], x[1], x[2], x[3], out); break;
+    default: hipLaunchKernelGGL((k_su3_staple_probe<Real, 3>), dim3(1), dim3(64), 0, st, L, g, x[0], x[1], x[2], x[3], out); break;
+    }
+}
 void launch_su3_staple_probe(int prec, const void *links, const Geom &g, const int x[4], int mu, double *out, hipStream_t st) {
-    if (prec == 32) hipLaunchKernelGGL((k_su3_staple_probe<float>), dim3(1), dim3(64), 0, st, (const float2 *)links, g, x[0], x[1], x[2], x[3], mu, out);
-    else hipLaunchKernelGGL((k_su3_staple_probe<double>), dim3(1), dim3(64), 0, st, (const double2 *)links, g, x[0], x[1], x[2], x[3], mu, out);
+    if (prec == 32) staple_probe_t<float>(links, g, x, mu, out, st);
+    else staple_probe_t<double>(links, g, x, mu, out, st);
 }
 
 // ------------------------------------------------------ reunitarisation
