@@ -16,8 +16,9 @@ against plain Metropolis, is measured on SU(3) 16^3x32 (learned_vs_plain).
 Prints one JSON line with the throughput (link updates/s), the roofline of
 the dominant kernel (k_su3_metropolis_ws -- k_su3_metropolis on lattices of
 fewer than 256 x 256 links per sub-pass -- one launch per checkerboard sub-pass:
-algorithmic bytes per launch / mean launch duration from HIP events on its
-stream) and the CPU baseline (the reference-exact restatement of
+algorithmic bytes per launch / mean launch duration, the device time of the
+timed region by one HIP event pair on the kernel's stream over its launch
+count) and the CPU baseline (the reference-exact restatement of
 metropolis_sweep timed on the host cores).
 """
 import argparse
@@ -223,9 +224,38 @@ def side_workload(name):
     """Workloads beside the headline sweep, each printed as its own JSON line:
     FP32-vs-FP64 observable errors for SU(2) (32^4, 48^4 x beta 2.3, 2.5)
     and SU(3) (24^4, 32^4 with 10 stout levels), and the batched 256-chain
-    glueball production."""
+    glueball production; SU(2) FP32 heat-bath and over-relaxation sweep
+    throughput at 32^4 and 48^4 (su2-sweeps)."""
     import ym_amd
 
+    if name == "su2-sweeps":
+        rows = []
+        for ext in (32, 48):
+            n = (ext,) * 4
+            lat = ym_amd.Lattice4(n, 32)
+            key = ym_amd.derive_chain_seed(0x2B00201211160101, 0)
+            lat.hot_start(key)
+            for s in range(5):
+                lat.heatbath_sweeps(2.3, key, s, 1)
+                lat.overrelax_sweeps(3)
+            reps = 10
+            lat.timer_begin()
+            for s in range(reps):
+                lat.heatbath_sweeps(2.3, key, 5 + s, 1)
+            hb = lat.timer_end() / reps
+            lat.timer_begin()
+            lat.overrelax_sweeps(3 * reps)
+            orx = lat.timer_end() / (3 * reps)
+            upd = 4 * ext ** 4
+            log(f"SU(2) {ext}^4: heat-bath sweep {hb * 1e3:.1f} us, over-relaxation sweep {orx * 1e3:.1f} us")
+            rows.append({"extent": ext, "beta": 2.3, "heatbath_us_per_sweep": hb * 1e3,
+                         "overrelax_us_per_sweep": orx * 1e3,
+                         "heatbath_link_updates_per_s": upd / (hb * 1e-3),
+                         "overrelax_link_updates_per_s": upd / (orx * 1e-3),
+                         "overrelax_hbm_fraction": upd * 144 / (orx * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                         "heatbath_hbm_fraction": upd * 144 / (hb * 1e-3) / (HBM_PEAK_GBS * 1e9)})
+        return {"workload": "SU(2) FP32 links, hot start, beta=2.3, checkerboard heat-bath / over-relaxation "
+                            "(144 B per link update: 9 transfers of 16 B)", "results": rows}
     if name == "su2-precision":
         rows = []
         for ext in (32, 48):
@@ -286,7 +316,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="su3",
-                    choices=["su3", "su2-precision", "su3-precision", "ensemble"],
+                    choices=["su3", "su2-sweeps", "su2-precision", "su3-precision", "ensemble"],
                     help="su3 (default): the SU(3) 32^4 sweep metric; the others print their own JSON line")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
@@ -335,9 +365,14 @@ def main():
     log(f"warmup {args.warmup} sweeps")
     lat.metropolis_sweeps(args.beta, angle, key, 0, args.warmup, args.hits)
     log(f"timed {args.steps} sweeps")
+    # One HIP event pair on the lattice stream around the whole timed region
+    # (8 sub-pass launches per sweep, nothing else on the stream): the mean
+    # launch duration is the region's device time over the launch count, so
+    # it includes the gaps between back-to-back launches.  Events between
+    # the launches are left out because each one delays the next launch.
+    launches = 8 * args.steps
     t0 = time.perf_counter()
-    prop, acc, total_ms, ksum_ms, launches = lat.metropolis_sweeps_profiled(
-        args.beta, angle, key, args.warmup, args.steps, args.hits)
+    prop, acc, total_ms = lat.metropolis_sweeps_timed(args.beta, angle, key, args.warmup, args.steps, args.hits)
     wall = time.perf_counter() - t0
     if args.traffic_calibration:
         log(f"calibration: device copy of {lat.device_bytes()} bytes")
@@ -346,7 +381,7 @@ def main():
     unit_res, det_res = lat.link_residuals()
     updates = 4 * V * args.steps
     value = updates / wall
-    avg_launch_s = ksum_ms / launches / 1e3
+    avg_launch_s = total_ms / launches / 1e3
     bytes_per_launch = (V // 2) * BYTES_PER_UPDATE_FP32
     achieved = bytes_per_launch / avg_launch_s / 1e9
     kernel = "k_su3_metropolis_ws" if (V // 2 + 255) // 256 >= 256 else "k_su3_metropolis"
