@@ -139,7 +139,10 @@ def learned_vs_plain(chains=8, sweeps=2500, discard=500, therm=2000, equil=1000,
         for c in range(chains):
             l = starts[c].clone()
             if kind == "plain":
-                l.autotune(6.0, 0.35, 8)
+                best, cand = l.autotune(6.0, 0.35, 8)
+                if c == 0:
+                    out["plain_sub_pass_mapping"] = {"best": f"{best[0]}/{best[1]}",
+                                                     "us_per_sub_pass": {f"{m}/{w}": t for (m, w), t in cand.items()}}
             vals = []
             t1 = time.perf_counter()
             for s in range(sweeps):

@@ -162,8 +162,11 @@ int32_t ym_su3_metropolis_sweeps(ym_lattice *lat, double beta, double proposal_a
  * (2 x 2 (x0, x1) rows x 64 slots) in X1-strips of {1, 2, 4, 8} -- on a
  * scratch copy of the links (reps timed sweeps each, after one untimed),
  * keeps the fastest for this lattice and returns it in *best_mode (0 row, 1
- * compact) and *best_strip; us_per_candidate (9 entries, may be NULL) gets
+ * compact) and *best_strip; us_per_candidate (10 entries, may be NULL) gets
  * the mean time per sub-pass, -1 where a mapping does not fit the extents.
+ * From 256 tiles of 256 links up the sub-pass uses the wave-specialised
+ * kernel; there the one-link-per-thread kernel is also timed with the best
+ * mapping (entry 9, else -1) and kept if faster, reported as *best_mode + 2. 
  * Every mapping permutes independent links, so results do not depend on it. */
 int32_t ym_su3_autotune(ym_lattice *lat, double beta, double proposal_angle, int32_t hits, int32_t reps,
                         int32_t *best_mode, int32_t *best_strip, float *us_per_candidate);

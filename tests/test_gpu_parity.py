@@ -534,3 +534,23 @@ def test_autotune_keeps_results_identical(ym):
     assert pa == (p1, a1) == (p2, a2)
     assert nl == 16 and 0 < ksum <= tot and ms > 0
     assert np.array_equal(a.download(), b.download()) and np.array_equal(a.download(), c.download())
+
+
+def test_autotune_times_per_link_kernel_on_large_lattices(ym):
+    """From 256 tiles up the sub-pass is the wave-specialised kernel and the
+    tuner also times the one-link-per-thread kernel with the best mapping;
+    whichever it keeps, the chain is bit-identical to an untuned lattice."""
+    n = (16, 16, 16, 32)  # 65536 links per (mu, parity): 256 tiles
+    a = ym.Su3Lattice4(n, 32)
+    b = ym.Su3Lattice4(n, 32)
+    key = ym.derive_su3_chain_seed(11, 2)
+    a.hot_start(key)
+    b.hot_start(key)
+    best, cand = a.autotune(6.0, 0.35, 8, reps=2)
+    assert best in cand
+    per_link = [c for c in cand if c[0].endswith("/per-link")]
+    assert len(per_link) == 1 and all(t > 0 for t in cand.values())
+    pa = a.metropolis_sweeps(6.0, 0.35, key, 0, 2, 8)
+    pb = b.metropolis_sweeps(6.0, 0.35, key, 0, 2, 8)
+    assert pa == pb
+    assert np.array_equal(a.download(), b.download())

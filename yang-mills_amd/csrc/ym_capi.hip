@@ -413,13 +413,41 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
             bs = strip[c];
         }
     }
+    // With at least 256 tiles the wave-specialised kernel runs; when a block
+    // gets only one or two tiles its producer and consumer waves barely
+    // overlap, so the one-link-per-thread kernel is timed too, with the best
+    // mapping, and kept when it is faster.
+    int kper = 0;
+    if (us_per_candidate) us_per_candidate[9] = -1.0f;
+    if (!st && (l->g.vh + 255) / 256 >= 256) {
+        ym::Geom g = l->g;
+        g.tmode = bm;
+        g.t1 = bs;
+        g.kper = 1;
+        ym::set_divisors(g);
+        for (int k = 0; k <= reps && !st; ++k) {
+            if (k == 1) st = hip_status(hipEventRecord(l->ev0, l->stream));
+            for (int mu = 0; mu < 4; ++mu)
+                for (int p = 0; p < 2; ++p)
+                    ym::launch_su3_metropolis(l->prec, scratch, g, mu, p, beta, angle, hits, 0x5EEDull, (uint64_t)k,
+                                              1, l->dctr, l->stream);
+        }
+        float ms = 0.0f;
+        if (!st) st = hip_status(hipEventRecord(l->ev1, l->stream));
+        if (!st) st = hip_status(hipEventSynchronize(l->ev1));
+        if (!st) st = hip_status(hipEventElapsedTime(&ms, l->ev0, l->ev1));
+        const float us = ms * 1e3f / (8.0f * (float)reps);
+        if (us_per_candidate) us_per_candidate[9] = us;
+        if (!st && us < best) kper = 1;
+    }
     if (!st) st = finish(l);
     (void)hipFree(scratch);
     if (!st) {
         l->g.tmode = bm;
         l->g.t1 = bs;
+        l->g.kper = kper;
         ym::set_divisors(l->g);
-        *best_mode = bm;
+        *best_mode = bm + 2 * kper;
         *best_strip = bs;
     }
     return st;

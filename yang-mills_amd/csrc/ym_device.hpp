@@ -207,6 +207,7 @@ struct Geom {
     int t1;    // x1 strip width of the tile visiting order, 0 = plain order
     int tq;    // 64-slot quarters per (x0, x1) pair for compact tiles, 0 when not available
     int tmode; // 0: a tile is 256 consecutive slots; 1: a tile is 2 x 2 (x0, x1) x 64 slots
+    int kper;  // SU(3) sub-pass kernel: 0 wave-specialised from 256 tiles up, 1 one link per thread
     // divisors of site_of / tile_order / tile_slot (set_divisors keeps them current)
     FastDiv dv_half, dv_n2, dv_n1, dv_n0, dv_h0, dv_tk, dv_tq, dv_t1, dv_strip;
 };
@@ -237,6 +238,7 @@ __host__ inline Geom make_geom(const int64_t n[4]) {
     g.t1 = (g.tk > 0 && g.n[1] % 4 == 0) ? 4 : 0;
     g.tq = (row_slots % 64 == 0 && g.n[0] % 2 == 0 && g.n[1] % 2 == 0) ? row_slots / 64 : 0;
     g.tmode = 0;
+    g.kper = 0;
     set_divisors(g);
     return g;
 }

@@ -151,7 +151,7 @@ static void launch_metro_t(void *links, const Geom &g, int mu, int p, double bet
     const double nbt = -(beta / 3.0);
     const int nb = (g.vh + 255) / 256;
     V2 *L = (V2 *)links;
-    if (nb >= 256) {
+    if (nb >= 256 && g.kper == 0) {
         // one 8-wave block per CU
         const int grid = 256;
         switch (mu) {

@@ -217,13 +217,19 @@ class Su3Lattice4(_Lattice):
     def autotune(self, beta=6.0, angle=0.35, hits=8, reps=3):
         """Pick the fastest thread-to-link mapping of the Metropolis sub-pass
         for this lattice.  Returns ((mode, strip), {(mode, strip): us per
-        sub-pass})."""
+        sub-pass}).  On lattices large enough for the wave-specialised kernel
+        the one-link-per-thread kernel is timed with the best mapping too,
+        under the mode "<mode>/per-link"; it is the result when faster."""
         mode, strip = C.c_int32(), C.c_int32()
-        us = (C.c_float * 9)()
+        us = (C.c_float * 10)()
         check(lib().ym_su3_autotune(self._h, float(beta), float(angle), int(hits), int(reps), C.byref(mode),
                                     C.byref(strip), us), "ym_su3_autotune")
-        best = (("row", "compact")[mode.value], strip.value)
-        return best, {c: us[i] for i, c in enumerate(self.AUTOTUNE_CANDIDATES) if us[i] >= 0}
+        names = ("row", "compact")
+        best = (names[mode.value % 2] + ("/per-link" if mode.value >= 2 else ""), strip.value)
+        cand = {c: us[i] for i, c in enumerate(self.AUTOTUNE_CANDIDATES) if us[i] >= 0}
+        if us[9] >= 0:
+            cand[(names[mode.value % 2] + "/per-link", strip.value)] = us[9]
+        return best, cand
 
     def metropolis_sweeps_profiled(self, beta, angle, key, first_sweep, n_sweeps, hits=8, reunitarize=True):
         """Same sweeps with HIP events around every sub-pass launch.
