@@ -228,9 +228,29 @@ def side_workload(name):
     FP32-vs-FP64 observable errors for SU(2) (32^4, 48^4 x beta 2.3, 2.5)
     and SU(3) (24^4, 32^4 with 10 stout levels), and the batched 256-chain
     glueball production; SU(2) FP32 heat-bath and over-relaxation sweep
-    throughput at 32^4 and 48^4 (su2-sweeps)."""
+    throughput at 32^4 and 48^4 (su2-sweeps); average_plaquette calls on
+    SU(3) and SU(2) FP32 lattices (plaquette)."""
     import ym_amd
 
+    if name == "plaquette":
+        rows = []
+        for group, n in ((3, (32,) * 4), (3, (16, 16, 16, 32)), (2, (32,) * 4)):
+            lat = (ym_amd.Su3Lattice4 if group == 3 else ym_amd.Lattice4)(n, 32)
+            lat.hot_start(5)
+            for _ in range(3):
+                lat.average_plaquette()
+            reps = 20
+            lat.timer_begin()
+            for _ in range(reps):
+                lat.average_plaquette()
+            us = lat.timer_end() / reps * 1e3
+            sites = n[0] * n[1] * n[2] * n[3]
+            nbytes = sites * 4 * (72 if group == 3 else 16)
+            log(f"SU({group}) {n}: {us:.1f} us per average_plaquette call")
+            rows.append({"group": group, "extents": list(n), "us_per_call": us, "link_bytes": nbytes,
+                         "algorithmic_gb_per_s_per_call": nbytes / (us * 1e-6) / 1e9})
+        return {"workload": "average_plaquette on FP32 hot-start lattices (kernel + partial sums + host copy "
+                            "per call; the kernel reads every link once)", "results": rows}
     if name == "su2-sweeps":
         rows = []
         for ext in (32, 48):
@@ -319,7 +339,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="su3",
-                    choices=["su3", "su2-sweeps", "su2-precision", "su3-precision", "ensemble"],
+                    choices=["su3", "su2-sweeps", "su2-precision", "su3-precision", "ensemble", "plaquette"],
                     help="su3 (default): the SU(3) 32^4 sweep metric; the others print their own JSON line")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)

@@ -52,6 +52,7 @@ struct ym_lattice {
     double *part = nullptr;  // per-block partial results
     size_t part_cap = 0;     // doubles
     double *dres = nullptr;  // small result buffer (64 doubles)
+    double *hres = nullptr;  // pinned host copy of dres (64 doubles)
     unsigned long long *dctr = nullptr;  // 4 counters
     int *dstatus = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -109,11 +110,18 @@ bool all_even(const ym_lattice *l) {
     return true;
 }
 
+// ncols <= 64 results through the pinned buffer: a device-to-host copy into
+// pageable memory is staged by the runtime and costs several microseconds
+// more per measurement call.
 int32_t sum_partials(ym_lattice *l, int nblk, int ncols, double *host_out) {
+    if (ncols < 1 || ncols > 64) return YM_INVALID_ARGUMENT;
     ym::launch_sum_partials(l->part, nblk, ncols, nblk, l->dres, l->stream);
     YM_HIP(hipGetLastError());
-    YM_HIP(hipMemcpyAsync(host_out, l->dres, sizeof(double) * ncols, hipMemcpyDeviceToHost, l->stream));
-    return finish(l);
+    YM_HIP(hipMemcpyAsync(l->hres, l->dres, sizeof(double) * ncols, hipMemcpyDeviceToHost, l->stream));
+    const int32_t st = finish(l);
+    if (st) return st;
+    std::memcpy(host_out, l->hres, sizeof(double) * ncols);
+    return YM_OK;
 }
 
 }  // namespace
@@ -246,6 +254,8 @@ int32_t ym_lattice_create(int32_t group, const int64_t extents[4], int32_t preci
     if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess) st = YM_HIP_ERROR;
     if (!st && hipMalloc(&l->links, l->link_bytes) != hipSuccess) st = YM_LATTICE_TOO_LARGE;
     if (!st && hipMalloc((void **)&l->dres, 64 * sizeof(double)) != hipSuccess) st = YM_ALLOCATION_FAILED;
+    if (!st && hipHostMalloc((void **)&l->hres, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        st = YM_ALLOCATION_FAILED;
     if (!st && hipMalloc((void **)&l->dctr, 4 * sizeof(unsigned long long)) != hipSuccess) st = YM_ALLOCATION_FAILED;
     if (!st && hipMalloc((void **)&l->dstatus, sizeof(int)) != hipSuccess) st = YM_ALLOCATION_FAILED;
     if (!st && (hipEventCreate(&l->ev0) != hipSuccess || hipEventCreate(&l->ev1) != hipSuccess)) st = YM_HIP_ERROR;
@@ -265,6 +275,7 @@ int32_t ym_lattice_destroy(ym_lattice *l) {
     if (l->links) (void)hipFree(l->links);
     if (l->part) (void)hipFree(l->part);
     if (l->dres) (void)hipFree(l->dres);
+    if (l->hres) (void)hipHostFree(l->hres);
     if (l->dctr) (void)hipFree(l->dctr);
     if (l->dstatus) (void)hipFree(l->dstatus);
     if (l->ev0) (void)hipEventDestroy(l->ev0);
@@ -776,13 +787,12 @@ int32_t ym_link_residuals(const ym_lattice *lc, double *unit, double *det) {
     if (l->group == 3) ym::launch_su3_residuals(l->prec, l->links, l->g, l->part, l->stream);
     else ym::launch_su2_residual(l->prec, l->links, l->g, l->part, l->stream);
     ym::launch_max_partials(l->part, nblk, 2, 2, l->dres, l->stream);
-    double r[2];
     YM_HIP(hipGetLastError());
-    YM_HIP(hipMemcpyAsync(r, l->dres, sizeof(r), hipMemcpyDeviceToHost, l->stream));
+    YM_HIP(hipMemcpyAsync(l->hres, l->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, l->stream));
     st = finish(l);
     if (st) return st;
-    *unit = r[0];
-    *det = r[1];
+    *unit = l->hres[0];
+    *det = l->hres[1];
     return YM_OK;
 }
 

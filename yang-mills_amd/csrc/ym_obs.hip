@@ -13,7 +13,9 @@ namespace ym {
 // ------------------------------------------------------------ plaquette
 // Per site: sum over mu < nu of (1/N) Re Tr P_{mu nu}(x) with
 // P = [U_mu(x) U_nu(x+mu)] [U_nu(x) U_mu(x+nu)]^dagger (plaquette_site);
-// part[parity * nblk + blk] = block sum.
+// part[parity * nblk + blk] = block sum.  Used for SU(3) FP64 links, whose
+// per-site registers leave no room for a 512-thread block; the other storage
+// types use k_plaquette_pp below.
 template <class G>
 __global__ __launch_bounds__(256) void k_plaquette(const typename G::Store *__restrict__ L, Geom g,
                                                    double *__restrict__ part) {
@@ -24,6 +26,45 @@ __global__ __launch_bounds__(256) void k_plaquette(const typename G::Store *__re
     if (s < g.vh) acc = plaquette_site<G>(L, g, p, s);
     const double t = block_sum<256>(acc, red);
     if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// The same partials with both parities of 256 consecutive slots (a tile) in
+// one 512-thread block (waves 0-3 parity 0, waves 4-7 parity 1): a site's
+// plaquette neighbours are mostly sites of the other parity in the same
+// tile, so a tile's links are fetched once for both parity sums.  Visiting
+// order: when an x0-slice is a multiple of 8 tiles (tps = tiles per slice),
+// each XCD takes the same eighth of every slice (a band of x1) and walks the
+// slices in x0 order, so the x0 and x1 neighbours a tile needs are loaded by
+// tiles that run at about the same time on the same XCD and are still in
+// its L2; otherwise tiles are dealt to the XCDs in contiguous runs
+// (xcd_tile).  The partial of a tile is written at the tile's own index and
+// each half reduces its 256 sites with block_sum<256>'s tree, so every
+// partial is bit-identical to k_plaquette's whatever the visiting order.
+__device__ __forceinline__ int plaquette_tile(int b, int nt, int tps) {
+    if (tps == 0) return xcd_tile(b, nt);
+    const int xcd = b & 7, idx = b >> 3, per = tps >> 3;
+    const int slice = idx / per;
+    return slice * tps + xcd * per + (idx - slice * per);
+}
+template <class G>
+__global__ __launch_bounds__(512) void k_plaquette_pp(const typename G::Store *__restrict__ L, Geom g, int tps,
+                                                      double *__restrict__ part) {
+    __shared__ double red[8];
+    const int tile = plaquette_tile(blockIdx.x, gridDim.x, tps);
+    const int p = threadIdx.x >> 8;
+    const int s = tile * 256 + (threadIdx.x & 255);
+    double v = 0.0;
+    if (s < g.vh) v = plaquette_site<G>(L, g, p, s);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if ((threadIdx.x & 255) == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) t += red[4 * p + w];
+        part[p * gridDim.x + tile] = t;
+    }
 }
 
 // Sum `n` partials in a fixed order into out[0] (one block).
@@ -53,15 +94,17 @@ void launch_max_partials(const double *part, int n, int ncols, int stride, doubl
 }
 
 int launch_plaquette(int group, int prec, const void *links, const Geom &g, double *part, hipStream_t st) {
-    dim3 grid((g.vh + 255) / 256, 2);
+    const int nt = (g.vh + 255) / 256;
+    const int slice = g.n[1] * g.n[2] * g.half;  // slots per x0-slice and parity
+    const int tps = (slice % 256 == 0 && (slice / 256) % 8 == 0 && g.vh == g.n[0] * slice) ? slice / 256 : 0;
     if (group == 3) {
-        if (prec == 32) hipLaunchKernelGGL((k_plaquette<G3<float>>), grid, dim3(256), 0, st, (const float2 *)links, g, part);
-        else hipLaunchKernelGGL((k_plaquette<G3<double>>), grid, dim3(256), 0, st, (const double2 *)links, g, part);
+        if (prec == 32) hipLaunchKernelGGL((k_plaquette_pp<G3<float>>), dim3(nt), dim3(512), 0, st, (const float2 *)links, g, tps, part);
+        else hipLaunchKernelGGL((k_plaquette<G3<double>>), dim3(nt, 2), dim3(256), 0, st, (const double2 *)links, g, part);
     } else {
-        if (prec == 32) hipLaunchKernelGGL((k_plaquette<G2<float>>), grid, dim3(256), 0, st, (const float4 *)links, g, part);
-        else hipLaunchKernelGGL((k_plaquette<G2<double>>), grid, dim3(256), 0, st, (const double4 *)links, g, part);
+        if (prec == 32) hipLaunchKernelGGL((k_plaquette_pp<G2<float>>), dim3(nt), dim3(512), 0, st, (const float4 *)links, g, tps, part);
+        else hipLaunchKernelGGL((k_plaquette_pp<G2<double>>), dim3(nt), dim3(512), 0, st, (const double4 *)links, g, tps, part);
     }
-    return (int)(grid.x * 2);
+    return 2 * nt;
 }
 
 // ------------------------------------------------------- Wilson loops
