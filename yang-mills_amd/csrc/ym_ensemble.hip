@@ -100,6 +100,39 @@ __global__ __launch_bounds__(256) void k_ens_plaquette(const typename Vec2<Real>
     if (threadIdx.x == 0) part[b] = t;
 }
 
+// FP32 links: both parities of one 256-slot tile per 512-thread block
+// (tiles[c+1]-tiles[c] = ceil(vh/256), the Metropolis table), as
+// k_plaquette_pp: a tile's links are fetched once for both parities.  Each
+// half writes the partial of its parity at the index k_ens_plaquette uses,
+// reduced with the same tree, so results are bit-identical.
+template <typename Real>
+__global__ __launch_bounds__(512) void k_ens_plaquette_pp(const typename Vec2<Real>::type *__restrict__ links,
+                                                          const ChainDesc *__restrict__ cd,
+                                                          const int *__restrict__ tiles,
+                                                          const int *__restrict__ begin, int nch,
+                                                          double *__restrict__ part) {
+    __shared__ double red[8];
+    const int b = blockIdx.x;
+    const int c = chain_of(tiles, nch, b);
+    const Geom g = cd[c].g;
+    const int nt = (g.vh + 255) / 256;
+    const int t = b - tiles[c];
+    const int p = threadIdx.x >> 8;
+    const int s = t * 256 + (threadIdx.x & 255);
+    double v = 0.0;
+    if (s < g.vh) v = plaquette_site<G3<Real>>(links + cd[c].link_off, g, p, s);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if ((threadIdx.x & 255) == 0) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) sum += red[4 * p + w];
+        part[begin[c] + p * nt + t] = sum;
+    }
+}
+
 // One stout level for every chain: begin[c+1]-begin[c] = 8 * ceil(vh/128)
 // (mu major, then parity, then 128-slot block).
 template <typename Real>
@@ -253,8 +286,12 @@ void enqueue_measure(ym_ensemble *E, int levels, double rho, double *out_plaq, d
     using V2 = typename ym::Vec2<Real>::type;
     const V2 *L = (const V2 *)E->links;
     const uint64_t ops_len = (uint64_t)(levels + 1) * E->tsum * 2;
-    hipLaunchKernelGGL((ym::k_ens_plaquette<Real>), dim3(E->h_plaq[E->nch]), dim3(256), 0, E->stream, L, E->d_cd,
-                       E->d_plaq, E->nch, E->part);
+    if constexpr (sizeof(Real) == 4)
+        hipLaunchKernelGGL((ym::k_ens_plaquette_pp<Real>), dim3(E->h_metro[E->nch]), dim3(512), 0, E->stream, L,
+                           E->d_cd, E->d_metro, E->d_plaq, E->nch, E->part);
+    else
+        hipLaunchKernelGGL((ym::k_ens_plaquette<Real>), dim3(E->h_plaq[E->nch]), dim3(256), 0, E->stream, L,
+                           E->d_cd, E->d_plaq, E->nch, E->part);
     hipLaunchKernelGGL(ym::k_ens_reduce_plaq, dim3(E->nch), dim3(256), 0, E->stream, E->part, E->d_cd, E->d_plaq,
                        E->nch, E->d_ctr, out_plaq);
     const V2 *cur = L;
