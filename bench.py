@@ -89,6 +89,51 @@ def cpu_baseline(seconds_target):
     }
 
 
+def cpu_device_order(seconds_target):
+    """The same sweep as the GPU kernels (device-order replay in C, FP32
+    storage, Philox draws, FP64 accept, 8 hits) on host threads, one 8^4
+    chain per thread: the CPU rate of the algorithm the GPU runs, beside the
+    reference-order baseline."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import numpy as np
+
+    import oracle_lib as O
+
+    L = O.lib()
+    cores = max(1, min(16, os.cpu_count() or 1))
+    n = (8, 8, 8, 8)
+    links_per_sweep = 4 * 8 ** 4
+    vh = L.ymd_slots_per_parity(O.ext(n))
+    done = [0] * cores
+    stop = time.perf_counter() + seconds_target
+
+    def work(i):
+        buf = np.zeros(8 * 18 * vh, dtype=np.float32)
+        key = O.lib().ymo_derive_su3_chain_seed(0x2C02198105010002, i)
+        assert L.ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), 32, key) == 0
+        p, a = C.c_uint64(), C.c_uint64()
+        sweep = 0
+        while time.perf_counter() < stop:
+            prm = O.DevSu3Params(6.0, 0.35, 8, 1, key, sweep)
+            st = L.ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), 32, C.byref(prm),
+                                            C.byref(p), C.byref(a))
+            assert st == 0
+            done[i] += links_per_sweep
+            sweep += 1
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    total = sum(done)
+    return {"value": total / dt, "unit": "link updates/s", "cores": cores, "kind": "device-order replay",
+            "sample": f"{cores} threads x hot-start SU(3) 8^4 FP32 chains, beta=6.0, 8 hits, "
+                      f"{total} link updates in {dt:.1f} s"}
+
+
 def learned_vs_plain(chains=8, sweeps=2500, discard=500, therm=2000, equil=1000, configs=20, epochs=10):
     """Plaquette ESS/s of the learned proposal (one MH-corrected independence
     proposal per link per sweep, BF16 MFMA inference, no Metropolis hits)
@@ -423,10 +468,12 @@ def main():
         log(f"concurrent chains: ESS/s learned {learned['learned']['concurrent']['ess_per_s']:.1f} vs plain "
             f"{learned['plain']['concurrent']['ess_per_s']:.1f} (x{learned['concurrent_ess_per_s_ratio']:.2f})")
 
-    cpu = None
+    cpu = cpu_dev = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cpu = cpu_baseline(args.cpu_seconds)
+        log("cpu device-order replay")
+        cpu_dev = cpu_device_order(min(8.0, args.cpu_seconds))
 
     out = {
         "metric": METRIC,
@@ -472,6 +519,7 @@ def main():
             "mean_launch_us": avg_launch_s * 1e6,
         },
         "cpu_baseline": cpu,
+        "cpu_device_order_replay": cpu_dev,
         "learned_vs_plain": learned,
     }
     print(json.dumps(out), flush=True)
