@@ -705,9 +705,10 @@ int32_t ym_su3_timeslice_a1pp(const ym_lattice *lc, double *plaq_t, double *rect
     const int nsp = l->g.n[0] * l->g.n[1] * l->g.n[2];
     const int nblk = (nsp + 255) / 256;
     const int T = l->g.n[3];
-    int32_t st = ensure_part(l, (size_t)nblk * 2 * T);
+    const size_t npart = (size_t)nblk * 2 * T;
+    int32_t st = ensure_part(l, npart + 2 * (size_t)l->g.volume);
     if (st) return st;
-    ym::launch_a1pp(l->prec, l->links, l->g, l->part, l->stream);
+    ym::launch_a1pp(l->prec, l->links, l->g, l->part + npart, l->part, l->stream);
     double *dout = nullptr;
     if (hipMalloc((void **)&dout, sizeof(double) * 2 * T) != hipSuccess) return YM_ALLOCATION_FAILED;
     ym::launch_sum_partials(l->part, nblk, 2 * T, nblk, dout, l->stream);

@@ -39,6 +39,7 @@ struct ChainDesc {
     uint64_t link_off;  // in link-pair (Vec2) elements
     int nsp;            // spatial sites N0 N1 N2
     int t_off;          // sum of N3 over the chains before this one
+    uint64_t sv_off;    // sum of the volumes of the chains before this one
 };
 
 // Chain of block b: largest c with begin[c] <= b.
@@ -151,21 +152,40 @@ __global__ __launch_bounds__(128) void k_ens_stout(const typename Vec2<Real>::ty
     if (s < g.vh) stout_link<Real>(in + off, out + off, g, rho, p, s, mu);
 }
 
-// A1++ partials: begin[c+1]-begin[c] = N3 * ceil(nsp/256), timeslice major;
-// part[2 b + k], k = 0 plaquettes, 1 rectangles.
+// Per-site A1++ values of every chain (a1pp_slot): both parities of one
+// 256-slot tile per 512-thread block, tiles[c+1]-tiles[c] = ceil(vh/256);
+// chain c's values at sv + 2 sv_off.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_ens_a1pp(const typename Vec2<Real>::type *__restrict__ links,
-                                                  const ChainDesc *__restrict__ cd, const int *__restrict__ begin,
-                                                  int nch, double *__restrict__ part) {
+__global__ __launch_bounds__(512) void k_ens_a1pp_sites(const typename Vec2<Real>::type *__restrict__ links,
+                                                        const ChainDesc *__restrict__ cd,
+                                                        const int *__restrict__ tiles, int nch,
+                                                        double *__restrict__ sv) {
+    const int b = blockIdx.x;
+    const int c = chain_of(tiles, nch, b);
+    const Geom g = cd[c].g;
+    a1pp_slot<Real>(links + cd[c].link_off, g, threadIdx.x >> 8, (b - tiles[c]) * 256 + (threadIdx.x & 255),
+                    sv + 2 * cd[c].sv_off);
+}
+
+// A1++ partials: begin[c+1]-begin[c] = N3 * ceil(nsp/256), timeslice major;
+// part[2 b + k], k = 0 plaquettes, 1 rectangles: block sums of 256
+// consecutive spatial sites of one timeslice.
+__global__ __launch_bounds__(256) void k_ens_a1pp(const double *__restrict__ sv, const ChainDesc *__restrict__ cd,
+                                                  const int *__restrict__ begin, int nch,
+                                                  double *__restrict__ part) {
     __shared__ double red[4];
     const int b = blockIdx.x;
     const int c = chain_of(begin, nch, b);
-    const Geom g = cd[c].g;
-    const int nsb = (cd[c].nsp + 255) / 256;
+    const int nsp = cd[c].nsp;
+    const int nsb = (nsp + 255) / 256;
     const int lb = b - begin[c];
     const int t = lb / nsb, site = (lb - t * nsb) * 256 + threadIdx.x;
     double ps = 0.0, rs = 0.0;
-    if (site < cd[c].nsp) a1pp_site<Real>(links + cd[c].link_off, g, site, t, ps, rs);
+    if (site < nsp) {
+        const size_t i = cd[c].sv_off + (size_t)t * nsp + site;
+        ps = sv[2 * i];
+        rs = sv[2 * i + 1];
+    }
     const double a0 = block_sum<256>(ps, red);
     const double a1 = block_sum<256>(rs, red);
     if (threadIdx.x == 0) {
@@ -251,6 +271,8 @@ struct ym_ensemble {
     ym::ChainDesc *d_cd = nullptr;
     int *d_metro = nullptr, *d_plaq = nullptr, *d_stout = nullptr, *d_a1pp = nullptr;
     double *part = nullptr;  // max(plaq blocks, 2 * a1pp blocks)
+    double *sv = nullptr;    // per-site A1++ values, 2 per site of every chain
+    uint64_t vsum = 0;       // sites of all chains
     unsigned long long *d_acc = nullptr;
     uint64_t *d_ctr = nullptr;  // [0] next sweep index, [1] output slot
     uint64_t sweep = 0;         // host mirror of ctr[0]
@@ -302,7 +324,9 @@ void enqueue_measure(ym_ensemble *E, int levels, double rho, double *out_plaq, d
                                E->d_cd, E->d_stout, E->nch, rho);
             cur = dst;
         }
-        hipLaunchKernelGGL((ym::k_ens_a1pp<Real>), dim3(E->h_a1pp[E->nch]), dim3(256), 0, E->stream, cur, E->d_cd,
+        hipLaunchKernelGGL((ym::k_ens_a1pp_sites<Real>), dim3(E->h_metro[E->nch]), dim3(512), 0, E->stream, cur,
+                           E->d_cd, E->d_metro, E->nch, E->sv);
+        hipLaunchKernelGGL(ym::k_ens_a1pp, dim3(E->h_a1pp[E->nch]), dim3(256), 0, E->stream, E->sv, E->d_cd,
                            E->d_a1pp, E->nch, E->part);
         hipLaunchKernelGGL(ym::k_ens_reduce_a1pp, dim3(E->nch, E->tmax), dim3(256), 0, E->stream, E->part, E->d_cd,
                            E->d_a1pp, lev, E->tsum, ops_len, E->d_ctr, out_ops);
@@ -366,6 +390,8 @@ int32_t ym_ensemble_create(int32_t nchains, const int64_t *extents, const uint64
         d.link_off = E->pairs;
         d.nsp = d.g.n[0] * d.g.n[1] * d.g.n[2];
         d.t_off = E->tsum;
+        d.sv_off = E->vsum;
+        E->vsum += (uint64_t)d.g.volume;
         E->pairs += ym::su3_pairs(d.g.vh);
         E->tsum += d.g.n[3];
         E->tmax = std::max(E->tmax, d.g.n[3]);
@@ -398,6 +424,7 @@ int32_t ym_ensemble_create(int32_t nchains, const int64_t *extents, const uint64
     alloc((void **)&E->d_stout, sizeof(int) * (nchains + 1), YM_ALLOCATION_FAILED);
     alloc((void **)&E->d_a1pp, sizeof(int) * (nchains + 1), YM_ALLOCATION_FAILED);
     alloc((void **)&E->part, sizeof(double) * nparts, YM_ALLOCATION_FAILED);
+    alloc((void **)&E->sv, sizeof(double) * 2 * E->vsum, YM_ALLOCATION_FAILED);
     alloc((void **)&E->d_acc, sizeof(unsigned long long) * nchains, YM_ALLOCATION_FAILED);
     alloc((void **)&E->d_ctr, sizeof(uint64_t) * 2, YM_ALLOCATION_FAILED);
     if (!st && (hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess)) st = YM_HIP_ERROR;
@@ -421,7 +448,7 @@ int32_t ym_ensemble_destroy(ym_ensemble *E) {
     if (!E) return YM_INVALID_ARGUMENT;
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     void *bufs[] = {E->links, E->tmp[0], E->tmp[1], E->d_cd, E->d_metro, E->d_plaq, E->d_stout, E->d_a1pp,
-                    E->part, E->d_acc, E->d_ctr};
+                    E->part, E->sv, E->d_acc, E->d_ctr};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (E->ev0) (void)hipEventDestroy(E->ev0);

@@ -39,7 +39,9 @@ int launch_plaquette(int group, int prec, const void *links, const Geom &g, doub
 int launch_wilson_loops(int group, int prec, const void *links, const Geom &g, int mu, int nu, int rmax, int tmax,
                         double *part, hipStream_t st);
 int launch_polyakov(int group, int prec, const void *links, const Geom &g, int dir, double *part, hipStream_t st);
-int launch_a1pp(int prec, const void *links, const Geom &g, double *part, hipStream_t st);
+// sv: 2 * volume doubles of per-site values; part: 2 * N3 * ceil(L^3 / 256)
+// partials; returns the partials per (timeslice, k).
+int launch_a1pp(int prec, const void *links, const Geom &g, double *sv, double *part, hipStream_t st);
 void launch_stout(int prec, const void *in, void *out, const Geom &g, double rho, hipStream_t st);
 void launch_block2(int prec, const void *in, void *out, const Geom &gf, const Geom &gc, hipStream_t st);
 void launch_path_holonomy(int group, int prec, const void *links, const Geom &g, const int x[4], const int *steps,

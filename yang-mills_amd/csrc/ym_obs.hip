@@ -281,19 +281,27 @@ int launch_polyakov(int group, int prec, const void *links, const Geom &g, int d
 }
 
 // --------------------------------------------------- A1++ timeslice sums
-// Time direction 3.  Per spatial site (x0,x1,x2) at slice t: the three 1x1
-// spatial plaquettes (0,1),(0,2),(1,2) and the six 2x1 rectangles
-// (0,1),(0,2),(1,0),(1,2),(2,0),(2,1) (glueball_operators.rs:24-26, 93-160),
-// (1/3) Re Tr each.  Grid (blocks over L^3, T); part[(t*2 + k)*nblk + blk].
+// Time direction 3.  k_a1pp_sites computes every site's (ps, rs) in slot
+// order (a1pp_slot, both parities of 256 slots per 512-thread block);
+// k_a1pp then sums them per timeslice: blocks over (L^3 / 256, T), each
+// reducing 256 consecutive spatial sites of one timeslice with
+// block_sum<256>; part[(t*2 + k)*nblk + blk].
 template <typename Real>
-__global__ __launch_bounds__(256) void k_a1pp(const typename Vec2<Real>::type *__restrict__ L, Geom g,
-                                              double *__restrict__ part) {
+__global__ __launch_bounds__(512) void k_a1pp_sites(const typename Vec2<Real>::type *__restrict__ L, Geom g,
+                                                    double *__restrict__ sv) {
+    a1pp_slot<Real>(L, g, threadIdx.x >> 8, blockIdx.x * 256 + (threadIdx.x & 255), sv);
+}
+__global__ __launch_bounds__(256) void k_a1pp(const double *__restrict__ sv, Geom g, double *__restrict__ part) {
     __shared__ double red[4];
     const int b = blockIdx.x * 256 + threadIdx.x;
     const int t = blockIdx.y;
     const int nsp = g.n[0] * g.n[1] * g.n[2];
     double ps = 0.0, rs = 0.0;
-    if (b < nsp) a1pp_site<Real>(L, g, b, t, ps, rs);
+    if (b < nsp) {
+        const size_t i = (size_t)t * nsp + b;
+        ps = sv[2 * i];
+        rs = sv[2 * i + 1];
+    }
     const double a0 = block_sum<256>(ps, red);
     const double a1 = block_sum<256>(rs, red);
     if (threadIdx.x == 0) {
@@ -301,11 +309,13 @@ __global__ __launch_bounds__(256) void k_a1pp(const typename Vec2<Real>::type *_
         part[(size_t)(t * 2 + 1) * gridDim.x + blockIdx.x] = a1;
     }
 }
-int launch_a1pp(int prec, const void *links, const Geom &g, double *part, hipStream_t st) {
+int launch_a1pp(int prec, const void *links, const Geom &g, double *sv, double *part, hipStream_t st) {
+    const int nt = (g.vh + 255) / 256;
+    if (prec == 32) hipLaunchKernelGGL((k_a1pp_sites<float>), dim3(nt), dim3(512), 0, st, (const float2 *)links, g, sv);
+    else hipLaunchKernelGGL((k_a1pp_sites<double>), dim3(nt), dim3(512), 0, st, (const double2 *)links, g, sv);
     const int nsp = g.n[0] * g.n[1] * g.n[2];
     dim3 grid((nsp + 255) / 256, g.n[3]);
-    if (prec == 32) hipLaunchKernelGGL((k_a1pp<float>), grid, dim3(256), 0, st, (const float2 *)links, g, part);
-    else hipLaunchKernelGGL((k_a1pp<double>), grid, dim3(256), 0, st, (const double2 *)links, g, part);
+    hipLaunchKernelGGL(k_a1pp, grid, dim3(256), 0, st, (const double *)sv, g, part);
     return (int)grid.x;
 }
 

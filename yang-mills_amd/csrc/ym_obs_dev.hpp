@@ -110,27 +110,90 @@ __device__ __forceinline__ double rect_loop_re(const typename Vec2<Real>::type *
     for (int k = 0; k < t; ++k) { shifted(g, c, n, -1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; h = G::mul(h, G::dag(G::load(L, g, c, n))); }
     return G::retr(h);
 }
-// A1++ sums of the spatial site b (x2 fastest) at timeslice t (time
-// direction 3): the three 1x1 spatial plaquettes (0,1),(0,2),(1,2) into ps
-// and the six 2x1 rectangles (0,1),(0,2),(1,0),(1,2),(2,0),(2,1) into rs,
-// (1/3) Re Tr each (glueball_operators.rs:24-26, 93-160).
+// (1/3) Re Tr of the R x T rectangle of rect_loop_re (SU(3)) with compile-time
+// directions and sides: the product starts from the first link instead of
+// identity x link (the same entries: multiplying by the identity adds only
+// exact zeros) and forms only the diagonal of the last product, with
+// mul_nn's FMA chain; so the value equals rect_loop_re's.
+template <typename Real, int M, int N, int R, int T>
+__device__ __forceinline__ double rect_re3(const typename Vec2<Real>::type *__restrict__ L, const Geom &g,
+                                           const int x0[4]) {
+    using G = G3<Real>;
+    int c[4] = {x0[0], x0[1], x0[2], x0[3]}, y[4];
+    auto step = [&](int d, int s) {
+        shifted(g, c, d, s, y);
+        c[0] = y[0]; c[1] = y[1]; c[2] = y[2]; c[3] = y[3];
+    };
+    M3<Real> h = G::load(L, g, c, M);
+    step(M, +1);
+#pragma unroll
+    for (int k = 1; k < R; ++k) { h = mul_nn(h, G::load(L, g, c, M)); step(M, +1); }
+#pragma unroll
+    for (int k = 0; k < T; ++k) { h = mul_nn(h, G::load(L, g, c, N)); step(N, +1); }
+#pragma unroll
+    for (int k = 0; k < R; ++k) { step(M, -1); h = mul_nn(h, dagger3(G::load(L, g, c, M))); }
+#pragma unroll
+    for (int k = 0; k < T - 1; ++k) { step(N, -1); h = mul_nn(h, dagger3(G::load(L, g, c, N))); }
+    step(N, -1);
+    const M3<Real> Y = dagger3(G::load(L, g, c, N));
+    Real d[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        Real re = YM_RE(h, i, 0) * YM_RE(Y, 0, i);
+        re = fmaR(-YM_IM(h, i, 0), YM_IM(Y, 0, i), re);
+        re = fmaR(YM_RE(h, i, 1), YM_RE(Y, 1, i), re);
+        re = fmaR(-YM_IM(h, i, 1), YM_IM(Y, 1, i), re);
+        re = fmaR(YM_RE(h, i, 2), YM_RE(Y, 2, i), re);
+        re = fmaR(-YM_IM(h, i, 2), YM_IM(Y, 2, i), re);
+        d[i] = re;
+    }
+    return ((double)d[0] + (double)d[1] + (double)d[2]) / 3.0;
+}
+
+// A1++ values of the site x (time direction 3): the three 1x1 spatial
+// plaquettes (0,1),(0,2),(1,2) into ps and the six 2x1 rectangles
+// (0,1),(0,2),(1,0),(1,2),(2,0),(2,1) into rs, (1/3) Re Tr each, added in
+// that order (glueball_operators.rs:24-26, 93-160).  A scheduling barrier
+// after each loop keeps the compiler from hoisting the loads of all nine
+// loops at once.
 template <typename Real>
-__device__ __forceinline__ void a1pp_site(const typename Vec2<Real>::type *__restrict__ L, const Geom &g, int b, int t,
-                                          double &ps, double &rs) {
+__device__ __forceinline__ void a1pp_site_x(const typename Vec2<Real>::type *__restrict__ L, const Geom &g,
+                                            const int x[4], double &ps, double &rs) {
+    ps += rect_re3<Real, 0, 1, 1, 1>(L, g, x);
+    __builtin_amdgcn_sched_barrier(0);
+    ps += rect_re3<Real, 0, 2, 1, 1>(L, g, x);
+    __builtin_amdgcn_sched_barrier(0);
+    ps += rect_re3<Real, 1, 2, 1, 1>(L, g, x);
+    __builtin_amdgcn_sched_barrier(0);
+    rs += rect_re3<Real, 0, 1, 2, 1>(L, g, x);
+    __builtin_amdgcn_sched_barrier(0);
+    rs += rect_re3<Real, 0, 2, 2, 1>(L, g, x);
+    __builtin_amdgcn_sched_barrier(0);
+    rs += rect_re3<Real, 1, 0, 2, 1>(L, g, x);
+    __builtin_amdgcn_sched_barrier(0);
+    rs += rect_re3<Real, 1, 2, 2, 1>(L, g, x);
+    __builtin_amdgcn_sched_barrier(0);
+    rs += rect_re3<Real, 2, 0, 2, 1>(L, g, x);
+    __builtin_amdgcn_sched_barrier(0);
+    rs += rect_re3<Real, 2, 1, 2, 1>(L, g, x);
+}
+
+// Per-site A1++ values (ps, rs) of the site at slot s of parity p into
+// sv[2 i], sv[2 i + 1] with i = t * N0 N1 N2 + (x0 N1 + x1) N2 + x2
+// (timeslice-major spatial order, x2 fastest).  Run over the slots of both
+// parities so that a wavefront's link loads read consecutive slots: a
+// timeslice is a strided subset of the slots, and visiting it directly
+// gathers several times more cache lines per load.
+template <typename Real>
+__device__ __forceinline__ void a1pp_slot(const typename Vec2<Real>::type *__restrict__ L, const Geom &g, int p, int s,
+                                          double *__restrict__ sv) {
     int x[4];
-    x[2] = b % g.n[2];
-    x[1] = (b / g.n[2]) % g.n[1];
-    x[0] = b / (g.n[2] * g.n[1]);
-    x[3] = t;
-    ps += rect_loop_re<Real>(L, g, x, 0, 1, 1, 1);
-    ps += rect_loop_re<Real>(L, g, x, 0, 2, 1, 1);
-    ps += rect_loop_re<Real>(L, g, x, 1, 2, 1, 1);
-    rs += rect_loop_re<Real>(L, g, x, 0, 1, 2, 1);
-    rs += rect_loop_re<Real>(L, g, x, 0, 2, 2, 1);
-    rs += rect_loop_re<Real>(L, g, x, 1, 0, 2, 1);
-    rs += rect_loop_re<Real>(L, g, x, 1, 2, 2, 1);
-    rs += rect_loop_re<Real>(L, g, x, 2, 0, 2, 1);
-    rs += rect_loop_re<Real>(L, g, x, 2, 1, 2, 1);
+    if (s >= g.vh || !site_of(g, p, s, x)) return;
+    double ps = 0.0, rs = 0.0;
+    a1pp_site_x<Real>(L, g, x, ps, rs);
+    const size_t i = (size_t)x[3] * (size_t)(g.n[0] * g.n[1] * g.n[2]) + (size_t)((x[0] * g.n[1] + x[1]) * g.n[2] + x[2]);
+    sv[2 * i] = ps;
+    sv[2 * i + 1] = rs;
 }
 
 // ------------------------------------------------------ spatial stout
