@@ -41,6 +41,24 @@ def test_chains_match_single_lattices_bitwise(ym, prec):
             assert np.array_equal(ops[lev, :, 0], pl) and np.array_equal(ops[lev, :, 1], rc)
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_wave_specialised_batch_matches_single_lattices(ym, prec):
+    """From 512 tiles of 256 links per sub-pass the batched sweep runs the
+    wave-specialised kernel (staples staged in LDS, tiles of all chains in one
+    list); every chain still equals a single lattice swept alone."""
+    geoms = [(12, 12, 12, 24)] * 6 + [(8, 8, 8, 16)] * 4  # 6 x 81 + 4 x 16 = 550 tiles
+    ks = keys(ym, len(geoms))
+    ens = ym.Ensemble(geoms, ks, prec)
+    res = ens.run_cycles(6.0, 0.35, hits=8, sweeps_per_cycle=2, n_cycles=1, stout_levels=0, rho=0.1,
+                         use_graph=False)
+    for c, (g, k) in enumerate(zip(geoms, ks)):
+        lat = ym.Su3Lattice4(g, prec)
+        _, acc = lat.metropolis_sweeps(6.0, 0.35, k, 0, 2, 8)
+        assert int(res.accepts[c]) == acc, c
+        assert np.array_equal(ens.download_chain(c), lat.download()), c
+        assert res.plaquette[0, c] == lat.average_plaquette()
+
+
 def test_graph_replays_match_eager(ym):
     ks = keys(ym, len(GEOMS))
     runs = []

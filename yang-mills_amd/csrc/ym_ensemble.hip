@@ -81,6 +81,71 @@ __global__ __launch_bounds__(256) void k_ens_metropolis(typename Vec2<Real>::typ
     if (threadIdx.x == 0 && tot) atomicAdd(acc + c, tot);
 }
 
+// Wave-specialised form of k_ens_metropolis for batches of at least 512
+// tiles (the single-lattice k_su3_metropolis_ws over the concatenated tile
+// list of all chains): one 512-thread block per CU walks a sequence of
+// tiles; waves 0-3 stage the staples and links of tile i+1 in LDS while
+// waves 4-7 run the hits of tile i.  The XCDs take contiguous ranges of the
+// tile list, parity-1 sub-passes walk them backwards.  Per link the
+// arithmetic is k_ens_metropolis's; accepted proposals are added per chain
+// by each consumer wave.
+template <typename Real, int MU>
+__global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::type *__restrict__ links,
+                                                           const ChainDesc *__restrict__ cd,
+                                                           const int *__restrict__ begin, int nch, int ntiles,
+                                                           int p, double nbt, double angle, int hits,
+                                                           const uint64_t *__restrict__ ctr, int sweep_k, int reunit,
+                                                           unsigned long long *__restrict__ acc) {
+    __shared__ LdsMat<Real> abuf[2], ubuf[2];
+    const bool producer = threadIdx.x < 256;
+    const int lane = threadIdx.x & 255;
+    const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+    const int nbx = ((int)gridDim.x - 1 - xcd) / 8 + 1;
+    const int t0 = (int)(((int64_t)ntiles * xcd) / 8), t1 = (int)(((int64_t)ntiles * (xcd + 1)) / 8);
+    const int nsteps = (t1 - t0 - j + nbx - 1) / nbx;
+    const bool rev = (p & 1) != 0;
+    const uint64_t sweep = ctr[0] + (uint64_t)sweep_k;
+    auto tile_at = [&](int i) { return rev ? t1 - 1 - (j + i * nbx) : t0 + j + i * nbx; };
+    auto produce = [&](int i) {
+        const int T = tile_at(i);
+        const int c = chain_of(begin, nch, T);
+        const Geom g = cd[c].g;
+        const typename Vec2<Real>::type *L = links + cd[c].link_off;
+        const int s = tile_slot(g, T - begin[c], lane);
+        int x[4];
+        if (s < g.vh && site_of(g, p, s, x)) {
+            M3<Real> A;
+            su3_staple_at<Real, MU>(L, g, p, s, x, A);
+            const M3<Real> U = load3<Real>(L, g.vh, MU, p, s);
+            abuf[i & 1].put(lane, A);
+            ubuf[i & 1].put(lane, U);
+        }
+    };
+    if (producer && nsteps > 0) produce(0);
+    __syncthreads();
+    for (int i = 0; i < nsteps; ++i) {
+        if (producer) {
+            if (i + 1 < nsteps) produce(i + 1);
+        } else {
+            const int T = tile_at(i);
+            const int c = chain_of(begin, nch, T);
+            const Geom g = cd[c].g;
+            typename Vec2<Real>::type *L = links + cd[c].link_off;
+            const int s = tile_slot(g, T - begin[c], lane);
+            unsigned long long nacc = 0;
+            int x[4];
+            if (s < g.vh && site_of(g, p, s, x)) {
+                const M3<Real> A = abuf[i & 1].get(lane), U = ubuf[i & 1].get(lane);
+                nacc = su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, cd[c].key, sweep, reunit);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) nacc += __shfl_down(nacc, off, 64);
+            if ((lane & 63) == 0 && nacc) atomicAdd(acc + c, nacc);
+        }
+        __syncthreads();
+    }
+}
+
 // Plaquette partial per block: begin[c+1]-begin[c] = 2 * tiles(c), parity
 // major.
 template <typename Real>
@@ -289,10 +354,20 @@ void enqueue_sweeps(ym_ensemble *E, double beta, double angle, int hits, int nsw
     using V2 = typename ym::Vec2<Real>::type;
     const double nbt = -(beta / 3.0);
     V2 *L = (V2 *)E->links;
-    const dim3 grid(E->h_metro[E->nch]);
+    const int ntiles = E->h_metro[E->nch];
+    const dim3 grid(ntiles);
     for (int k = 0; k < nsweeps; ++k)
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p) {
+                if (ntiles >= 512) {
+                    switch (mu) {
+                    case 0: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 0>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    case 1: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 1>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    case 2: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 2>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    default: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 3>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    }
+                    continue;
+                }
                 switch (mu) {
                 case 0: hipLaunchKernelGGL((ym::k_ens_metropolis<Real, 0>), grid, dim3(256), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
                 case 1: hipLaunchKernelGGL((ym::k_ens_metropolis<Real, 1>), grid, dim3(256), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;

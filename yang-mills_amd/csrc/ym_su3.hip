@@ -56,47 +56,6 @@ __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::typ
 // larger than the 256 MB cache: the same direction every time evicts each
 // line just before it is needed again).  Tiles are independent, so the
 // direction changes timing only.
-// One 256-link staging buffer in LDS, 16 bytes per lane per access (the
-// pairing of the FP32 tile layout; FP64 entries are double2 already), so a
-// link is 5 (FP32) or 9 (FP64) b128/b64 LDS accesses instead of 18.
-template <typename Real> struct LdsMat;
-template <> struct LdsMat<float> {
-    float4 q[4][256];
-    float2 r[256];
-    __device__ __forceinline__ void put(int l, const M3<float> &m) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) q[k][l] = make_float4(m.v[4 * k], m.v[4 * k + 1], m.v[4 * k + 2], m.v[4 * k + 3]);
-        r[l] = make_float2(m.v[16], m.v[17]);
-    }
-    __device__ __forceinline__ M3<float> get(int l) const {
-        M3<float> m;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float4 t = q[k][l];
-            m.v[4 * k] = t.x; m.v[4 * k + 1] = t.y; m.v[4 * k + 2] = t.z; m.v[4 * k + 3] = t.w;
-        }
-        const float2 t = r[l];
-        m.v[16] = t.x; m.v[17] = t.y;
-        return m;
-    }
-};
-template <> struct LdsMat<double> {
-    double2 e[9][256];
-    __device__ __forceinline__ void put(int l, const M3<double> &m) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) e[k][l] = make_double2(m.v[2 * k], m.v[2 * k + 1]);
-    }
-    __device__ __forceinline__ M3<double> get(int l) const {
-        M3<double> m;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const double2 t = e[k][l];
-            m.v[2 * k] = t.x; m.v[2 * k + 1] = t.y;
-        }
-        return m;
-    }
-};
-
 template <typename Real, int MU>
 __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::type *__restrict__ L, Geom g, int p,
                                                            double nbt, double angle, int hits, uint64_t key,
