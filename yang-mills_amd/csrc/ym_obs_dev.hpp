@@ -98,23 +98,12 @@ __device__ __forceinline__ double plaquette_site(const typename G::Store *__rest
     return acc;
 }
 
-template <typename Real>
-__device__ __forceinline__ double rect_loop_re(const typename Vec2<Real>::type *__restrict__ L, const Geom &g,
-                                               const int x0[4], int m, int n, int r, int t) {
-    using G = G3<Real>;
-    typename G::Mat h = G::id();
-    int c[4] = {x0[0], x0[1], x0[2], x0[3]}, y[4];
-    for (int k = 0; k < r; ++k) { h = G::mul(h, G::load(L, g, c, m)); shifted(g, c, m, +1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; }
-    for (int k = 0; k < t; ++k) { h = G::mul(h, G::load(L, g, c, n)); shifted(g, c, n, +1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; }
-    for (int k = 0; k < r; ++k) { shifted(g, c, m, -1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; h = G::mul(h, G::dag(G::load(L, g, c, m))); }
-    for (int k = 0; k < t; ++k) { shifted(g, c, n, -1, y); c[0]=y[0]; c[1]=y[1]; c[2]=y[2]; c[3]=y[3]; h = G::mul(h, G::dag(G::load(L, g, c, n))); }
-    return G::retr(h);
-}
-// (1/3) Re Tr of the R x T rectangle of rect_loop_re (SU(3)) with compile-time
-// directions and sides: the product starts from the first link instead of
-// identity x link (the same entries: multiplying by the identity adds only
-// exact zeros) and forms only the diagonal of the last product, with
-// mul_nn's FMA chain; so the value equals rect_loop_re's.
+// (1/3) Re Tr of the SU(3) rectangle from x: R links along M, T along N,
+// R back along M (daggered), T back along N (daggered), multiplied left to
+// right with mul_nn's FMA chain in the storage precision
+// (path_holonomy.rs:219-260).  The product starts from the first link (an
+// identity start would add only exact zeros) and only the diagonal of the
+// last product is formed.
 template <typename Real, int M, int N, int R, int T>
 __device__ __forceinline__ double rect_re3(const typename Vec2<Real>::type *__restrict__ L, const Geom &g,
                                            const int x0[4]) {
