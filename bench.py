@@ -442,6 +442,13 @@ def main():
     t0 = time.perf_counter()
     prop, acc, total_ms = lat.metropolis_sweeps_timed(args.beta, angle, key, args.warmup, args.steps, args.hits)
     wall = time.perf_counter() - t0
+    # four more regions of the same length (the sweeps that follow), so the
+    # spread between repeats is reported beside the first region's value
+    repeats_ms = [total_ms / args.steps]
+    if not args.traffic_calibration:
+        for r in range(4):
+            repeats_ms.append(lat.metropolis_sweeps_timed(args.beta, angle, key, args.warmup + (r + 1) * args.steps,
+                                                          args.steps, args.hits)[2] / args.steps)
     if args.traffic_calibration:
         log(f"calibration: device copy of {lat.device_bytes()} bytes")
         lat.clone()
@@ -452,7 +459,8 @@ def main():
     avg_launch_s = total_ms / launches / 1e3
     bytes_per_launch = (V // 2) * BYTES_PER_UPDATE_FP32
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    kernel = "k_su3_metropolis_ws" if (V // 2 + 255) // 256 >= 256 else "k_su3_metropolis"
+    per_link = best[0].endswith("/per-link") or (V // 2 + 255) // 256 < 256
+    kernel = "k_su3_metropolis" if per_link else "k_su3_metropolis_ws"
     log(f"{value:.4e} link updates/s, {kernel} mean {avg_launch_s * 1e6:.1f} us, "
         f"{achieved:.0f} GB/s algorithmic")
 
@@ -505,6 +513,8 @@ def main():
             "unitarity_residual_fp32_storage": unit_res,
             "determinant_residual_fp32_storage": det_res,
             "device_time_ms_per_step": total_ms / args.steps,
+            "repeat_device_ms_per_step": repeats_ms,
+            "median_repeat_link_updates_per_s": 4 * V / (sorted(repeats_ms)[len(repeats_ms) // 2] * 1e-3),
             "autotuned_tile_mapping": {"mode": best[0], "strip": best[1]},
         },
         "roofline": {
