@@ -13,6 +13,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 GEOMS = [(4, 4, 4, 6), (6, 6, 6, 4), (4, 4, 4, 8), (6, 6, 6, 6)]
+# 6 x 81 + 4 x 16 = 550 tiles of 256 links: the wave-specialised batched sweep
+BIG = [(12, 12, 12, 24)] * 6 + [(8, 8, 8, 16)] * 4
 
 
 def keys(ym, n):
@@ -46,7 +48,7 @@ def test_wave_specialised_batch_matches_single_lattices(ym, prec):
     """From 512 tiles of 256 links per sub-pass the batched sweep runs the
     wave-specialised kernel (staples staged in LDS, tiles of all chains in one
     list); every chain still equals a single lattice swept alone."""
-    geoms = [(12, 12, 12, 24)] * 6 + [(8, 8, 8, 16)] * 4  # 6 x 81 + 4 x 16 = 550 tiles
+    geoms = BIG
     ks = keys(ym, len(geoms))
     ens = ym.Ensemble(geoms, ks, prec)
     res = ens.run_cycles(6.0, 0.35, hits=8, sweeps_per_cycle=2, n_cycles=1, stout_levels=0, rho=0.1,
@@ -59,26 +61,27 @@ def test_wave_specialised_batch_matches_single_lattices(ym, prec):
         assert res.plaquette[0, c] == lat.average_plaquette()
 
 
-def test_graph_replays_match_eager(ym):
-    ks = keys(ym, len(GEOMS))
+@pytest.mark.parametrize("geoms", [GEOMS, BIG], ids=["small", "550-tiles"])
+def test_graph_replays_match_eager(ym, geoms):
+    ks = keys(ym, len(geoms))
     runs = []
     for use_graph in (True, False):
-        ens = ym.Ensemble(GEOMS, ks, 32)
+        ens = ym.Ensemble(geoms, ks, 32)
         a = ens.run_cycles(5.7, 0.35, sweeps_per_cycle=2, n_cycles=3, stout_levels=2, use_graph=use_graph)
         # eager work and synchronisation between two graph calls
         mid = ens.sweeps(5.7, 0.35, 1)
         b = ens.run_cycles(5.7, 0.35, sweeps_per_cycle=1, n_cycles=2, stout_levels=1, use_graph=use_graph)
-        links = [ens.download_chain(c) for c in range(len(GEOMS))]
+        links = [ens.download_chain(c) for c in range(len(geoms))]
         runs.append((a, mid, b, links, ens.sweep_index()))
         ens.close()
     (a1, m1, b1, l1, s1), (a2, m2, b2, l2, s2) = runs
     assert s1 == s2 == 2 * 3 + 1 + 2
     assert np.array_equal(a1.plaquette, a2.plaquette) and np.array_equal(b1.plaquette, b2.plaquette)
     for k in range(3):
-        for c in range(len(GEOMS)):
+        for c in range(len(geoms)):
             assert np.array_equal(a1.ops[k][c], a2.ops[k][c])
     for k in range(2):
-        for c in range(len(GEOMS)):
+        for c in range(len(geoms)):
             assert np.array_equal(b1.ops[k][c], b2.ops[k][c])
     assert np.array_equal(a1.accepts, a2.accepts) and np.array_equal(m1, m2) and np.array_equal(b1.accepts, b2.accepts)
     for x, y in zip(l1, l2):
