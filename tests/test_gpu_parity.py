@@ -227,6 +227,31 @@ def test_su3_plaquette_and_action_fp64(ym):
     assert_rel(lat.wilson_action(6.0), ref.value, scale=abs(ref.value))
 
 
+@pytest.mark.parametrize("group,n", [(3, (16, 16, 16, 32)), (3, (8, 8, 32, 16)), (2, (32, 32, 32, 32))])
+def test_fp32_plaquette_tile_order_vs_fp64(ym, group, n):
+    """FP32 links on lattices whose x0-slices are a multiple of 8 tiles, so the
+    plaquette kernel deals x1-bands to the XCDs (and A1++ runs its slot-order
+    site pass): the results equal those of an FP64 lattice holding the same
+    values (the FP64 kernels are pinned against the reference restatement)
+    to FP32 product rounding; a missing or doubled tile would move the
+    plaquette by ~1/tiles.  (The reference restatement itself rejects FP32
+    links: its products check unitarity to 5e-12.)"""
+    cls = ym.Su3Lattice4 if group == 3 else ym.Lattice4
+    lat = cls(n, 32)
+    lat.hot_start(77)
+    if group == 3:
+        lat.metropolis_sweeps(6.0, 0.35, 78, 0, 1, 8)
+    else:
+        lat.heatbath_sweeps(2.3, 78, 0, 1)
+    twin = cls(n, 64)
+    twin.upload(lat.download())
+    assert abs(lat.average_plaquette() - twin.average_plaquette()) <= 1e-6
+    if group == 3 and n[0] == n[1] == n[2]:
+        pl, rc = lat.timeslice_a1pp()
+        pl64, rc64 = twin.timeslice_a1pp()
+        assert np.max(np.abs(pl - pl64)) <= 1e-6 and np.max(np.abs(rc - rc64)) <= 1e-6
+
+
 def test_su2_plaquette_and_staple_fp64(ym):
     n = (4, 4, 2, 6)
     lat, aos = hot_fp64(ym, n, 4, group=2)
