@@ -83,6 +83,7 @@ def cpu_baseline(seconds_target):
         "value": total / dt,
         "unit": "link updates/s",
         "cores": cores,
+        "per_core": total / dt / cores,
         "kind": "port",
         "sample": (f"{cores} threads x {sweeps} reference-order sweeps of SU(3) 6^4 at beta=6.0 "
                    f"(8 checked-product Metropolis proposals per link, SplitMix64), {total} link updates in {dt:.1f} s"),
@@ -129,7 +130,8 @@ def cpu_device_order(seconds_target):
         t.join()
     dt = time.perf_counter() - t0
     total = sum(done)
-    return {"value": total / dt, "unit": "link updates/s", "cores": cores, "kind": "device-order replay",
+    return {"value": total / dt, "unit": "link updates/s", "cores": cores, "per_core": total / dt / cores,
+            "kind": "device-order replay",
             "sample": f"{cores} threads x hot-start SU(3) 8^4 FP32 chains, beta=6.0, 8 hits, "
                       f"{total} link updates in {dt:.1f} s"}
 
