@@ -92,7 +92,8 @@ __global__ __launch_bounds__(256) void k_ens_metropolis(typename Vec2<Real>::typ
 template <typename Real, int MU>
 __global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::type *__restrict__ links,
                                                            const ChainDesc *__restrict__ cd,
-                                                           const int *__restrict__ begin, int nch, int ntiles,
+                                                           const int *__restrict__ begin,
+                                                           const int *__restrict__ tile_chain, int ntiles,
                                                            int p, double nbt, double angle, int hits,
                                                            const uint64_t *__restrict__ ctr, int sweep_k, int reunit,
                                                            unsigned long long *__restrict__ acc) {
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::
     auto tile_at = [&](int i) { return rev ? t1 - 1 - (j + i * nbx) : t0 + j + i * nbx; };
     auto produce = [&](int i) {
         const int T = tile_at(i);
-        const int c = chain_of(begin, nch, T);
+        const int c = tile_chain[T];
         const Geom g = cd[c].g;
         const typename Vec2<Real>::type *L = links + cd[c].link_off;
         const int s = tile_slot(g, T - begin[c], lane);
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::
             if (i + 1 < nsteps) produce(i + 1);
         } else {
             const int T = tile_at(i);
-            const int c = chain_of(begin, nch, T);
+            const int c = tile_chain[T];
             const Geom g = cd[c].g;
             typename Vec2<Real>::type *L = links + cd[c].link_off;
             const int s = tile_slot(g, T - begin[c], lane);
@@ -329,12 +330,14 @@ struct ym_ensemble {
     int nch = 0, prec = 0;
     std::vector<ym::ChainDesc> h_cd;
     std::vector<int> h_metro, h_plaq, h_stout, h_a1pp;  // prefix tables, nch + 1 entries
+    std::vector<int> h_tile_chain;                       // chain of every Metropolis tile
     int tsum = 0, tmax = 0;
     uint64_t pairs = 0;      // link pairs of all chains
     void *links = nullptr;   // pairs elements of Vec2<Real>
     void *tmp[2] = {nullptr, nullptr};
     ym::ChainDesc *d_cd = nullptr;
     int *d_metro = nullptr, *d_plaq = nullptr, *d_stout = nullptr, *d_a1pp = nullptr;
+    int *d_tile_chain = nullptr;  // chain of every Metropolis tile
     double *part = nullptr;  // max(plaq blocks, 2 * a1pp blocks)
     double *sv = nullptr;    // per-site A1++ values, 2 per site of every chain
     uint64_t vsum = 0;       // sites of all chains
@@ -361,10 +364,10 @@ void enqueue_sweeps(ym_ensemble *E, double beta, double angle, int hits, int nsw
             for (int p = 0; p < 2; ++p) {
                 if (ntiles >= 512) {
                     switch (mu) {
-                    case 0: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 0>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
-                    case 1: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 1>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
-                    case 2: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 2>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
-                    default: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 3>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->nch, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    case 0: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 0>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    case 1: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 1>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    case 2: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 2>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    default: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 3>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
                     }
                     continue;
                 }
@@ -495,6 +498,7 @@ int32_t ym_ensemble_create(int32_t nchains, const int64_t *extents, const uint64
     alloc(&E->tmp[1], link_bytes, YM_LATTICE_TOO_LARGE);
     alloc((void **)&E->d_cd, sizeof(ym::ChainDesc) * nchains, YM_ALLOCATION_FAILED);
     alloc((void **)&E->d_metro, sizeof(int) * (nchains + 1), YM_ALLOCATION_FAILED);
+    alloc((void **)&E->d_tile_chain, sizeof(int) * std::max(1, E->h_metro[nchains]), YM_ALLOCATION_FAILED);
     alloc((void **)&E->d_plaq, sizeof(int) * (nchains + 1), YM_ALLOCATION_FAILED);
     alloc((void **)&E->d_stout, sizeof(int) * (nchains + 1), YM_ALLOCATION_FAILED);
     alloc((void **)&E->d_a1pp, sizeof(int) * (nchains + 1), YM_ALLOCATION_FAILED);
@@ -510,6 +514,12 @@ int32_t ym_ensemble_create(int32_t nchains, const int64_t *extents, const uint64
     for (int k = 0; k < 4 && !st; ++k)
         st = hst(hipMemcpyAsync(dtabs[k], tabs[k]->data(), sizeof(int) * (nchains + 1), hipMemcpyHostToDevice,
                                 E->stream));
+    E->h_tile_chain.assign(std::max(1, E->h_metro[nchains]), 0);
+    for (int c = 0; c < nchains; ++c)
+        for (int t = E->h_metro[c]; t < E->h_metro[c + 1]; ++t) E->h_tile_chain[t] = c;
+    if (!st)
+        st = hst(hipMemcpyAsync(E->d_tile_chain, E->h_tile_chain.data(), sizeof(int) * E->h_tile_chain.size(),
+                                hipMemcpyHostToDevice, E->stream));
     if (!st) st = ym_ensemble_set_identity(E);
     if (st) {
         ym_ensemble_destroy(E);
@@ -522,7 +532,8 @@ int32_t ym_ensemble_create(int32_t nchains, const int64_t *extents, const uint64
 int32_t ym_ensemble_destroy(ym_ensemble *E) {
     if (!E) return YM_INVALID_ARGUMENT;
     if (E->stream) (void)hipStreamSynchronize(E->stream);
-    void *bufs[] = {E->links, E->tmp[0], E->tmp[1], E->d_cd, E->d_metro, E->d_plaq, E->d_stout, E->d_a1pp,
+    void *bufs[] = {E->links, E->tmp[0], E->tmp[1], E->d_cd, E->d_metro, E->d_tile_chain, E->d_plaq, E->d_stout,
+                    E->d_a1pp,
                     E->part, E->sv, E->d_acc, E->d_ctr};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
