@@ -176,11 +176,12 @@ template <typename Real>
 __global__ __launch_bounds__(512) void k_ens_plaquette_pp(const typename Vec2<Real>::type *__restrict__ links,
                                                           const ChainDesc *__restrict__ cd,
                                                           const int *__restrict__ tiles,
-                                                          const int *__restrict__ begin, int nch,
+                                                          const int *__restrict__ tile_chain,
+                                                          const int *__restrict__ begin,
                                                           double *__restrict__ part) {
     __shared__ double red[8];
     const int b = blockIdx.x;
-    const int c = chain_of(tiles, nch, b);
+    const int c = tile_chain[b];
     const Geom g = cd[c].g;
     const int nt = (g.vh + 255) / 256;
     const int t = b - tiles[c];
@@ -224,10 +225,11 @@ __global__ __launch_bounds__(128) void k_ens_stout(const typename Vec2<Real>::ty
 template <typename Real>
 __global__ __launch_bounds__(512) void k_ens_a1pp_sites(const typename Vec2<Real>::type *__restrict__ links,
                                                         const ChainDesc *__restrict__ cd,
-                                                        const int *__restrict__ tiles, int nch,
+                                                        const int *__restrict__ tiles,
+                                                        const int *__restrict__ tile_chain,
                                                         double *__restrict__ sv) {
     const int b = blockIdx.x;
-    const int c = chain_of(tiles, nch, b);
+    const int c = tile_chain[b];
     const Geom g = cd[c].g;
     a1pp_slot<Real>(links + cd[c].link_off, g, threadIdx.x >> 8, (b - tiles[c]) * 256 + (threadIdx.x & 255),
                     sv + 2 * cd[c].sv_off);
@@ -388,7 +390,7 @@ void enqueue_measure(ym_ensemble *E, int levels, double rho, double *out_plaq, d
     const uint64_t ops_len = (uint64_t)(levels + 1) * E->tsum * 2;
     if constexpr (sizeof(Real) == 4)
         hipLaunchKernelGGL((ym::k_ens_plaquette_pp<Real>), dim3(E->h_metro[E->nch]), dim3(512), 0, E->stream, L,
-                           E->d_cd, E->d_metro, E->d_plaq, E->nch, E->part);
+                           E->d_cd, E->d_metro, E->d_tile_chain, E->d_plaq, E->part);
     else
         hipLaunchKernelGGL((ym::k_ens_plaquette<Real>), dim3(E->h_plaq[E->nch]), dim3(256), 0, E->stream, L,
                            E->d_cd, E->d_plaq, E->nch, E->part);
@@ -403,7 +405,7 @@ void enqueue_measure(ym_ensemble *E, int levels, double rho, double *out_plaq, d
             cur = dst;
         }
         hipLaunchKernelGGL((ym::k_ens_a1pp_sites<Real>), dim3(E->h_metro[E->nch]), dim3(512), 0, E->stream, cur,
-                           E->d_cd, E->d_metro, E->nch, E->sv);
+                           E->d_cd, E->d_metro, E->d_tile_chain, E->sv);
         hipLaunchKernelGGL(ym::k_ens_a1pp, dim3(E->h_a1pp[E->nch]), dim3(256), 0, E->stream, E->sv, E->d_cd,
                            E->d_a1pp, E->nch, E->part);
         hipLaunchKernelGGL(ym::k_ens_reduce_a1pp, dim3(E->nch, E->tmax), dim3(256), 0, E->stream, E->part, E->d_cd,
