@@ -211,6 +211,106 @@ static inline int site_of(const geom *g, int p, int64_t s, int64_t x[4]) {
     return x[3] < g->n[3];
 }
 
+/* ------------------------------------------------ SU(2) quaternions */
+static inline double qnorm2(const double *a) {
+    double n = a[0] * a[0];
+    n = fma(a[1], a[1], n); n = fma(a[2], a[2], n); n = fma(a[3], a[3], n);
+    return n;
+}
+/* product in the reference convention (lib.rs:212-224) as an FMA chain */
+static inline void qmul64(const double *a, const double *b, double *r) {
+    double r0 = a[0] * b[0];
+    r0 = fma(-a[1], b[1], r0); r0 = fma(-a[2], b[2], r0); r0 = fma(-a[3], b[3], r0);
+    double r1 = a[0] * b[1];
+    r1 = fma(a[1], b[0], r1); r1 = fma(a[2], b[3], r1); r1 = fma(-a[3], b[2], r1);
+    double r2 = a[0] * b[2];
+    r2 = fma(-a[1], b[3], r2); r2 = fma(a[2], b[0], r2); r2 = fma(a[3], b[1], r2);
+    double r3 = a[0] * b[3];
+    r3 = fma(a[1], b[2], r3); r3 = fma(-a[2], b[1], r3); r3 = fma(a[3], b[0], r3);
+    r[0] = r0; r[1] = r1; r[2] = r2; r[3] = r3;
+}
+static inline void qnormalize(double *a) {
+    double inv = 1.0 / sqrt(qnorm2(a));
+    for (int c = 0; c < 4; ++c) a[c] *= inv;
+}
+
+/* Heat-bath update of one SU(2) link from its FP64 staple a: the device form
+ * of sample_local_heatbath + heatbath_link (su2_ensemble.rs:220-281).
+ *   alpha = beta |a|; per link 1/alpha and tm = 1 - e^{-2 alpha};
+ *   attempt k draws Philox(site, mu | HB_ATTEMPT << 2 | k << 5, sweep) -> u1, u2;
+ *   r0 = 2 u1 - 1 when alpha <= 1e-300 (the reference's zero limit), else
+ *   r0 = 1 + L / alpha with L = ln(1 - u1 tm): log1p(-u1 tm) when
+ *   1/alpha > 2 (small alpha, where 1 - u1 tm is close to 1), log of the
+ *   fused 1 - u1 tm otherwise; r0 clamped to [-1, 1];
+ *   accepted iff u2^2 <= 1 - r0^2 (the reference's u2 <= sqrt(1 - r0^2));
+ *   direction z = 2 u3 - 1, phi = 2 pi u4 from Philox(site, mu | HB_DIRECTION << 2, sweep);
+ *   R = (r0, sqrt(1 - r0^2) n), U = R (a / |a|)^dagger (U = R when |a| = 0), normalised.
+ * Returns 1 with the link in out, or 0 (out untouched) when all max_attempts
+ * attempts were rejected; *attempts = attempts used. */
+int ymd_su2_heatbath_link(const double a[4], double beta, uint64_t key, uint32_t site, int mu, uint64_t sweep,
+                          int64_t max_attempts, double out[4], int64_t *attempts) {
+    double norm = sqrt(qnorm2(a));
+    double alpha = beta * norm;
+    double inv_alpha = 0.0, tm = 0.0;
+    if (alpha > 1.0e-300) {
+        inv_alpha = 1.0 / alpha;
+        tm = -ymd_expm1(-2.0 * alpha);
+    }
+    double r0 = 0.0;
+    int64_t att = 0;
+    int ok = 0;
+    while (att < max_attempts) {
+        uint32_t o[4];
+        philox_draw(key, site, word1(mu, KIND_HB_ATTEMPT, (uint32_t)att), sweep, o);
+        att += 1;
+        double u1 = unit53(o[0], o[1]), u2 = unit53(o[2], o[3]);
+        if (inv_alpha == 0.0) {
+            r0 = 2.0 * u1 - 1.0;
+        } else {
+            double L = inv_alpha > 2.0 ? ymd_log1p(-u1 * tm) : ymd_log(fma(-u1, tm, 1.0));
+            r0 = fma(L, inv_alpha, 1.0);
+            if (r0 < -1.0) r0 = -1.0;
+            if (r0 > 1.0) r0 = 1.0;
+        }
+        if (u2 * u2 <= fma(-r0, r0, 1.0)) { ok = 1; break; }
+    }
+    *attempts = att;
+    if (!ok) return 0;
+    double radial = sqrt(fmax(fma(-r0, r0, 1.0), 0.0));
+    uint32_t o[4];
+    philox_draw(key, site, word1(mu, KIND_HB_DIRECTION, 0), sweep, o);
+    double z = 2.0 * unit53(o[0], o[1]) - 1.0;
+    double sn, cs;
+    ymd_sincos_turns(unit53(o[2], o[3]), &sn, &cs);
+    double planar = sqrt(fmax(fma(-z, z, 1.0), 0.0));
+    double rp = radial * planar;
+    double R[4] = {r0, rp * cs, rp * sn, radial * z};
+    if (norm == 0.0) {
+        memcpy(out, R, 4 * sizeof(double));
+    } else {
+        double inv_norm = 1.0 / norm;
+        double ad[4] = {a[0] * inv_norm, -(a[1] * inv_norm), -(a[2] * inv_norm), -(a[3] * inv_norm)};
+        qmul64(R, ad, out);
+    }
+    qnormalize(out);
+    return 1;
+}
+
+/* n draws of ymd_su2_heatbath_link for one staple, draw i at site i (density
+ * tests); out: n x 4, attempts: n (0 links of exhausted draws). */
+int ymd_su2_heatbath_samples(const double a[4], double beta, uint64_t key, int mu, uint64_t sweep,
+                             int64_t max_attempts, int64_t n, double *out, int64_t *attempts) {
+    int status = YMO_OK;
+    for (int64_t i = 0; i < n; ++i) {
+        double *u = out + 4 * i;
+        if (!ymd_su2_heatbath_link(a, beta, key, (uint32_t)i, mu, sweep, max_attempts, u, &attempts[i])) {
+            u[0] = u[1] = u[2] = u[3] = 0.0;
+            status = YMO_HEATBATH_ATTEMPT_LIMIT;
+        }
+    }
+    return status;
+}
+
 #define REAL float
 #define RSUF f32
 #define RFMA fmaf

@@ -19,7 +19,9 @@
  *    with R the identity rotation (c, s) = (1, 0) when rejected, FP64
  *    Gram-Schmidt reunitarisation before the store;
  *  - SU(2): the reference heat-bath algorithm (truncated-exponential inverse
- *    CDF + sqrt(1-r0^2) rejection) and over-relaxation on FP32/FP64 storage;
+ *    CDF + sqrt(1-r0^2) rejection, written with 1/alpha and 1/|V| formed once
+ *    per link and no division or square root per attempt) and
+ *    over-relaxation on FP32/FP64 storage;
  *  - transcendental functions from named published methods (Cody-Waite
  *    reduction + Taylor for exp; atanh series for log; Taylor sin/cos on a
  *    quarter-turn reduced argument), evaluated with explicit FMA only.
@@ -56,4 +58,9 @@ int ymd_su3_staple(const int64_t n[4], const void *dev, int precision, const int
 int ymd_su2_heatbath_sweep(const int64_t n[4], void *dev, int precision, double beta, uint64_t key, uint64_t sweep,
                            int64_t max_attempts, uint64_t *proposals, uint64_t *updates);
 int ymd_su2_overrelax_sweep(const int64_t n[4], void *dev, int precision);
+/* one heat-bath draw from an FP64 staple (the per-link step of the sweep) */
+int ymd_su2_heatbath_link(const double a[4], double beta, uint64_t key, uint32_t site, int mu, uint64_t sweep,
+                          int64_t max_attempts, double out[4], int64_t *attempts);
+int ymd_su2_heatbath_samples(const double a[4], double beta, uint64_t key, int mu, uint64_t sweep,
+                             int64_t max_attempts, int64_t n, double *out, int64_t *attempts);
 #endif

@@ -438,29 +438,6 @@ static void F(su2_staple)(const REAL *L, const geom *g, const int64_t x[4], int 
     }
 }
 
-#ifndef YM_QNORM_DEFINED
-#define YM_QNORM_DEFINED
-static inline double qnorm2(const double *a) {
-    double n = a[0] * a[0];
-    n = fma(a[1], a[1], n); n = fma(a[2], a[2], n); n = fma(a[3], a[3], n);
-    return n;
-}
-static inline void qmul64(const double *a, const double *b, double *r) {
-    double r0 = a[0] * b[0];
-    r0 = fma(-a[1], b[1], r0); r0 = fma(-a[2], b[2], r0); r0 = fma(-a[3], b[3], r0);
-    double r1 = a[0] * b[1];
-    r1 = fma(a[1], b[0], r1); r1 = fma(a[2], b[3], r1); r1 = fma(-a[3], b[2], r1);
-    double r2 = a[0] * b[2];
-    r2 = fma(-a[1], b[3], r2); r2 = fma(a[2], b[0], r2); r2 = fma(a[3], b[1], r2);
-    double r3 = a[0] * b[3];
-    r3 = fma(a[1], b[2], r3); r3 = fma(-a[2], b[1], r3); r3 = fma(a[3], b[0], r3);
-    r[0] = r0; r[1] = r1; r[2] = r2; r[3] = r3;
-}
-static inline void qnormalize(double *a) {
-    double inv = 1.0 / sqrt(qnorm2(a));
-    for (int c = 0; c < 4; ++c) a[c] *= inv;
-}
-#endif
 
 static int F(su2_hb)(const int64_t n[4], void *dev, double beta, uint64_t key, uint64_t sweep, int64_t max_attempts,
                      uint64_t *proposals, uint64_t *updates) {
@@ -474,51 +451,15 @@ static int F(su2_hb)(const int64_t n[4], void *dev, double beta, uint64_t key, u
             for (int64_t s = 0; s < g.vh; ++s) {
                 int64_t x[4];
                 if (!site_of(&g, p, s, x)) continue;
-                uint32_t site = lex_of(&g, x);
                 REAL A[4];
                 F(su2_staple)(L, &g, x, mu, A);
                 double a[4] = {(double)A[0], (double)A[1], (double)A[2], (double)A[3]};
-                double norm = sqrt(qnorm2(a));
-                double alpha = beta * norm;
-                double r0 = 0.0, radial = 0.0;
+                double un[4];
                 int64_t att = 0;
-                int ok = 0;
-                while (att < max_attempts) {
-                    uint32_t o[4];
-                    philox_draw(key, site, word1(mu, KIND_HB_ATTEMPT, (uint32_t)att), sweep, o);
-                    att += 1;
-                    double u1 = unit53(o[0], o[1]), u2 = unit53(o[2], o[3]);
-                    if (alpha <= 1.0e-300) {
-                        r0 = 2.0 * u1 - 1.0;
-                    } else {
-                        double tm = -ymd_expm1(-2.0 * alpha);
-                        double y = -ymd_log1p(-u1 * tm) / alpha;
-                        r0 = 1.0 - y;
-                        if (r0 < -1.0) r0 = -1.0;
-                        if (r0 > 1.0) r0 = 1.0;
-                    }
-                    radial = sqrt(fmax(fma(-r0, r0, 1.0), 0.0));
-                    if (u2 <= radial) { ok = 1; break; }
-                }
+                int ok = ymd_su2_heatbath_link(a, beta, key, lex_of(&g, x), mu, sweep, max_attempts, un, &att);
                 prop += (uint64_t)att;
                 if (!ok) { status = YMO_HEATBATH_ATTEMPT_LIMIT; continue; }
                 upd += 1;
-                uint32_t o[4];
-                philox_draw(key, site, word1(mu, KIND_HB_DIRECTION, 0), sweep, o);
-                double z = 2.0 * unit53(o[0], o[1]) - 1.0;
-                double sn, cs;
-                ymd_sincos_turns(unit53(o[2], o[3]), &sn, &cs);
-                double planar = sqrt(fmax(fma(-z, z, 1.0), 0.0));
-                double rp = radial * planar;
-                double R[4] = {r0, rp * cs, rp * sn, radial * z};
-                double un[4];
-                if (norm == 0.0) {
-                    memcpy(un, R, sizeof un);
-                } else {
-                    double ad[4] = {a[0] / norm, -(a[1] / norm), -(a[2] / norm), -(a[3] / norm)};
-                    qmul64(R, ad, un);
-                }
-                qnormalize(un);
                 REAL v[4] = {(REAL)un[0], (REAL)un[1], (REAL)un[2], (REAL)un[3]};
                 F(store_link)(L, &g, x, mu, 4, v);
             }

@@ -411,6 +411,38 @@ def test_polyakov_fp64_vs_reference(ym, group):
         assert abs(ab - accabs / cnt) <= REL
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_polyakov_centre_slice_and_closed_form(ym, prec):
+    """ym_polyakov_loop (mean over the base) under the centre-slice transform
+    (P -> z P, tests/su3_center.rs:97-120) and on diagonal fields whose loop
+    has a closed form (test_polyakov_pin.py pins the oracle the same way)."""
+    import test_polyakov_pin as PP
+    n = (4, 4, 6, 8)
+    tol = 1e-13 if prec == 64 else 5e-6
+    links = PP.hot_su3(n, 0xABC)
+    lat = ym.Su3Lattice4(n, prec)
+    lat.upload(links)
+    other = ym.Su3Lattice4(n, prec)
+    p0 = lat.average_plaquette()
+    for d in range(4):
+        before, ab = lat.polyakov_loop(d)
+        for element in (1, 2):
+            other.upload(PP.centre_transform(n, links, d, n[d] - 1, element))
+            after, ab2 = other.polyakov_loop(d)
+            assert abs(after - PP.Z[element] * before) <= tol, (d, element)
+            assert abs(ab2 - ab) <= tol
+            assert abs(other.average_plaquette() - p0) <= tol
+    for d in range(4):
+        diag, ang = PP.diagonal_field(n, d, 70 + d)
+        lat.upload(diag)
+        val, ab = lat.polyakov_loop(d)
+        base = [x for x in np.ndindex(*n) if x[d] == 0]
+        want = np.mean([PP.closed_form(n, ang, d, x) for x in base])
+        want_abs = np.mean([abs(PP.closed_form(n, ang, d, x)) for x in base])
+        assert abs(val - want) <= tol, d
+        assert abs(ab - want_abs) <= tol, d
+
+
 def test_a1pp_fp64_vs_reference(ym):
     n = (4, 4, 4, 5)
     lat, aos = hot_fp64(ym, n, 17)

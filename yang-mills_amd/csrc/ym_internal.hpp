@@ -25,6 +25,11 @@ void launch_su3_reunit(int prec, void *links, const Geom &g, hipStream_t st);
 int launch_su3_residuals(int prec, const void *links, const Geom &g, double *part, hipStream_t st);
 void launch_su3_from_aos(int prec, void *links, const Geom &g, const double *aos, hipStream_t st);
 void launch_su3_to_aos(int prec, const void *links, const Geom &g, double *aos, hipStream_t st);
+void launch_su3_from_aos_range(int prec, void *links, const Geom &g, const double *aos, int site0, int nsites,
+                               hipStream_t st);
+void launch_su3_to_aos_range(int prec, const void *links, const Geom &g, double *aos, int site0, int nsites,
+                             hipStream_t st);
+void launch_su3_identity(int prec, void *links, const Geom &g, hipStream_t st);
 
 void launch_su2_heatbath(int prec, void *links, const Geom &g, int mu, int p, double beta, uint64_t key, uint64_t sweep,
                          int max_attempts, unsigned long long *counters, int *status, hipStream_t st);
@@ -32,6 +37,12 @@ void launch_su2_overrelax(int prec, void *links, const Geom &g, int mu, int p, h
 void launch_su2_hot(int prec, void *links, const Geom &g, uint64_t key, hipStream_t st);
 void launch_su2_from_aos(int prec, void *links, const Geom &g, const double *aos, hipStream_t st);
 void launch_su2_to_aos(int prec, const void *links, const Geom &g, double *aos, hipStream_t st);
+void launch_su2_from_aos_range(int prec, void *links, const Geom &g, const double *aos, int site0, int nsites,
+                               hipStream_t st);
+void launch_su2_to_aos_range(int prec, const void *links, const Geom &g, double *aos, int site0, int nsites,
+                             hipStream_t st);
+void launch_su2_identity(int prec, void *links, const Geom &g, hipStream_t st);
+void launch_su2_normalize(int prec, void *links, const Geom &g, hipStream_t st);
 void launch_su2_staple_probe(int prec, const void *links, const Geom &g, const int x[4], int mu, double *out, hipStream_t st);
 int launch_su2_residual(int prec, const void *links, const Geom &g, double *part, hipStream_t st);
 
@@ -51,5 +62,26 @@ void launch_max_partials(const double *part, int n, int ncols, int stride, doubl
 void launch_fill_zero(double *p, size_t n, hipStream_t st);
 void launch_fill_zero_u64(unsigned long long *p, size_t n, hipStream_t st);
 void launch_fill_zero_i32(int *p, size_t n, hipStream_t st);
+
+// Host AoS image (reference layout) <-> tiled field of one lattice through a
+// device staging buffer of `stage_sites` sites: the image is moved in
+// lexicographic site chunks on stream `st`; returns a YM_* status.
+int32_t upload_aos_staged(int group, int prec, void *links, const Geom &g, const double *aos, double *stage,
+                          int64_t stage_sites, hipStream_t st);
+int32_t download_aos_staged(int group, int prec, const void *links, const Geom &g, double *aos, double *stage,
+                            int64_t stage_sites, hipStream_t st);
+// Sites per staging buffer for a lattice of `volume` sites.
+int64_t stage_sites_for(int64_t volume);
+// The reference's checks of a host AoS image before it is stored (fail
+// closed): SU(3) Su3::from_checked_matrix (su3.rs:242-253) -- finite,
+// unitarity and determinant residuals within `tol`; SU(2)
+// Su2::from_normalized_components / from_preserved_unit_components
+// (lib.rs:155-180) -- finite, non-zero, unit norm within `tol`.  The first
+// failing link in site order decides the status.
+int32_t validate_aos(int group, const double *aos, int64_t nlinks, double tol);
+// Tolerance of those checks for a storage precision: the reference's 5e-12
+// (SU(3), su3.rs:20) / 2e-12 (SU(2), lib.rs:20) for FP64 storage, 2^-16 for
+// FP32 storage (links rounded to FP32 carry residuals near 1e-7).
+double link_check_tolerance(int group, int prec);
 
 }  // namespace ym

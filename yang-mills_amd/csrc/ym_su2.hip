@@ -54,38 +54,46 @@ __device__ __forceinline__ Q4<Real> su2_staple(const typename Vec4<Real>::type *
     return A;
 }
 
-// One heat-bath attempt: r0 from the truncated exponential by inverse CDF,
-// accepted iff u2 <= sqrt(1 - r0^2) (su2_ensemble.rs:229-250).
-__device__ __forceinline__ bool hb_attempt(uint64_t key, uint32_t site, int mu, int att, uint64_t sweep, double alpha,
-                                           double tm, double &r0, double &radial) {
+// One heat-bath attempt (su2_ensemble.rs:229-250): r0 = 2 u1 - 1 when
+// alpha <= 1e-300 (inv_alpha = 0), else r0 = 1 + ln(1 - u1 tm) / alpha with
+// tm = 1 - e^{-2 alpha}, the logarithm as log1p(-u1 tm) for small alpha
+// (inv_alpha > 2) and as the log of the fused 1 - u1 tm otherwise (that
+// argument is >= e^{-1} there, so its rounding error is harmless); accepted
+// iff u2^2 <= 1 - r0^2, the reference's u2 <= sqrt(1 - r0^2) without the
+// square root.  No division: 1/alpha is formed once per link.
+__device__ __forceinline__ bool hb_attempt(uint64_t key, uint32_t site, int mu, int att, uint64_t sweep,
+                                           double inv_alpha, double tm, double &r0) {
     uint32_t o[4];
     draw(key, site, mu, KIND_HB_ATTEMPT, (uint32_t)att, sweep, o);
     const double u1 = unit53(o[0], o[1]), u2 = unit53(o[2], o[3]);
-    if (alpha <= 1.0e-300) {
+    if (inv_alpha == 0.0) {
         r0 = 2.0 * u1 - 1.0;
     } else {
-        const double y = -log1p_dev(-u1 * tm) / alpha;
-        r0 = 1.0 - y;
+        const double L = inv_alpha > 2.0 ? log1p_dev(-u1 * tm) : log_pos(__builtin_fma(-u1, tm, 1.0));
+        r0 = __builtin_fma(L, inv_alpha, 1.0);
         r0 = r0 < -1.0 ? -1.0 : (r0 > 1.0 ? 1.0 : r0);
     }
-    radial = __builtin_sqrt(fmax(__builtin_fma(-r0, r0, 1.0), 0.0));
-    return u2 <= radial;
+    return u2 * u2 <= __builtin_fma(-r0, r0, 1.0);
 }
 
 // The rejection loop is run by the whole wavefront together.  Each round the
-// P links still without an accepted attempt get k = 64 / P lanes each; lane o
-// of a link evaluates its attempt next + o.  The link takes the lowest
+// P links still without an accepted attempt get k = 64 / 2^ceil(log2 P) lanes
+// each (a power of two, so lane -> (link, offset) is a shift and a mask);
+// lane o of link r evaluates attempt next_r + o.  A link takes the lowest
 // accepted attempt index of the round (or moves next on by k), so every link
 // ends with exactly the attempt, r0 and proposal count of its own sequential
 // loop, while a wave needs about (total attempts) / 64 rounds instead of the
-// largest attempt count among its 64 links.
+// largest attempt count among its 64 links.  The pending links' parameters
+// and the accepted r0 pass through per-wave LDS tables indexed by rank.
 template <typename Real, int MU>
 __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type *__restrict__ L, Geom g, int p,
                                                       double beta, uint64_t key, uint64_t sweep, int max_attempts,
                                                       unsigned long long *__restrict__ counters,
                                                       int *__restrict__ status) {
     __shared__ unsigned long long red[4];
-    __shared__ int owner[4][64];
+    __shared__ double2 tab_par[4][64];  // (1/alpha, tm) of pending link `rank`
+    __shared__ int2 tab_idx[4][64];     // (site, next attempt) of pending link `rank`
+    __shared__ double tab_r0[4][64];    // r0 of the first accepted attempt of link `rank`
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // parity-1 sub-passes walk the tiles backwards (the field of a 48^4
     // lattice is larger than the memory-side cache; see k_su3_metropolis_ws)
@@ -95,48 +103,57 @@ __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type 
     int x[4] = {0, 0, 0, 0};
     const bool has = s < g.vh && site_of(g, p, s, x);
     Q4<double> a;
-    double norm = 0.0, alpha = 0.0, tm = 0.0;
+    double norm = 0.0, inv_alpha = 0.0, tm = 0.0;
     uint32_t site = 0;
     if (has) {
         const Q4<Real> Af = su2_staple<Real, MU>(L, g, x, p, s);
 #pragma unroll
         for (int c = 0; c < 4; ++c) a.c[c] = (double)Af.c[c];
         norm = __builtin_sqrt(qnorm2(a));
-        alpha = beta * norm;
+        const double alpha = beta * norm;
         site = (uint32_t)lex_of(g, x);
-        tm = alpha <= 1.0e-300 ? 0.0 : -expm1_nonpos(-2.0 * alpha);
+        if (alpha > 1.0e-300) {
+            inv_alpha = 1.0 / alpha;
+            tm = -expm1_nonpos(-2.0 * alpha);
+        }
     }
     bool pending = has, ok = false;
     int next = 0;
-    double r0 = 0.0, radial = 0.0;
+    double r0 = 0.0;
+    const unsigned long long below = (1ull << lane) - 1ull;
     while (true) {
         const unsigned long long pm = __ballot(pending);
         if (pm == 0ull) break;
         const int P = __popcll(pm);
-        const int k = 64 / P;
-        const int rank = __popcll(pm & ((1ull << lane) - 1ull));
-        if (pending) owner[wave][rank] = lane;
-        wave_lds_sync();
-        const int r = lane / k, o = lane - r * k;
-        const int src = r < P ? owner[wave][r] : lane;
-        const double alpha_o = __shfl(alpha, src), tm_o = __shfl(tm, src);
-        const uint32_t site_o = (uint32_t)__shfl((int)site, src);
-        const int att = __shfl(next, src) + o;
-        double tr0 = 0.0, trad = 0.0;
-        const bool hit = r < P && att < max_attempts && hb_attempt(key, site_o, MU, att, sweep, alpha_o, tm_o, tr0, trad);
-        const unsigned long long am = __ballot(hit);
-        // lanes of this link: [rank * k, rank * k + k)
-        const unsigned long long span = (k == 64) ? ~0ull : (((1ull << k) - 1ull) << (rank * k));
-        const unsigned long long mine = pending ? (am & span) : 0ull;
-        const int first = mine ? __ffsll((long long)mine) - 1 : lane;
-        const double r0_f = __shfl(tr0, first), rad_f = __shfl(trad, first);
+        const int lg = P > 1 ? 32 - __clz(P - 1) : 0;  // ceil(log2 P)
+        const int sh = 6 - lg, k = 1 << sh;            // lanes per link
+        const int rank = __popcll(pm & below);
         if (pending) {
+            tab_par[wave][rank] = make_double2(inv_alpha, tm);
+            tab_idx[wave][rank] = make_int2((int)site, next);
+        }
+        wave_lds_sync();
+        const int r = lane >> sh, o = lane & (k - 1);
+        bool hit = false;
+        double tr0 = 0.0;
+        if (r < P) {
+            const double2 par = tab_par[wave][r];
+            const int2 idx = tab_idx[wave][r];
+            const int att = idx.y + o;
+            hit = att < max_attempts && hb_attempt(key, (uint32_t)idx.x, MU, att, sweep, par.x, par.y, tr0);
+        }
+        const unsigned long long am = __ballot(hit);
+        // the first accepted lane of each link's group publishes its r0
+        const unsigned long long grp = (k == 64) ? ~0ull : (((1ull << k) - 1ull) << (r << sh));
+        if (hit && (am & grp & below) == 0ull) tab_r0[wave][r] = tr0;
+        wave_lds_sync();
+        if (pending) {
+            const unsigned long long mine = (k == 64) ? am : ((am >> (rank << sh)) & ((1ull << k) - 1ull));
             if (mine) {
                 ok = true;
                 pending = false;
-                r0 = r0_f;
-                radial = rad_f;
-                nprop = (unsigned long long)(next + (first - rank * k) + 1);
+                r0 = tab_r0[wave][rank];
+                nprop = (unsigned long long)(next + __ffsll((long long)mine));
             } else {
                 next += k;
                 if (next >= max_attempts) {
@@ -152,6 +169,7 @@ __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type 
             atomicOr(status, 1);
         } else {
             nupd = 1;
+            const double radial = __builtin_sqrt(fmax(__builtin_fma(-r0, r0, 1.0), 0.0));
             uint32_t o[4];
             draw(key, site, MU, KIND_HB_DIRECTION, 0, sweep, o);
             const double z = 2.0 * unit53(o[0], o[1]) - 1.0;
@@ -165,8 +183,10 @@ __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type 
             if (norm == 0.0) {
                 un = R;
             } else {
+                const double inv_norm = 1.0 / norm;
                 Q4<double> ad;
-                ad.c[0] = a.c[0] / norm; ad.c[1] = -(a.c[1] / norm); ad.c[2] = -(a.c[2] / norm); ad.c[3] = -(a.c[3] / norm);
+                ad.c[0] = a.c[0] * inv_norm; ad.c[1] = -(a.c[1] * inv_norm);
+                ad.c[2] = -(a.c[2] * inv_norm); ad.c[3] = -(a.c[3] * inv_norm);
                 un = qmul(R, ad);
             }
             qnormalize(un);
@@ -321,6 +341,91 @@ void launch_su2_to_aos(int prec, const void *links, const Geom &g, double *aos, 
     dim3 grid((g.vh + 255) / 256, 2, 4);
     if (prec == 32) hipLaunchKernelGGL((k_su2_to_aos<float>), grid, dim3(256), 0, st, (const float4 *)links, g, aos);
     else hipLaunchKernelGGL((k_su2_to_aos<double>), grid, dim3(256), 0, st, (const double4 *)links, g, aos);
+}
+
+// Host-layout transfer by lexicographic site range (see ym_su3.hip): thread
+// i owns link (site0 + i / 4, mu = i % 4), 4 doubles at aos[4 i].
+__device__ __forceinline__ void su2_site_of_lex(const Geom &g, int lex, int x[4]) {
+    x[3] = lex % g.n[3];
+    int r = lex / g.n[3];
+    x[2] = r % g.n[2];
+    r /= g.n[2];
+    x[1] = r % g.n[1];
+    x[0] = r / g.n[1];
+}
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su2_from_aos_range(typename Vec4<Real>::type *__restrict__ L, Geom g,
+                                                            const double *__restrict__ aos, int site0, int nlinks) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nlinks) return;
+    int x[4];
+    su2_site_of_lex(g, site0 + (i >> 2), x);
+    Q4<Real> v;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v.c[c] = (Real)aos[(size_t)i * 4 + c];
+    store2<Real>(L, g.vh, i & 3, parity_of(x), slot_of(g, x), v);
+}
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su2_to_aos_range(const typename Vec4<Real>::type *__restrict__ L, Geom g,
+                                                          double *__restrict__ aos, int site0, int nlinks) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nlinks) return;
+    int x[4];
+    su2_site_of_lex(g, site0 + (i >> 2), x);
+    const Q4<Real> v = load2<Real>(L, g.vh, i & 3, parity_of(x), slot_of(g, x));
+#pragma unroll
+    for (int c = 0; c < 4; ++c) aos[(size_t)i * 4 + c] = (double)v.c[c];
+}
+void launch_su2_from_aos_range(int prec, void *links, const Geom &g, const double *aos, int site0, int nsites,
+                               hipStream_t st) {
+    const int nl = 4 * nsites;
+    const dim3 grid((nl + 255) / 256);
+    if (prec == 32) hipLaunchKernelGGL((k_su2_from_aos_range<float>), grid, dim3(256), 0, st, (float4 *)links, g, aos, site0, nl);
+    else hipLaunchKernelGGL((k_su2_from_aos_range<double>), grid, dim3(256), 0, st, (double4 *)links, g, aos, site0, nl);
+}
+void launch_su2_to_aos_range(int prec, const void *links, const Geom &g, double *aos, int site0, int nsites,
+                             hipStream_t st) {
+    const int nl = 4 * nsites;
+    const dim3 grid((nl + 255) / 256);
+    if (prec == 32) hipLaunchKernelGGL((k_su2_to_aos_range<float>), grid, dim3(256), 0, st, (const float4 *)links, g, aos, site0, nl);
+    else hipLaunchKernelGGL((k_su2_to_aos_range<double>), grid, dim3(256), 0, st, (const double4 *)links, g, aos, site0, nl);
+}
+
+// Identity in every slot (padding included); FP64 normalisation of every
+// link, a0^2 + a1^2 + a2^2 + a3^2 by an FMA chain and one reciprocal square
+// root per link.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su2_identity(typename Vec4<Real>::type *__restrict__ L, Geom g) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= g.vh) return;
+    Q4<Real> v;
+    v.c[0] = 1; v.c[1] = 0; v.c[2] = 0; v.c[3] = 0;
+    store2<Real>(L, g.vh, blockIdx.z, blockIdx.y, s, v);
+}
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su2_normalize(typename Vec4<Real>::type *__restrict__ L, Geom g) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    int x[4];
+    if (s >= g.vh || !site_of(g, blockIdx.y, s, x)) return;
+    const Q4<Real> v = load2<Real>(L, g.vh, blockIdx.z, blockIdx.y, s);
+    Q4<double> q;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) q.c[c] = (double)v.c[c];
+    qnormalize(q);
+    Q4<Real> w;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) w.c[c] = (Real)q.c[c];
+    store2<Real>(L, g.vh, blockIdx.z, blockIdx.y, s, w);
+}
+void launch_su2_identity(int prec, void *links, const Geom &g, hipStream_t st) {
+    const dim3 grid((g.vh + 255) / 256, 2, 4);
+    if (prec == 32) hipLaunchKernelGGL((k_su2_identity<float>), grid, dim3(256), 0, st, (float4 *)links, g);
+    else hipLaunchKernelGGL((k_su2_identity<double>), grid, dim3(256), 0, st, (double4 *)links, g);
+}
+void launch_su2_normalize(int prec, void *links, const Geom &g, hipStream_t st) {
+    const dim3 grid((g.vh + 255) / 256, 2, 4);
+    if (prec == 32) hipLaunchKernelGGL((k_su2_normalize<float>), grid, dim3(256), 0, st, (float4 *)links, g);
+    else hipLaunchKernelGGL((k_su2_normalize<double>), grid, dim3(256), 0, st, (double4 *)links, g);
 }
 
 // Staple at one site (parity tests).

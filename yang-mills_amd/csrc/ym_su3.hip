@@ -316,4 +316,68 @@ void launch_su3_to_aos(int prec, const void *links, const Geom &g, double *aos, 
     else hipLaunchKernelGGL((k_su3_to_aos<double>), grid, dim3(256), 0, st, (const double2 *)links, g, aos);
 }
 
+// Host-layout transfer by lexicographic site range: thread i of a chunk owns
+// link (site0 + i / 4, mu = i % 4), whose 18 doubles sit at aos[18 i]; the
+// chunk moves through a fixed staging buffer, so no transfer needs device
+// memory proportional to the lattice.  Padding slots (odd N3) are not
+// touched; they keep the identity written when the field was created.
+__device__ __forceinline__ void site_of_lex(const Geom &g, int lex, int x[4]) {
+    x[3] = lex % g.n[3];
+    int r = lex / g.n[3];
+    x[2] = r % g.n[2];
+    r /= g.n[2];
+    x[1] = r % g.n[1];
+    x[0] = r / g.n[1];
+}
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su3_from_aos_range(typename Vec2<Real>::type *__restrict__ L, Geom g,
+                                                            const double *__restrict__ aos, int site0, int nlinks) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nlinks) return;
+    int x[4];
+    site_of_lex(g, site0 + (i >> 2), x);
+    M3<Real> m;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) m.v[c] = (Real)aos[(size_t)i * 18 + c];
+    store3<Real>(L, g.vh, i & 3, parity_of(x), slot_of(g, x), m);
+}
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su3_to_aos_range(const typename Vec2<Real>::type *__restrict__ L, Geom g,
+                                                          double *__restrict__ aos, int site0, int nlinks) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nlinks) return;
+    int x[4];
+    site_of_lex(g, site0 + (i >> 2), x);
+    const M3<Real> m = load3<Real>(L, g.vh, i & 3, parity_of(x), slot_of(g, x));
+#pragma unroll
+    for (int c = 0; c < 18; ++c) aos[(size_t)i * 18 + c] = (double)m.v[c];
+}
+void launch_su3_from_aos_range(int prec, void *links, const Geom &g, const double *aos, int site0, int nsites,
+                               hipStream_t st) {
+    const int nl = 4 * nsites;
+    const dim3 grid((nl + 255) / 256);
+    if (prec == 32) hipLaunchKernelGGL((k_su3_from_aos_range<float>), grid, dim3(256), 0, st, (float2 *)links, g, aos, site0, nl);
+    else hipLaunchKernelGGL((k_su3_from_aos_range<double>), grid, dim3(256), 0, st, (double2 *)links, g, aos, site0, nl);
+}
+void launch_su3_to_aos_range(int prec, const void *links, const Geom &g, double *aos, int site0, int nsites,
+                             hipStream_t st) {
+    const int nl = 4 * nsites;
+    const dim3 grid((nl + 255) / 256);
+    if (prec == 32) hipLaunchKernelGGL((k_su3_to_aos_range<float>), grid, dim3(256), 0, st, (const float2 *)links, g, aos, site0, nl);
+    else hipLaunchKernelGGL((k_su3_to_aos_range<double>), grid, dim3(256), 0, st, (const double2 *)links, g, aos, site0, nl);
+}
+
+// Identity in every slot of every (mu, parity) plane, padding included.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_su3_identity(typename Vec2<Real>::type *__restrict__ L, Geom g) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= g.vh) return;
+    store3<Real>(L, g.vh, blockIdx.z, blockIdx.y, s, identity3<Real>());
+}
+void launch_su3_identity(int prec, void *links, const Geom &g, hipStream_t st) {
+    const dim3 grid((g.vh + 255) / 256, 2, 4);
+    if (prec == 32) hipLaunchKernelGGL((k_su3_identity<float>), grid, dim3(256), 0, st, (float2 *)links, g);
+    else hipLaunchKernelGGL((k_su3_identity<double>), grid, dim3(256), 0, st, (double2 *)links, g);
+}
+
 }  // namespace ym
