@@ -111,6 +111,7 @@ enum {
     YM_VAR_PROJECTION_NORMALIZATION = 147, /* VariationalError::ProjectionNormalizationFailure */
     YM_WRONG_CONFIGURATION = 90,      /* benchmark gate: schedule does not allow the split */
     YM_STATIONARITY_FAILED = 91,      /* benchmark gate: constant half or zero spread */
+    YM_WORKSPACE_EXCEEDED = 95,       /* input larger than an allocation-free entry point's fixed workspace */
     YM_INVALID_ARGUMENT = 96,         /* null pointer, wrong group/precision, size mismatch */
     YM_NO_DEVICE = 97,
     YM_HIP_ERROR = 98,
@@ -194,6 +195,19 @@ int32_t ym_su2_overrelax_sweeps(ym_lattice *lat, int32_t n_sweeps);
 /* su2_ensemble.rs:348-373 / su3_ensemble.rs:228-259 average_plaquette:
  * (1/6V) sum_x sum_{mu<nu} (1/N) Re Tr P_{mu nu}(x), FP64 accumulation. */
 int32_t ym_average_plaquette(const ym_lattice *lat, double *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops or harnesses that
+ * forbid hipMalloc): ym_average_plaquette with the same kernels and the same
+ * result, bit for bit, but its per-block partials in a fixed library-wide
+ * device array of 65,536 doubles (512 KiB per GPU, held until the process
+ * ends) instead of the handle's lazily grown buffer; it calls no hipMalloc,
+ * hipFree or allocating library routine.  Limit: 2 * ceil(Vh / 256) <= 65,536
+ * partials with Vh the slots per parity (V/2 for even extents), i.e. at most
+ * 2^23 slots per parity (a 64^4 lattice); larger lattices return
+ * YM_WORKSPACE_EXCEEDED.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (two host threads, two
+ * streams) race. */
+int32_t ym_average_plaquette_noalloc(const ym_lattice *lat, double *out);
 /* lib.rs:363-382 / su3_lattice.rs:214-242 wilson_action: beta * sum (1 - P). */
 int32_t ym_wilson_action(const ym_lattice *lat, double beta, double *out);
 /* lib.rs:507-546 staple (SU(2)); for SU(3) the same six-term sum A with

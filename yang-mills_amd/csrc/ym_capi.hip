@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -43,6 +44,7 @@ void launch_fill_zero_i32(int *p, size_t n, hipStream_t st) {
 
 struct ym_lattice {
     int group = 0, prec = 0;
+    int device = 0;          // HIP device current when the handle was created
     int64_t n[4] = {0, 0, 0, 0};
     ym::Geom g{};
     int ncomp = 0;           // reals per link
@@ -113,15 +115,18 @@ bool all_even(const ym_lattice *l) {
 // ncols <= 64 results through the pinned buffer: a device-to-host copy into
 // pageable memory is staged by the runtime and costs several microseconds
 // more per measurement call.
-int32_t sum_partials(ym_lattice *l, int nblk, int ncols, double *host_out) {
+int32_t sum_partials_from(ym_lattice *l, const double *part, int nblk, int ncols, double *host_out) {
     if (ncols < 1 || ncols > 64) return YM_INVALID_ARGUMENT;
-    ym::launch_sum_partials(l->part, nblk, ncols, nblk, l->dres, l->stream);
+    ym::launch_sum_partials(part, nblk, ncols, nblk, l->dres, l->stream);
     YM_HIP(hipGetLastError());
     YM_HIP(hipMemcpyAsync(l->hres, l->dres, sizeof(double) * ncols, hipMemcpyDeviceToHost, l->stream));
     const int32_t st = finish(l);
     if (st) return st;
     std::memcpy(host_out, l->hres, sizeof(double) * ncols);
     return YM_OK;
+}
+int32_t sum_partials(ym_lattice *l, int nblk, int ncols, double *host_out) {
+    return sum_partials_from(l, l->part, nblk, ncols, host_out);
 }
 
 }  // namespace
@@ -209,6 +214,7 @@ const char *ym_status_string(int32_t s) {
     case YM_VAR_PROJECTION_NORMALIZATION: return "projected reference correlator is not normalized";
     case YM_WRONG_CONFIGURATION: return "schedule does not admit the half split";
     case YM_STATIONARITY_FAILED: return "stationarity gate failed (constant half or zero spread)";
+    case YM_WORKSPACE_EXCEEDED: return "input exceeds the fixed device workspace of an allocation-free entry point";
     case YM_INVALID_ARGUMENT: return "invalid argument";
     case YM_NO_DEVICE: return "no HIP device";
     case YM_HIP_ERROR: return "HIP runtime error";
@@ -241,8 +247,11 @@ int32_t ym_lattice_create(int32_t group, const int64_t extents[4], int32_t preci
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return YM_NO_DEVICE;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return YM_NO_DEVICE;
     ym_lattice *l = new (std::nothrow) ym_lattice();
     if (!l) return YM_ALLOCATION_FAILED;
+    l->device = dev;
     l->group = group;
     l->prec = precision;
     for (int d = 0; d < 4; ++d) l->n[d] = extents[d];
@@ -565,6 +574,68 @@ int32_t ym_average_plaquette(const ym_lattice *lc, double *out) {
     if (!std::isfinite(mean)) return YM_NON_FINITE_STATISTIC;
     *out = mean;
     return YM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ allocation-free variant
+// Per-block partials of ym_average_plaquette_noalloc live in this array: the
+// code object defines it in the device memory of every GPU the library runs
+// on, where it stays until the process ends.  Its host-side name is only a
+// stand-in; kernels get the device address that hipGetSymbolAddress returns
+// for the current device, looked up once per device and cached below.
+namespace {
+constexpr int kNoallocPartials = 1 << 16;  // doubles: 512 KiB per GPU
+constexpr int kMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) double scratch[kNoallocPartials];
+namespace {
+std::mutex g_scratch_mutex;
+double *g_scratch_address[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev`, which must be current.
+int32_t scratch_address(int dev, double **out) {
+    if (dev < 0 || dev >= kMaxDevices) return YM_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return YM_HIP_ERROR;
+        g_scratch_address[dev] = (double *)p;
+    }
+    *out = g_scratch_address[dev];
+    return YM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// ym_average_plaquette with its partials in `scratch`: no hipMalloc, no
+// hipFree, no allocating runtime or library call.  Every partial that the
+// sum reads is written by the plaquette kernel of the same call, so nothing
+// has to be cleared first.  Calls that overlap on one GPU (two host threads,
+// or work on two streams) share the array and race.
+int32_t ym_average_plaquette_noalloc(const ym_lattice *lc, double *out) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    if (!l || !out) return YM_INVALID_ARGUMENT;
+    const int nblk = 2 * ((l->g.vh + 255) / 256);
+    if (nblk > kNoallocPartials) return YM_WORKSPACE_EXCEEDED;
+    int cur = 0;
+    YM_HIP(hipGetDevice(&cur));
+    if (cur != l->device) YM_HIP(hipSetDevice(l->device));
+    double *part = nullptr;
+    int32_t st = scratch_address(l->device, &part);
+    if (!st) {
+        ym::launch_plaquette(l->group, l->prec, l->links, l->g, part, l->stream);
+        double sum = 0.0;
+        st = sum_partials_from(l, part, nblk, 1, &sum);
+        if (!st) {
+            const double mean = sum / (6.0 * (double)l->g.volume);
+            if (std::isfinite(mean)) *out = mean;
+            else st = YM_NON_FINITE_STATISTIC;
+        }
+    }
+    if (cur != l->device) (void)hipSetDevice(cur);
+    return st;
 }
 
 int32_t ym_wilson_action(const ym_lattice *l, double beta, double *out) {

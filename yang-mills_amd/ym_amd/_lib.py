@@ -78,6 +78,7 @@ SIGNATURES = {
     "ym_su2_heatbath_sweeps": (i32, [C.c_void_p, dbl, u64, u64, i32, i32, P(u64), P(u64)]),
     "ym_su2_overrelax_sweeps": (i32, [C.c_void_p, i32]),
     "ym_average_plaquette": (i32, [C.c_void_p, P(dbl)]),
+    "ym_average_plaquette_noalloc": (i32, [C.c_void_p, P(dbl)]),
     "ym_wilson_action": (i32, [C.c_void_p, dbl, P(dbl)]),
     "ym_staple": (i32, [C.c_void_p, P(i64), i32, P(dbl)]),
     "ym_path_holonomy": (i32, [C.c_void_p, P(i64), P(i32), i64, P(dbl), P(i64)]),
