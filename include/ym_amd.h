@@ -45,10 +45,14 @@ enum {
     YM_PATH_TOO_LONG = 10,            /* PathHolonomyError::PathTooLong */
     YM_ZERO_RECTANGLE_EXTENT = 12,
     YM_SU2_NON_FINITE = 13,           /* ReferenceError::NonFiniteSu2Component */
+    YM_SU2_ZERO_NORM = 14,            /* ReferenceError::ZeroSu2Norm */
     YM_SU2_NON_UNIT = 15,             /* ReferenceError::NonUnitSu2 */
     YM_COORDINATE_OUT_OF_RANGE = 17,  /* ReferenceError::CoordinateOutOfRange */
     YM_INVALID_CONFIGURATION_ENCODING = 18, /* ReferenceError::InvalidConfigurationEncoding */
     YM_LATTICE_SIZE_OVERFLOW = 19,    /* ReferenceError::LatticeSizeOverflow */
+    YM_SU3_NON_FINITE_MATRIX = 23,    /* Su3Error::NonFiniteMatrix */
+    YM_SU3_NON_UNITARY = 25,          /* Su3Error::NonUnitaryMatrix */
+    YM_SU3_DETERMINANT_NOT_ONE = 26,  /* Su3Error::DeterminantNotOne */
     YM_BETA_OUT_OF_RANGE = 30,        /* Su3EnsembleError / EnsembleError::BetaOutsideReferenceRange */
     YM_ANGLE_OUT_OF_RANGE = 31,       /* Su3EnsembleError::ProposalAngleOutsideReferenceRange */
     YM_MEASUREMENT_COUNT = 32,
@@ -135,8 +139,21 @@ int32_t ym_lattice_set_identity(ym_lattice *lat);
  * Philox4x32-10 keyed by `seed`, counter (site, mu, draw). */
 int32_t ym_lattice_hot_start(ym_lattice *lat, uint64_t seed);
 /* Lattice4::set_link / Su3Lattice4::set_link for every link at once
- * (lib.rs:323-331, su3_lattice.rs:154-162): `count` = links * (4 or 18). */
+ * (lib.rs:323-331, su3_lattice.rs:154-162): `count` = links * (4 or 18).
+ * Fails closed, before anything is stored, on a link the reference's types
+ * cannot hold (the first such link in site order decides the status):
+ * SU(3) as Su3::from_checked_matrix (su3.rs:242-253) -- YM_SU3_NON_FINITE_MATRIX,
+ * YM_SU3_NON_UNITARY (max entry of |U^+U - 1|, |UU^+ - 1|), then
+ * YM_SU3_DETERMINANT_NOT_ONE; SU(2) as Su2 (lib.rs:155-180) --
+ * YM_SU2_NON_FINITE, YM_SU2_ZERO_NORM, YM_SU2_NON_UNIT.  Tolerance: the
+ * reference's 5e-12 (SU(3)) / 2e-12 (SU(2)) for FP64 storage, 2^-16 for FP32
+ * storage.  The image moves through a staging buffer of at most 2^16 sites
+ * that the handle allocates on first use and keeps. */
 int32_t ym_lattice_upload(ym_lattice *lat, const double *aos, int64_t count);
+/* ym_lattice_upload without the link checks (no reference counterpart: the
+ * reference's link types cannot carry unchecked values).  For tests and for
+ * callers that have checked the configuration themselves. */
+int32_t ym_lattice_upload_unchecked(ym_lattice *lat, const double *aos, int64_t count);
 /* Lattice4::link / Su3Lattice4::link for every link (lib.rs:319-321). */
 int32_t ym_lattice_download(const ym_lattice *lat, double *aos, int64_t count);
 /* Clone (same group, precision and extents). */
@@ -190,6 +207,14 @@ int32_t ym_su2_heatbath_sweeps(ym_lattice *lat, double beta, uint64_t key, uint6
                                int32_t max_attempts, uint64_t *proposals, uint64_t *updates);
 /* su2_ensemble.rs:283-299, 344-346 overrelaxation_sweep, checkerboard order. */
 int32_t ym_su2_overrelax_sweeps(ym_lattice *lat, int32_t n_sweeps);
+/* su2_ensemble.rs:465-486 markov_sweep, n_sweeps times: one heat-bath sweep
+ * (sweep index first_sweep + k) then `overrelaxation_sweeps_per_heatbath`
+ * (0..16) over-relaxation sweeps.  Counts accumulate as in
+ * ym_su2_heatbath_sweeps; YM_HEATBATH_ATTEMPT_LIMIT after the sweeps if a
+ * link exhausted its attempts. */
+int32_t ym_su2_markov_sweeps(ym_lattice *lat, double beta, uint64_t key, uint64_t first_sweep, int32_t n_sweeps,
+                             int32_t overrelaxation_sweeps_per_heatbath, int32_t max_attempts, uint64_t *proposals,
+                             uint64_t *updates);
 
 /* ----------------------------------------------------------- observables */
 /* su2_ensemble.rs:348-373 / su3_ensemble.rs:228-259 average_plaquette:
@@ -299,6 +324,40 @@ typedef struct {
  * config->measurements doubles. */
 int32_t ym_su3_run_chain(const ym_su3_chain_config *config, double *measurements_out, ym_su3_chain_report *report);
 
+/* su2_ensemble.rs:56-68 EnsembleConfig, plus the storage precision and the
+ * start (the reference is cold-start only). */
+typedef struct {
+    int64_t extents[4];
+    double beta;
+    uint64_t master_seed, chain_id;
+    int64_t thermalization_sweeps, measurements, sweeps_between_measurements;
+    int64_t overrelaxation_sweeps_per_heatbath;
+    int64_t max_heatbath_attempts_per_link;
+    int64_t autocorrelation_max_lag;
+    int32_t precision;   /* 32 or 64 */
+    int32_t hot_start;   /* 0 = cold (reference), 1 = Haar-random from the chain key */
+} ym_su2_chain_config;
+
+/* su2_ensemble.rs:173-186 ChainReport (measurements returned separately). */
+typedef struct {
+    uint64_t derived_seed;
+    int64_t total_heatbath_sweeps, total_overrelaxation_sweeps;
+    uint64_t heatbath_proposals, heatbath_updates;
+    double mean_heatbath_acceptance;
+    double final_link_unit_residual;
+    ym_chain_diagnostics diagnostics;
+} ym_su2_chain_report;
+
+/* su2_ensemble.rs:488-552 run_chain on the device: EnsembleConfig::validate
+ * (su2_ensemble.rs:70-103, same bounds and order: beta in [0, 8], at most
+ * 2^20 sweeps, >= 2 measurements, stride > 0, <= 16 over-relaxation sweeps,
+ * attempt limit > 0, lag < measurements), key = derive_chain_seed(master,
+ * chain) (su2_ensemble.rs:46-55), thermalisation Markov sweeps, then
+ * `measurements` x (`sweeps_between_measurements` Markov sweeps + average
+ * plaquette), diagnostics, acceptance and the final unit residual.
+ * measurements_out holds config->measurements doubles. */
+int32_t ym_su2_run_chain(const ym_su2_chain_config *config, double *measurements_out, ym_su2_chain_report *report);
+
 /* ------------------------------------------------------ batched ensembles */
 /* Independent SU(3) chains of uneven geometries in one device buffer; every
  * sweep and measurement is one batched launch over all chains.  Reference
@@ -321,7 +380,8 @@ int32_t ym_ensemble_set_identity(ym_ensemble *ens);
 /* Haar-random start of every chain from its key (no reference counterpart). */
 int32_t ym_ensemble_hot_start(ym_ensemble *ens);
 int32_t ym_ensemble_chain_count(const ym_ensemble *ens, int32_t *nchains, int64_t *total_sites, int64_t *time_sum);
-/* Reference AoS FP64 layout of one chain (volume x 4 x 18 doubles). */
+/* Reference AoS FP64 layout of one chain (volume x 4 x 18 doubles); the
+ * upload applies ym_lattice_upload's link checks and fails closed. */
 int32_t ym_ensemble_upload_chain(ym_ensemble *ens, int32_t chain, const double *aos, int64_t count);
 int32_t ym_ensemble_download_chain(const ym_ensemble *ens, int32_t chain, double *aos, int64_t count);
 int32_t ym_ensemble_sweep_index(const ym_ensemble *ens, uint64_t *sweep);

@@ -5,6 +5,8 @@ start, 192 + 128 x 2 sweeps, 8 proposals per link, angle 0.35;
 su3_confirmation_reference.rs) against the reference's frozen pooled
 plaquette 0.5992920042685975 +- 0.0008268453534905086
 (su3-plaquette-baseline-v2.json), with FP64 link residuals <= 1e-12.
+SU(2): device heat-bath + over-relaxation chains against reference-order
+CPU chains of the same job and the published 10^4 beta = 2.3 pooled mean.
 FP32 storage: observables of an FP32 copy of an FP64 configuration against
 FP64 (SU(2) heat-bath/over-relaxation lattices and SU(3) with stout).
 """
@@ -103,6 +105,68 @@ def test_level_c_against_reference_chains_in_the_same_job(ym):
     assert z < 2.0, (gm, gse, cm, cse, z)
     assert abs(np.mean(acc) - np.mean([c[1] for c in cpu])) < 0.004
     assert 0.56 <= cm <= 0.63  # the reference's external envelope (su3_benchmark.rs:23-28)
+
+
+# docs/PUBLISHED_PLAQUETTE_BASELINE.md:109-126: two 10^4 chains at beta = 2.3,
+# 40 + 48 Markov sweeps (1 heat-bath + 1 over-relaxation), pooled mean and SE
+PUBLISHED_SU2_MEAN, PUBLISHED_SU2_SE = 0.6022919679191184, 0.0004173286094490
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_su2_level_c_against_reference_chains_and_published_mean(ym, prec):
+    """Level (c) for SU(2): device heat-bath + over-relaxation chains
+    (ym_su2_run_chain: checkerboard order, Philox) with the reference's
+    published schedule (su2_benchmark.rs:21-35: 10^4, beta = 2.3, cold start,
+    40 thermalisation + 48 measurements, one over-relaxation sweep per
+    heat-bath sweep) against reference-order CPU chains run in this job (the
+    oracle's restatement of su2_ensemble.rs:488-552: lexicographic order,
+    SplitMix64, normalised FP64 quaternion products; about 7 s per chain, 8
+    in parallel threads), and against the published pooled mean.  Means
+    agree within 2 sigma of the jackknife over chains; heat-bath
+    acceptances agree."""
+    import ctypes as C
+    import threading
+
+    import oracle_lib as O
+
+    n, therm, meas = (10, 10, 10, 10), 40, 48
+    master = 0x2B00201211160101
+    ncpu = 8
+    cpu = [None] * ncpu
+
+    def run(i):
+        cfg = O.Su2ChainConfig()
+        cfg.extents[:] = list(n)
+        cfg.beta, cfg.master_seed, cfg.chain_id = 2.3, master, 1000 + i
+        cfg.thermalization_sweeps, cfg.measurements, cfg.sweeps_between_measurements = therm, meas, 1
+        cfg.overrelaxation_sweeps_per_heatbath, cfg.max_heatbath_attempts_per_link = 1, 10_000
+        cfg.autocorrelation_max_lag, cfg.no_caps = 12, 0
+        m = np.zeros(meas)
+        d, r = O.ChainDiagnostics(), O.Su2ChainReport()
+        assert O.lib().ymo_su2_run_chain(C.byref(cfg), O.dptr(m), C.byref(d), C.byref(r), None) == 0
+        cpu[i] = (m, r.mean_heatbath_acceptance)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(ncpu)]
+    for t in threads:
+        t.start()
+    gpu, acc = [], []
+    for chain in range(16):
+        rep = ym.su2_run_chain(n, 2.3, master, 2000 + chain, therm, meas, 1, 1, 10_000, 12, precision=prec)
+        assert rep.total_heatbath_sweeps == therm + meas and rep.total_overrelaxation_sweeps == therm + meas
+        assert rep.heatbath_updates == (therm + meas) * 4 * 10**4
+        assert rep.final_link_unit_residual <= (1e-12 if prec == 64 else 1e-6)
+        gpu.append(rep.measurements.mean())
+        acc.append(rep.mean_heatbath_acceptance)
+    for t in threads:
+        t.join()
+    gm, gse = _jackknife_over_chains(gpu)
+    cm, cse = _jackknife_over_chains([c[0].mean() for c in cpu])
+    z = abs(gm - cm) / math.hypot(gse, cse)
+    assert z < 2.0, (gm, gse, cm, cse, z)
+    zp = abs(gm - PUBLISHED_SU2_MEAN) / math.hypot(gse, PUBLISHED_SU2_SE)
+    assert zp < 2.0, (gm, gse, zp)
+    assert abs(np.mean(acc) - np.mean([c[1] for c in cpu])) < 0.003, (np.mean(acc), [c[1] for c in cpu])
+    assert 0.59 <= gm <= 0.62  # the published source-resolution envelope (su2_benchmark.rs:23-24)
 
 
 def test_su2_fp32_storage_errors(ym):

@@ -7,9 +7,12 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <algorithm>
+#include <complex>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/ym_amd.h"
@@ -40,6 +43,125 @@ void launch_fill_zero_i32(int *p, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_fill_zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
 }
 
+int64_t stage_sites_for(int64_t volume) { return std::min<int64_t>(volume, (int64_t)1 << 16); }
+
+int32_t upload_aos_staged(int group, int prec, void *links, const Geom &g, const double *aos, double *stage,
+                          int64_t stage_sites, hipStream_t st) {
+    const int nc = group == 3 ? 18 : 4;
+    for (int64_t s0 = 0; s0 < g.volume; s0 += stage_sites) {
+        const int64_t ns = std::min<int64_t>(stage_sites, g.volume - s0);
+        // the copy into the staging buffer is ordered after the previous
+        // chunk's kernel on the same stream
+        if (hipMemcpyAsync(stage, aos + s0 * 4 * nc, sizeof(double) * (size_t)ns * 4 * nc, hipMemcpyHostToDevice,
+                           st) != hipSuccess)
+            return YM_HIP_ERROR;
+        if (group == 3) launch_su3_from_aos_range(prec, links, g, stage, (int)s0, (int)ns, st);
+        else launch_su2_from_aos_range(prec, links, g, stage, (int)s0, (int)ns, st);
+        if (hipGetLastError() != hipSuccess) return YM_HIP_ERROR;
+    }
+    return hipStreamSynchronize(st) == hipSuccess ? YM_OK : YM_HIP_ERROR;
+}
+
+int32_t download_aos_staged(int group, int prec, const void *links, const Geom &g, double *aos, double *stage,
+                            int64_t stage_sites, hipStream_t st) {
+    const int nc = group == 3 ? 18 : 4;
+    for (int64_t s0 = 0; s0 < g.volume; s0 += stage_sites) {
+        const int64_t ns = std::min<int64_t>(stage_sites, g.volume - s0);
+        if (group == 3) launch_su3_to_aos_range(prec, links, g, stage, (int)s0, (int)ns, st);
+        else launch_su2_to_aos_range(prec, links, g, stage, (int)s0, (int)ns, st);
+        if (hipGetLastError() != hipSuccess) return YM_HIP_ERROR;
+        if (hipMemcpyAsync(aos + s0 * 4 * nc, stage, sizeof(double) * (size_t)ns * 4 * nc, hipMemcpyDeviceToHost,
+                           st) != hipSuccess)
+            return YM_HIP_ERROR;
+    }
+    return hipStreamSynchronize(st) == hipSuccess ? YM_OK : YM_HIP_ERROR;
+}
+
+double link_check_tolerance(int group, int prec) {
+    if (prec == 32) return 0x1p-16;
+    return group == 3 ? 5.0e-12 : 2.0e-12;
+}
+
+namespace {
+// Su3::from_checked_matrix (su3.rs:242-253): finite entries, then
+// max |U^+U - 1| and |UU^+ - 1| entrywise (Mat3::unitarity_residual,
+// su3.rs:219-223), then |det U - 1| (determinant_residual, su3.rs:225-229).
+int32_t check_su3_link(const double *m, double tol) {
+    for (int c = 0; c < 18; ++c)
+        if (!std::isfinite(m[c])) return YM_SU3_NON_FINITE_MATRIX;
+    typedef std::complex<double> cd;
+    cd u[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) u[i][j] = cd(m[(3 * i + j) * 2], m[(3 * i + j) * 2 + 1]);
+    double r = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            cd a = 0.0, b = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                a += std::conj(u[k][i]) * u[k][j];
+                b += u[i][k] * std::conj(u[j][k]);
+            }
+            const double d = (i == j) ? 1.0 : 0.0;
+            r = std::max(r, std::max(std::abs(a - d), std::abs(b - d)));
+        }
+    if (!(r <= tol)) return YM_SU3_NON_UNITARY;
+    const cd det = u[0][0] * (u[1][1] * u[2][2] - u[1][2] * u[2][1]) -
+                   u[0][1] * (u[1][0] * u[2][2] - u[1][2] * u[2][0]) +
+                   u[0][2] * (u[1][0] * u[2][1] - u[1][1] * u[2][0]);
+    if (!(std::abs(det - 1.0) <= tol)) return YM_SU3_DETERMINANT_NOT_ONE;
+    return YM_OK;
+}
+// Su2 values (lib.rs:155-180): finite components, non-zero norm, unit norm
+// and every component within 1 + tol.
+int32_t check_su2_link(const double *q, double tol) {
+    for (int c = 0; c < 4; ++c)
+        if (!std::isfinite(q[c])) return YM_SU2_NON_FINITE;
+    const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (n2 == 0.0) return YM_SU2_ZERO_NORM;
+    for (int c = 0; c < 4; ++c)
+        if (std::fabs(q[c]) > 1.0 + tol) return YM_SU2_NON_UNIT;
+    if (!(std::fabs(n2 - 1.0) <= tol)) return YM_SU2_NON_UNIT;
+    return YM_OK;
+}
+}  // namespace
+
+int32_t validate_aos(int group, const double *aos, int64_t nlinks, double tol) {
+    const int nc = group == 3 ? 18 : 4;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::max(1u, std::min(nt, 16u));
+    if (nlinks < (1 << 16)) nt = 1;
+    std::vector<int64_t> first(nt, INT64_MAX);
+    std::vector<int32_t> code(nt, YM_OK);
+    auto work = [&](unsigned t) {
+        const int64_t lo = nlinks * (int64_t)t / nt, hi = nlinks * (int64_t)(t + 1) / nt;
+        for (int64_t i = lo; i < hi; ++i) {
+            const int32_t c = group == 3 ? check_su3_link(aos + i * nc, tol) : check_su2_link(aos + i * nc, tol);
+            if (c) {
+                first[t] = i;
+                code[t] = c;
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    } catch (...) {
+        for (auto &th : pool) th.join();
+        return YM_ALLOCATION_FAILED;
+    }
+    work(0);
+    for (auto &th : pool) th.join();
+    int64_t best = INT64_MAX;
+    int32_t st = YM_OK;
+    for (unsigned t = 0; t < nt; ++t)
+        if (code[t] && first[t] < best) {
+            best = first[t];
+            st = code[t];
+        }
+    return st;
+}
+
 }  // namespace ym
 
 struct ym_lattice {
@@ -59,6 +181,16 @@ struct ym_lattice {
     int *dstatus = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t aux0 = nullptr, aux1 = nullptr;  // timing of the learned sweeps
+    // Device memory created on first use and kept until destroy, so no
+    // call after the first allocates: the host-layout staging buffer
+    // (stage_sites sites of AoS links), a second link field (stout
+    // ping-pong, autotuning) and a byte buffer for path steps and result
+    // vectors longer than dres.
+    double *stage = nullptr;
+    int64_t stage_sites = 0;
+    void *spare = nullptr;
+    void *aux = nullptr;
+    size_t aux_cap = 0;
 };
 
 namespace ym {
@@ -97,6 +229,40 @@ int32_t ensure_part(ym_lattice *l, size_t n) {
     l->part_cap = 0;
     if (hipMalloc((void **)&l->part, n * sizeof(double)) != hipSuccess) return YM_ALLOCATION_FAILED;
     l->part_cap = n;
+    return YM_OK;
+}
+
+int32_t ensure_stage(ym_lattice *l) {
+    if (l->stage) return YM_OK;
+    const int64_t sites = ym::stage_sites_for(l->g.volume);
+    if (hipMalloc((void **)&l->stage, sizeof(double) * (size_t)sites * 4 * l->ncomp) != hipSuccess) {
+        l->stage = nullptr;
+        return YM_ALLOCATION_FAILED;
+    }
+    l->stage_sites = sites;
+    return YM_OK;
+}
+
+int32_t ensure_spare(ym_lattice *l) {
+    if (l->spare) return YM_OK;
+    if (hipMalloc(&l->spare, l->link_bytes) != hipSuccess) {
+        l->spare = nullptr;
+        return YM_ALLOCATION_FAILED;
+    }
+    return YM_OK;
+}
+
+int32_t ensure_aux(ym_lattice *l, size_t bytes) {
+    if (bytes <= l->aux_cap) return YM_OK;
+    if (l->aux) {
+        (void)hipStreamSynchronize(l->stream);
+        (void)hipFree(l->aux);
+    }
+    l->aux = nullptr;
+    l->aux_cap = 0;
+    const size_t cap = std::max(bytes, (size_t)4096);
+    if (hipMalloc(&l->aux, cap) != hipSuccess) return YM_ALLOCATION_FAILED;
+    l->aux_cap = cap;
     return YM_OK;
 }
 
@@ -214,6 +380,10 @@ const char *ym_status_string(int32_t s) {
     case YM_VAR_PROJECTION_NORMALIZATION: return "projected reference correlator is not normalized";
     case YM_WRONG_CONFIGURATION: return "schedule does not admit the half split";
     case YM_STATIONARITY_FAILED: return "stationarity gate failed (constant half or zero spread)";
+    case YM_SU2_ZERO_NORM: return "SU(2) link has zero norm";
+    case YM_SU3_NON_FINITE_MATRIX: return "SU(3) link has a non-finite entry";
+    case YM_SU3_NON_UNITARY: return "SU(3) link violates the unitarity tolerance";
+    case YM_SU3_DETERMINANT_NOT_ONE: return "SU(3) link determinant differs from one beyond the tolerance";
     case YM_WORKSPACE_EXCEEDED: return "input exceeds the fixed device workspace of an allocation-free entry point";
     case YM_INVALID_ARGUMENT: return "invalid argument";
     case YM_NO_DEVICE: return "no HIP device";
@@ -291,6 +461,9 @@ int32_t ym_lattice_destroy(ym_lattice *l) {
     if (l->ev1) (void)hipEventDestroy(l->ev1);
     if (l->aux0) (void)hipEventDestroy(l->aux0);
     if (l->aux1) (void)hipEventDestroy(l->aux1);
+    if (l->stage) (void)hipFree(l->stage);
+    if (l->spare) (void)hipFree(l->spare);
+    if (l->aux) (void)hipFree(l->aux);
     if (l->stream) (void)hipStreamDestroy(l->stream);
     delete l;
     return YM_OK;
@@ -304,14 +477,9 @@ int32_t ym_lattice_bytes(const ym_lattice *l, int64_t *bytes) {
 
 int32_t ym_lattice_set_identity(ym_lattice *l) {
     if (!l) return YM_INVALID_ARGUMENT;
-    // An identity AoS image is uploaded through the layout kernel.
-    const size_t count = (size_t)l->g.volume * 4 * l->ncomp;
-    std::vector<double> h(count, 0.0);
-    for (size_t k = 0; k < (size_t)l->g.volume * 4; ++k) {
-        if (l->ncomp == 18) { h[k * 18 + 0] = 1.0; h[k * 18 + 8] = 1.0; h[k * 18 + 16] = 1.0; }
-        else h[k * 4] = 1.0;
-    }
-    return ym_lattice_upload(l, h.data(), (int64_t)count);
+    if (l->group == 3) ym::launch_su3_identity(l->prec, l->links, l->g, l->stream);
+    else ym::launch_su2_identity(l->prec, l->links, l->g, l->stream);
+    return finish(l);
 }
 
 int32_t ym_lattice_hot_start(ym_lattice *l, uint64_t seed) {
@@ -321,34 +489,29 @@ int32_t ym_lattice_hot_start(ym_lattice *l, uint64_t seed) {
     return finish(l);
 }
 
+int32_t ym_lattice_upload_unchecked(ym_lattice *l, const double *aos, int64_t count) {
+    if (!l || !aos) return YM_INVALID_ARGUMENT;
+    if (count != (int64_t)l->g.volume * 4 * l->ncomp) return YM_INVALID_ARGUMENT;
+    int32_t st = ensure_stage(l);
+    if (st) return st;
+    return ym::upload_aos_staged(l->group, l->prec, l->links, l->g, aos, l->stage, l->stage_sites, l->stream);
+}
+
 int32_t ym_lattice_upload(ym_lattice *l, const double *aos, int64_t count) {
     if (!l || !aos) return YM_INVALID_ARGUMENT;
     if (count != (int64_t)l->g.volume * 4 * l->ncomp) return YM_INVALID_ARGUMENT;
-    double *d = nullptr;
-    if (hipMalloc((void **)&d, sizeof(double) * count) != hipSuccess) return YM_ALLOCATION_FAILED;
-    int32_t st = hip_status(hipMemcpyAsync(d, aos, sizeof(double) * count, hipMemcpyHostToDevice, l->stream));
-    if (!st) {
-        if (l->group == 3) ym::launch_su3_from_aos(l->prec, l->links, l->g, d, l->stream);
-        else ym::launch_su2_from_aos(l->prec, l->links, l->g, d, l->stream);
-        st = finish(l);
-    }
-    (void)hipFree(d);
-    return st;
+    const int32_t st = ym::validate_aos(l->group, aos, count / l->ncomp, ym::link_check_tolerance(l->group, l->prec));
+    if (st) return st;
+    return ym_lattice_upload_unchecked(l, aos, count);
 }
 
 int32_t ym_lattice_download(const ym_lattice *lc, double *aos, int64_t count) {
     ym_lattice *l = const_cast<ym_lattice *>(lc);
     if (!l || !aos) return YM_INVALID_ARGUMENT;
     if (count != (int64_t)l->g.volume * 4 * l->ncomp) return YM_INVALID_ARGUMENT;
-    double *d = nullptr;
-    if (hipMalloc((void **)&d, sizeof(double) * count) != hipSuccess) return YM_ALLOCATION_FAILED;
-    if (l->group == 3) ym::launch_su3_to_aos(l->prec, l->links, l->g, d, l->stream);
-    else ym::launch_su2_to_aos(l->prec, l->links, l->g, d, l->stream);
-    int32_t st = hip_status(hipGetLastError());
-    if (!st) st = hip_status(hipMemcpyAsync(aos, d, sizeof(double) * count, hipMemcpyDeviceToHost, l->stream));
-    if (!st) st = finish(l);
-    (void)hipFree(d);
-    return st;
+    int32_t st = ensure_stage(l);
+    if (st) return st;
+    return ym::download_aos_staged(l->group, l->prec, l->links, l->g, aos, l->stage, l->stage_sites, l->stream);
 }
 
 int32_t ym_lattice_copy(ym_lattice *dst, const ym_lattice *src) {
@@ -397,9 +560,10 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
     // the sweeps run on a scratch copy of the field.
     const int mode[9] = {0, 0, 0, 0, 0, 1, 1, 1, 1};
     const int strip[9] = {0, 2, 4, 8, 16, 1, 2, 4, 8};
-    void *scratch = nullptr;
-    if (hipMalloc(&scratch, l->link_bytes) != hipSuccess) return YM_ALLOCATION_FAILED;
-    int32_t st = hip_status(hipMemcpyAsync(scratch, l->links, l->link_bytes, hipMemcpyDeviceToDevice, l->stream));
+    int32_t st = ensure_spare(l);
+    if (st) return st;
+    void *scratch = l->spare;
+    st = hip_status(hipMemcpyAsync(scratch, l->links, l->link_bytes, hipMemcpyDeviceToDevice, l->stream));
     float best = 0.0f;
     int bm = l->g.tmode, bs = l->g.t1;
     for (int c = 0; c < 9 && !st; ++c) {
@@ -461,7 +625,6 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
         if (!st && us < best) kper = 1;
     }
     if (!st) st = finish(l);
-    (void)hipFree(scratch);
     if (!st) {
         l->g.tmode = bm;
         l->g.t1 = bs;
@@ -680,14 +843,12 @@ int32_t ym_path_holonomy(const ym_lattice *lc, const int64_t start[4], const int
     }
     for (int64_t k = 0; k < nsteps; ++k)
         if (steps[k] == 0 || steps[k] > 4 || steps[k] < -4) return YM_DIRECTION_OUT_OF_RANGE;
-    int *dsteps = nullptr;
-    double *dout = nullptr;
-    if (hipMalloc((void **)&dsteps, sizeof(int) * nsteps) != hipSuccess) return YM_ALLOCATION_FAILED;
-    if (hipMalloc((void **)&dout, sizeof(double) * 22) != hipSuccess) {
-        (void)hipFree(dsteps);
-        return YM_ALLOCATION_FAILED;
-    }
-    int32_t st = hip_status(hipMemcpyAsync(dsteps, steps, sizeof(int) * nsteps, hipMemcpyHostToDevice, l->stream));
+    // aux: 22 doubles of result (matrix + end point), then the steps
+    int32_t st = ensure_aux(l, sizeof(double) * 22 + sizeof(int) * (size_t)nsteps);
+    if (st) return st;
+    double *dout = (double *)l->aux;
+    int *dsteps = (int *)(dout + 22);
+    st = hip_status(hipMemcpyAsync(dsteps, steps, sizeof(int) * nsteps, hipMemcpyHostToDevice, l->stream));
     if (!st) {
         ym::launch_path_holonomy(l->group, l->prec, l->links, l->g, xi, dsteps, (int)nsteps, dout, l->stream);
         st = hip_status(hipGetLastError());
@@ -695,8 +856,6 @@ int32_t ym_path_holonomy(const ym_lattice *lc, const int64_t start[4], const int
     double h[22];
     if (!st) st = hip_status(hipMemcpyAsync(h, dout, sizeof(h), hipMemcpyDeviceToHost, l->stream));
     if (!st) st = finish(l);
-    (void)hipFree(dsteps);
-    (void)hipFree(dout);
     if (st) return st;
     for (int e = 0; e < l->ncomp; ++e) out[e] = h[e];
     if (end)
@@ -729,12 +888,12 @@ int32_t ym_wilson_loops(const ym_lattice *lc, int32_t rmax, int32_t tmax, int32_
         ym::launch_wilson_loops(l->group, l->prec, l->links, l->g, planes[k][0], planes[k][1], rmax, tmax, l->part, l->stream);
     std::vector<double> sums(ncols);
     if (ncols > 64) {
-        double *dout = nullptr;
-        if (hipMalloc((void **)&dout, sizeof(double) * ncols) != hipSuccess) return YM_ALLOCATION_FAILED;
+        st = ensure_aux(l, sizeof(double) * ncols);
+        if (st) return st;
+        double *dout = (double *)l->aux;
         ym::launch_sum_partials(l->part, nblk, ncols, nblk, dout, l->stream);
         st = hip_status(hipMemcpyAsync(sums.data(), dout, sizeof(double) * ncols, hipMemcpyDeviceToHost, l->stream));
         if (!st) st = finish(l);
-        (void)hipFree(dout);
     } else {
         st = sum_partials(l, nblk, ncols, sums.data());
     }
@@ -779,14 +938,14 @@ int32_t ym_su3_timeslice_a1pp(const ym_lattice *lc, double *plaq_t, double *rect
     const size_t npart = (size_t)nblk * 2 * T;
     int32_t st = ensure_part(l, npart + 2 * (size_t)l->g.volume);
     if (st) return st;
+    st = ensure_aux(l, sizeof(double) * 2 * T);
+    if (st) return st;
+    double *dout = (double *)l->aux;
     ym::launch_a1pp(l->prec, l->links, l->g, l->part + npart, l->part, l->stream);
-    double *dout = nullptr;
-    if (hipMalloc((void **)&dout, sizeof(double) * 2 * T) != hipSuccess) return YM_ALLOCATION_FAILED;
     ym::launch_sum_partials(l->part, nblk, 2 * T, nblk, dout, l->stream);
     std::vector<double> s(2 * T);
     st = hip_status(hipMemcpyAsync(s.data(), dout, sizeof(double) * 2 * T, hipMemcpyDeviceToHost, l->stream));
     if (!st) st = finish(l);
-    (void)hipFree(dout);
     if (st) return st;
     for (int t = 0; t < T; ++t) {
         plaq_t[t] = s[2 * t] / (3.0 * nsp);
@@ -809,8 +968,11 @@ int32_t ym_su3_stout_spatial(ym_lattice *dst, const ym_lattice *src, double rho,
     if (dst == src) return YM_INVALID_ARGUMENT;
     YM_HIP(hipStreamSynchronize(src->stream));
     if (levels == 0) return ym_lattice_copy(dst, src);
-    void *tmp = nullptr;
-    if (levels > 1 && hipMalloc(&tmp, src->link_bytes) != hipSuccess) return YM_ALLOCATION_FAILED;
+    if (levels > 1) {
+        const int32_t e = ensure_spare(dst);
+        if (e) return e;
+    }
+    void *tmp = dst->spare;
     // Level k writes dst when (levels-1-k) is even, tmp otherwise, so the
     // buffers alternate and the last level lands in dst.
     const void *cur = src->links;
@@ -819,9 +981,7 @@ int32_t ym_su3_stout_spatial(ym_lattice *dst, const ym_lattice *src, double rho,
         ym::launch_stout(src->prec, cur, target, src->g, rho, dst->stream);
         cur = target;
     }
-    int32_t st = finish(dst);
-    if (tmp) (void)hipFree(tmp);
-    return st;
+    return finish(dst);
 }
 
 int32_t ym_su3_blocked_extents(const ym_lattice *src, int64_t coarse[4]) {
@@ -874,19 +1034,8 @@ int32_t ym_reunitarize(ym_lattice *l) {
         ym::launch_su3_reunit(l->prec, l->links, l->g, l->stream);
         return finish(l);
     }
-    // SU(2): one over-relaxation-free normalisation pass via download/upload.
-    const size_t count = (size_t)l->g.volume * 16;
-    std::vector<double> h(count);
-    int32_t st = ym_lattice_download(l, h.data(), (int64_t)count);
-    if (st) return st;
-    for (size_t k = 0; k < count / 4; ++k) {
-        double *q = &h[4 * k];
-        double n = q[0] * q[0];
-        for (int c = 1; c < 4; ++c) n = std::fma(q[c], q[c], n);
-        const double inv = 1.0 / std::sqrt(n);
-        for (int c = 0; c < 4; ++c) q[c] *= inv;
-    }
-    return ym_lattice_upload(l, h.data(), (int64_t)count);
+    ym::launch_su2_normalize(l->prec, l->links, l->g, l->stream);
+    return finish(l);
 }
 
 int32_t ym_timer_begin(ym_lattice *l) {
@@ -962,6 +1111,19 @@ int32_t ym_diagnostics(const double *v, int64_t n, int64_t max_lag, ym_chain_dia
     return YM_OK;
 }
 
+// Sweeps counted in int64 by the chain drivers run in int32-sized calls.
+static int32_t su3_sweeps64(ym_lattice *l, const ym_su3_chain_config *c, uint64_t key, uint64_t first, int64_t n,
+                            uint64_t *prop, uint64_t *acc) {
+    int32_t st = YM_OK;
+    for (int64_t done = 0; done < n && !st;) {
+        const int32_t k = (int32_t)std::min<int64_t>(n - done, (int64_t)1 << 30);
+        st = ym_su3_metropolis_sweeps(l, c->beta, c->proposal_angle, c->hits, key, first + (uint64_t)done, k,
+                                      c->reunitarize, prop, acc);
+        done += k;
+    }
+    return st;
+}
+
 int32_t ym_su3_run_chain(const ym_su3_chain_config *c, double *meas, ym_su3_chain_report *rep) {
     if (!c || !meas || !rep) return YM_INVALID_ARGUMENT;
     if (!std::isfinite(c->beta) || c->beta < 0.0 || c->beta > 12.0) return YM_BETA_OUT_OF_RANGE;
@@ -970,20 +1132,21 @@ int32_t ym_su3_run_chain(const ym_su3_chain_config *c, double *meas, ym_su3_chai
     if (c->sweeps_between_measurements <= 0) return YM_ZERO_STRIDE;
     if (c->autocorrelation_max_lag < 0 || c->autocorrelation_max_lag >= c->measurements) return YM_LAG_OUT_OF_RANGE;
     if (c->thermalization_sweeps < 0 || c->hits < 1) return YM_SCHEDULE_OUT_OF_RANGE;
+    // the sweep index (a Philox counter word) must not wrap: total < 2^62
+    const int64_t lim = (int64_t)1 << 62;
+    if (c->sweeps_between_measurements > lim / c->measurements ||
+        c->thermalization_sweeps > lim - c->measurements * c->sweeps_between_measurements)
+        return YM_SCHEDULE_OUT_OF_RANGE;
     ym_lattice *l = nullptr;
     int32_t st = ym_lattice_create(3, c->extents, c->precision, &l);
     if (st) return st;
     const uint64_t key = ym_derive_su3_chain_seed(c->master_seed, c->chain_id);
     if (c->hot_start) st = ym_lattice_hot_start(l, key);
     uint64_t prop = 0, acc = 0, sweep = 0;
-    if (!st && c->thermalization_sweeps > 0) {
-        st = ym_su3_metropolis_sweeps(l, c->beta, c->proposal_angle, c->hits, key, sweep, (int32_t)c->thermalization_sweeps,
-                                      c->reunitarize, &prop, &acc);
-        sweep += (uint64_t)c->thermalization_sweeps;
-    }
+    if (!st) st = su3_sweeps64(l, c, key, sweep, c->thermalization_sweeps, &prop, &acc);
+    sweep += (uint64_t)c->thermalization_sweeps;
     for (int64_t m = 0; !st && m < c->measurements; ++m) {
-        st = ym_su3_metropolis_sweeps(l, c->beta, c->proposal_angle, c->hits, key, sweep,
-                                      (int32_t)c->sweeps_between_measurements, c->reunitarize, &prop, &acc);
+        st = su3_sweeps64(l, c, key, sweep, c->sweeps_between_measurements, &prop, &acc);
         sweep += (uint64_t)c->sweeps_between_measurements;
         if (!st) st = ym_average_plaquette(l, &meas[m]);
         if (!st && (meas[m] < -0.5 - 1e-12 || meas[m] > 1.0 + 1e-12)) st = YM_NON_FINITE_STATISTIC;
@@ -996,6 +1159,94 @@ int32_t ym_su3_run_chain(const ym_su3_chain_config *c, double *meas, ym_su3_chai
         rep->proposals = prop;
         rep->accepts = acc;
         rep->mean_acceptance = prop ? (double)acc / (double)prop : 0.0;
+    }
+    ym_lattice_destroy(l);
+    return st;
+}
+
+int32_t ym_su2_markov_sweeps(ym_lattice *l, double beta, uint64_t key, uint64_t first_sweep, int32_t n_sweeps,
+                             int32_t overrelaxation_sweeps_per_heatbath, int32_t max_attempts, uint64_t *proposals,
+                             uint64_t *updates) {
+    if (!l || l->group != 2 || n_sweeps < 0) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(beta) || beta < 0.0 || beta > 8.0) return YM_BETA_OUT_OF_RANGE;
+    if (overrelaxation_sweeps_per_heatbath < 0 || overrelaxation_sweeps_per_heatbath > 16) return YM_TOO_MANY_OR;
+    if (max_attempts <= 0) return YM_ZERO_ATTEMPT_LIMIT;
+    if (!all_even(l)) return YM_UNSUPPORTED;
+    ym::launch_fill_zero_u64(l->dctr, 2, l->stream);
+    ym::launch_fill_zero_i32(l->dstatus, 1, l->stream);
+    for (int k = 0; k < n_sweeps; ++k) {
+        for (int mu = 0; mu < 4; ++mu)
+            for (int p = 0; p < 2; ++p)
+                ym::launch_su2_heatbath(l->prec, l->links, l->g, mu, p, beta, key, first_sweep + (uint64_t)k,
+                                        max_attempts, l->dctr, l->dstatus, l->stream);
+        for (int o = 0; o < overrelaxation_sweeps_per_heatbath; ++o)
+            for (int mu = 0; mu < 4; ++mu)
+                for (int p = 0; p < 2; ++p) ym::launch_su2_overrelax(l->prec, l->links, l->g, mu, p, l->stream);
+    }
+    YM_HIP(hipGetLastError());
+    unsigned long long ctr[2] = {0, 0};
+    int status = 0;
+    YM_HIP(hipMemcpyAsync(ctr, l->dctr, sizeof(ctr), hipMemcpyDeviceToHost, l->stream));
+    YM_HIP(hipMemcpyAsync(&status, l->dstatus, sizeof(int), hipMemcpyDeviceToHost, l->stream));
+    const int32_t st = finish(l);
+    if (st) return st;
+    if (proposals) *proposals += ctr[0];
+    if (updates) *updates += ctr[1];
+    return status ? YM_HEATBATH_ATTEMPT_LIMIT : YM_OK;
+}
+
+int32_t ym_su2_run_chain(const ym_su2_chain_config *c, double *meas, ym_su2_chain_report *rep) {
+    if (!c || !meas || !rep) return YM_INVALID_ARGUMENT;
+    // EnsembleConfig::validate (su2_ensemble.rs:70-103), in its order
+    if (!std::isfinite(c->beta) || c->beta < 0.0 || c->beta > 8.0) return YM_BETA_OUT_OF_RANGE;
+    const int64_t max_sweeps = (int64_t)1 << 20;
+    if (c->measurements < 0 || c->sweeps_between_measurements < 0 || c->thermalization_sweeps < 0 ||
+        c->overrelaxation_sweeps_per_heatbath < 0 || c->max_heatbath_attempts_per_link < 0 ||
+        c->autocorrelation_max_lag < 0)
+        return YM_SCHEDULE_OUT_OF_RANGE;
+    if (c->measurements > max_sweeps) return YM_SCHEDULE_OUT_OF_RANGE;
+    if (c->sweeps_between_measurements > 0 && c->measurements > max_sweeps / c->sweeps_between_measurements)
+        return YM_SCHEDULE_OUT_OF_RANGE;
+    if (c->thermalization_sweeps > max_sweeps - c->measurements * c->sweeps_between_measurements)
+        return YM_SCHEDULE_OUT_OF_RANGE;
+    if (c->measurements < 2) return YM_MEASUREMENT_COUNT;
+    if (c->sweeps_between_measurements == 0) return YM_ZERO_STRIDE;
+    if (c->overrelaxation_sweeps_per_heatbath > 16) return YM_TOO_MANY_OR;
+    if (c->max_heatbath_attempts_per_link == 0) return YM_ZERO_ATTEMPT_LIMIT;
+    if (c->autocorrelation_max_lag >= c->measurements) return YM_LAG_OUT_OF_RANGE;
+    if (c->precision != 32 && c->precision != 64) return YM_INVALID_ARGUMENT;
+    // a link needing more than 2^31 - 1 attempts is not a realistic event; the
+    // kernels count attempts in int32
+    const int32_t max_att = (int32_t)std::min<int64_t>(c->max_heatbath_attempts_per_link, 0x7fffffff);
+    const int32_t nor = (int32_t)c->overrelaxation_sweeps_per_heatbath;
+    ym_lattice *l = nullptr;
+    int32_t st = ym_lattice_create(2, c->extents, c->precision, &l);
+    if (st) return st;
+    const uint64_t key = ym_derive_chain_seed(c->master_seed, c->chain_id);
+    if (c->hot_start) st = ym_lattice_hot_start(l, key);
+    uint64_t prop = 0, upd = 0, sweep = 0;
+    if (!st && c->thermalization_sweeps > 0)
+        st = ym_su2_markov_sweeps(l, c->beta, key, sweep, (int32_t)c->thermalization_sweeps, nor, max_att, &prop, &upd);
+    sweep += (uint64_t)c->thermalization_sweeps;
+    for (int64_t m = 0; !st && m < c->measurements; ++m) {
+        st = ym_su2_markov_sweeps(l, c->beta, key, sweep, (int32_t)c->sweeps_between_measurements, nor, max_att,
+                                  &prop, &upd);
+        sweep += (uint64_t)c->sweeps_between_measurements;
+        if (!st) st = ym_average_plaquette(l, &meas[m]);
+    }
+    if (!st) st = ym_diagnostics(meas, c->measurements, c->autocorrelation_max_lag, &rep->diagnostics, nullptr);
+    double unit = 0.0, det = 0.0;
+    if (!st) st = ym_link_residuals(l, &unit, &det);
+    if (!st) {
+        const double acc = (double)upd / (double)prop;
+        if (!std::isfinite(acc) || !std::isfinite(unit)) st = YM_NON_FINITE_STATISTIC;
+        rep->derived_seed = key;
+        rep->total_heatbath_sweeps = (int64_t)sweep;
+        rep->total_overrelaxation_sweeps = (int64_t)sweep * nor;
+        rep->heatbath_proposals = prop;
+        rep->heatbath_updates = upd;
+        rep->mean_heatbath_acceptance = acc;
+        rep->final_link_unit_residual = unit;
     }
     ym_lattice_destroy(l);
     return st;

@@ -348,6 +348,8 @@ struct ym_ensemble {
     uint64_t sweep = 0;         // host mirror of ctr[0]
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *stage = nullptr;   // host-layout staging buffer, created on first transfer
+    int64_t stage_sites = 0;
 };
 
 namespace {
@@ -536,7 +538,7 @@ int32_t ym_ensemble_destroy(ym_ensemble *E) {
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     void *bufs[] = {E->links, E->tmp[0], E->tmp[1], E->d_cd, E->d_metro, E->d_tile_chain, E->d_plaq, E->d_stout,
                     E->d_a1pp,
-                    E->part, E->sv, E->d_acc, E->d_ctr};
+                    E->part, E->sv, E->d_acc, E->d_ctr, E->stage};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (E->ev0) (void)hipEventDestroy(E->ev0);
@@ -568,19 +570,32 @@ int32_t ym_ensemble_chain_count(const ym_ensemble *E, int32_t *nchains, int64_t 
     return YM_OK;
 }
 
+// Staging buffer for one chain's host image: sized for the largest chain
+// (at most 2^16 sites), created on the first transfer and kept.
+static int32_t ens_ensure_stage(ym_ensemble *E) {
+    if (E->stage) return YM_OK;
+    int64_t vmax = 0;
+    for (const auto &c : E->h_cd) vmax = std::max<int64_t>(vmax, c.g.volume);
+    const int64_t sites = ym::stage_sites_for(vmax);
+    if (hipMalloc((void **)&E->stage, sizeof(double) * (size_t)sites * 4 * 18) != hipSuccess) {
+        E->stage = nullptr;
+        return YM_ALLOCATION_FAILED;
+    }
+    E->stage_sites = sites;
+    return YM_OK;
+}
+
+// Checked as ym_lattice_upload (Su3::from_checked_matrix, su3.rs:242-253).
 int32_t ym_ensemble_upload_chain(ym_ensemble *E, int32_t chain, const double *aos, int64_t count) {
     if (!E || !aos || chain < 0 || chain >= E->nch) return YM_INVALID_ARGUMENT;
     const ym::ChainDesc &d = E->h_cd[chain];
     if (count != (int64_t)d.g.volume * 4 * 18) return YM_INVALID_ARGUMENT;
-    double *dev = nullptr;
-    if (hipMalloc((void **)&dev, sizeof(double) * count) != hipSuccess) return YM_ALLOCATION_FAILED;
-    int32_t st = hst(hipMemcpyAsync(dev, aos, sizeof(double) * count, hipMemcpyHostToDevice, E->stream));
+    int32_t st = ym::validate_aos(3, aos, count / 18, ym::link_check_tolerance(3, E->prec));
+    if (!st) st = ens_ensure_stage(E);
+    if (st) return st;
     const size_t pb = 2 * (size_t)(E->prec / 8);
-    if (!st) ym::launch_su3_from_aos(E->prec, (char *)E->links + d.link_off * pb, d.g, dev, E->stream);
-    if (!st) st = hst(hipGetLastError());
-    if (!st) st = hst(hipStreamSynchronize(E->stream));
-    (void)hipFree(dev);
-    return st;
+    return ym::upload_aos_staged(3, E->prec, (char *)E->links + d.link_off * pb, d.g, aos, E->stage, E->stage_sites,
+                                 E->stream);
 }
 
 int32_t ym_ensemble_download_chain(const ym_ensemble *Ec, int32_t chain, double *aos, int64_t count) {
@@ -588,15 +603,11 @@ int32_t ym_ensemble_download_chain(const ym_ensemble *Ec, int32_t chain, double 
     if (!E || !aos || chain < 0 || chain >= E->nch) return YM_INVALID_ARGUMENT;
     const ym::ChainDesc &d = E->h_cd[chain];
     if (count != (int64_t)d.g.volume * 4 * 18) return YM_INVALID_ARGUMENT;
-    double *dev = nullptr;
-    if (hipMalloc((void **)&dev, sizeof(double) * count) != hipSuccess) return YM_ALLOCATION_FAILED;
+    const int32_t st = ens_ensure_stage(E);
+    if (st) return st;
     const size_t pb = 2 * (size_t)(E->prec / 8);
-    ym::launch_su3_to_aos(E->prec, (const char *)E->links + d.link_off * pb, d.g, dev, E->stream);
-    int32_t st = hst(hipGetLastError());
-    if (!st) st = hst(hipMemcpyAsync(aos, dev, sizeof(double) * count, hipMemcpyDeviceToHost, E->stream));
-    if (!st) st = hst(hipStreamSynchronize(E->stream));
-    (void)hipFree(dev);
-    return st;
+    return ym::download_aos_staged(3, E->prec, (const char *)E->links + d.link_off * pb, d.g, aos, E->stage,
+                                   E->stage_sites, E->stream);
 }
 
 int32_t ym_ensemble_hot_start(ym_ensemble *E) {

@@ -33,6 +33,19 @@ class Su3ChainReport(C.Structure):
                 ("final_determinant_residual", dbl), ("diagnostics", ChainDiagnostics)]
 
 
+class Su2ChainConfig(C.Structure):
+    _fields_ = [("extents", i64 * 4), ("beta", dbl), ("master_seed", u64), ("chain_id", u64),
+                ("thermalization_sweeps", i64), ("measurements", i64), ("sweeps_between_measurements", i64),
+                ("overrelaxation_sweeps_per_heatbath", i64), ("max_heatbath_attempts_per_link", i64),
+                ("autocorrelation_max_lag", i64), ("precision", i32), ("hot_start", i32)]
+
+
+class Su2ChainReport(C.Structure):
+    _fields_ = [("derived_seed", u64), ("total_heatbath_sweeps", i64), ("total_overrelaxation_sweeps", i64),
+                ("heatbath_proposals", u64), ("heatbath_updates", u64), ("mean_heatbath_acceptance", dbl),
+                ("final_link_unit_residual", dbl), ("diagnostics", ChainDiagnostics)]
+
+
 class EfficiencyReport(C.Structure):
     _fields_ = [("diagnostics", ChainDiagnostics), ("reweighting_efficiency", dbl), ("weight_only_ess", dbl),
                 ("combined_ess_proxy", dbl), ("total_cost", dbl), ("sampling_cost_per_ess", dbl),
@@ -69,6 +82,7 @@ SIGNATURES = {
     "ym_lattice_set_identity": (i32, [C.c_void_p]),
     "ym_lattice_hot_start": (i32, [C.c_void_p, u64]),
     "ym_lattice_upload": (i32, [C.c_void_p, P(dbl), i64]),
+    "ym_lattice_upload_unchecked": (i32, [C.c_void_p, P(dbl), i64]),
     "ym_lattice_download": (i32, [C.c_void_p, P(dbl), i64]),
     "ym_lattice_copy": (i32, [C.c_void_p, C.c_void_p]),
     "ym_su3_metropolis_sweeps": (i32, [C.c_void_p, dbl, dbl, i32, u64, u64, i32, i32, P(u64), P(u64)]),
@@ -77,6 +91,8 @@ SIGNATURES = {
     "ym_su3_autotune": (i32, [C.c_void_p, dbl, dbl, i32, i32, P(i32), P(i32), P(flt)]),
     "ym_su2_heatbath_sweeps": (i32, [C.c_void_p, dbl, u64, u64, i32, i32, P(u64), P(u64)]),
     "ym_su2_overrelax_sweeps": (i32, [C.c_void_p, i32]),
+    "ym_su2_markov_sweeps": (i32, [C.c_void_p, dbl, u64, u64, i32, i32, i32, P(u64), P(u64)]),
+    "ym_su2_run_chain": (i32, [P(Su2ChainConfig), P(dbl), P(Su2ChainReport)]),
     "ym_average_plaquette": (i32, [C.c_void_p, P(dbl)]),
     "ym_average_plaquette_noalloc": (i32, [C.c_void_p, P(dbl)]),
     "ym_wilson_action": (i32, [C.c_void_p, dbl, P(dbl)]),

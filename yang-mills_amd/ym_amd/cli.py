@@ -104,29 +104,14 @@ def su3_ensemble(extents=(2, 2, 2, 2), beta=0.0, master_seed=0x3C00198212230101,
 
 def _su2_chain(extents, beta, master_seed, chain_id, thermalization_sweeps, measurements,
                sweeps_between_measurements, overrelaxation_sweeps_per_heatbath, max_attempts, precision):
-    """su2_ensemble.rs run_chain on the device: cold start, Markov sweeps of
-    one heat-bath sweep then `overrelaxation_sweeps_per_heatbath`
+    """su2_ensemble.rs run_chain on the device (ym_su2_run_chain): cold start,
+    Markov sweeps of one heat-bath sweep then `overrelaxation_sweeps_per_heatbath`
     over-relaxation sweeps (su2_ensemble.rs:465-486), the plaquette after
     every `sweeps_between_measurements` sweeps."""
-    key = _lat.derive_chain_seed(master_seed, chain_id)
-    lat = _lat.Lattice4(extents, precision)
-    sweep = prop = upd = 0
-
-    def markov():
-        nonlocal sweep, prop, upd
-        p, u = lat.heatbath_sweeps(beta, key, sweep, 1, max_attempts)
-        prop, upd, sweep = prop + p, upd + u, sweep + 1
-        if overrelaxation_sweeps_per_heatbath:
-            lat.overrelax_sweeps(overrelaxation_sweeps_per_heatbath)
-
-    for _ in range(thermalization_sweeps):
-        markov()
-    meas = np.zeros(measurements)
-    for m in range(measurements):
-        for _ in range(sweeps_between_measurements):
-            markov()
-        meas[m] = lat.average_plaquette()
-    return key, meas, upd / prop, lat.max_link_unit_residual()
+    rep = _lat.su2_run_chain(extents, beta, master_seed, chain_id, thermalization_sweeps, measurements,
+                             sweeps_between_measurements, overrelaxation_sweeps_per_heatbath, max_attempts,
+                             autocorrelation_max_lag=1, precision=precision)
+    return rep.derived_seed, rep.measurements, rep.mean_heatbath_acceptance, rep.final_link_unit_residual
 
 
 def su2_ensemble(extents=(2, 2, 2, 2), beta=0.0, master_seed=0x0123456789ABCDEF, chain_id=0,

@@ -4,7 +4,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import ChainDiagnostics, Su3ChainConfig, Su3ChainReport, check, lib
+from ._lib import ChainDiagnostics, Su2ChainConfig, Su3ChainConfig, Su3ChainReport, check, lib
+from ._lib import Su2ChainReport as _Su2ChainReportC
 
 
 def _ext(extents):
@@ -74,9 +75,15 @@ class _Lattice:
     def hot_start(self, seed):
         check(lib().ym_lattice_hot_start(self._h, int(seed)), "ym_lattice_hot_start")
 
-    def upload(self, aos):
+    def upload(self, aos, checked=True):
+        """Set every link from a host AoS image.  checked=True applies the
+        reference's link checks (finite, unitary, unit determinant / unit
+        norm) and fails closed; checked=False stores the values as given."""
         a = np.ascontiguousarray(aos, dtype=np.float64).reshape(-1)
-        check(lib().ym_lattice_upload(self._h, _dp(a), a.size), "ym_lattice_upload")
+        if checked:
+            check(lib().ym_lattice_upload(self._h, _dp(a), a.size), "ym_lattice_upload")
+        else:
+            check(lib().ym_lattice_upload_unchecked(self._h, _dp(a), a.size), "ym_lattice_upload_unchecked")
 
     def download(self):
         a = np.empty(self.link_count() * self.NCOMP, dtype=np.float64)
@@ -178,6 +185,16 @@ class Lattice4(_Lattice):
 
     def overrelax_sweeps(self, n_sweeps=1):
         check(lib().ym_su2_overrelax_sweeps(self._h, int(n_sweeps)), "ym_su2_overrelax_sweeps")
+
+    def markov_sweeps(self, beta, key, first_sweep, n_sweeps=1, overrelaxation_sweeps_per_heatbath=1,
+                      max_attempts=10_000):
+        """su2_ensemble.rs:465-486 markov_sweep x n: (heat-bath proposals, updates)."""
+        p, u = C.c_uint64(0), C.c_uint64(0)
+        st = lib().ym_su2_markov_sweeps(self._h, float(beta), int(key), int(first_sweep), int(n_sweeps),
+                                        int(overrelaxation_sweeps_per_heatbath), int(max_attempts), C.byref(p),
+                                        C.byref(u))
+        check(st, "ym_su2_markov_sweeps")
+        return p.value, u.value
 
     def to_canonical_config(self):
         """lib.rs:587-614: the YM-SU2-CONFIG-V1 text of this configuration."""
@@ -366,3 +383,43 @@ def run_chain(extents, beta, master_seed, chain_id, thermalization_sweeps, measu
     check(lib().ym_su3_run_chain(C.byref(cfg), _dp(meas), C.byref(rep)), "ym_su3_run_chain")
     return ChainReport(rep.derived_seed, rep.total_sweeps, rep.proposals, rep.accepts, rep.mean_acceptance,
                        meas, rep.diagnostics, rep.final_unitarity_residual, rep.final_determinant_residual)
+
+
+@dataclass
+class Su2ChainReport:
+    """su2_ensemble.rs:173-186 ChainReport."""
+    derived_seed: int
+    total_heatbath_sweeps: int
+    total_overrelaxation_sweeps: int
+    heatbath_proposals: int
+    heatbath_updates: int
+    mean_heatbath_acceptance: float
+    measurements: np.ndarray
+    diagnostics: ChainDiagnostics
+    final_link_unit_residual: float
+
+
+def su2_run_chain(extents, beta, master_seed, chain_id, thermalization_sweeps, measurements,
+                  sweeps_between_measurements, overrelaxation_sweeps_per_heatbath=1,
+                  max_heatbath_attempts_per_link=10_000, autocorrelation_max_lag=12, precision=64,
+                  hot_start=False):
+    """su2_ensemble.rs:488-552 run_chain on the device."""
+    cfg = Su2ChainConfig()
+    cfg.extents[:] = [int(v) for v in extents]
+    cfg.beta = float(beta)
+    cfg.master_seed = int(master_seed)
+    cfg.chain_id = int(chain_id)
+    cfg.thermalization_sweeps = int(thermalization_sweeps)
+    cfg.measurements = int(measurements)
+    cfg.sweeps_between_measurements = int(sweeps_between_measurements)
+    cfg.overrelaxation_sweeps_per_heatbath = int(overrelaxation_sweeps_per_heatbath)
+    cfg.max_heatbath_attempts_per_link = int(max_heatbath_attempts_per_link)
+    cfg.autocorrelation_max_lag = int(autocorrelation_max_lag)
+    cfg.precision = int(precision)
+    cfg.hot_start = 1 if hot_start else 0
+    meas = np.zeros(max(int(measurements), 0))
+    rep = _Su2ChainReportC()
+    check(lib().ym_su2_run_chain(C.byref(cfg), _dp(meas), C.byref(rep)), "ym_su2_run_chain")
+    return Su2ChainReport(rep.derived_seed, rep.total_heatbath_sweeps, rep.total_overrelaxation_sweeps,
+                          rep.heatbath_proposals, rep.heatbath_updates, rep.mean_heatbath_acceptance, meas,
+                          rep.diagnostics, rep.final_link_unit_residual)
