@@ -311,6 +311,19 @@ int ymd_su2_heatbath_samples(const double a[4], double beta, uint64_t key, int m
     return status;
 }
 
+/* Loops over the independent links of one checkerboard sub-pass (and over
+ * sites in the layout changes and hot starts) run on OpenMP threads when the
+ * oracle is built with -fopenmp: each link's arithmetic is unchanged and
+ * counts are integer sums, so results do not depend on the thread count. */
+#define YMD_PRAGMA(x) _Pragma(#x)
+#ifdef _OPENMP
+#define YMD_PARALLEL_FOR YMD_PRAGMA(omp parallel for schedule(static))
+#define YMD_PARALLEL_FOR_REDUCE(...) YMD_PRAGMA(omp parallel for schedule(static) reduction(__VA_ARGS__))
+#else
+#define YMD_PARALLEL_FOR
+#define YMD_PARALLEL_FOR_REDUCE(...)
+#endif
+
 #define REAL float
 #define RSUF f32
 #define RFMA fmaf

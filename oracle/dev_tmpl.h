@@ -27,14 +27,17 @@ static int F(to_device)(const int64_t n[4], const double *aos, void *dev, int nc
     geom g = make_geom(n);
     REAL *L = (REAL *)dev;
     /* padding slots hold the identity */
+    YMD_PARALLEL_FOR
     for (int mu = 0; mu < 4; ++mu) for (int p = 0; p < 2; ++p) for (int c = 0; c < nc; ++c) for (int64_t s = 0; s < g.vh; ++s) {
         REAL v = 0;
         if (nc == 18 && (c == 0 || c == 8 || c == 16)) v = 1;
         if (nc == 4 && c == 0) v = 1;
         *F(lk)(L, &g, mu, p, nc, c, s) = v;
     }
-    int64_t x[4];
-    for (x[0] = 0; x[0] < n[0]; ++x[0]) for (x[1] = 0; x[1] < n[1]; ++x[1]) for (x[2] = 0; x[2] < n[2]; ++x[2]) for (x[3] = 0; x[3] < n[3]; ++x[3]) {
+    YMD_PARALLEL_FOR
+    for (int64_t x0 = 0; x0 < n[0]; ++x0)
+    for (int64_t x1 = 0; x1 < n[1]; ++x1) for (int64_t x2 = 0; x2 < n[2]; ++x2) for (int64_t x3 = 0; x3 < n[3]; ++x3) {
+        const int64_t x[4] = {x0, x1, x2, x3};
         int64_t site = ((x[0] * n[1] + x[1]) * n[2] + x[2]) * n[3] + x[3];
         for (int mu = 0; mu < 4; ++mu) {
             REAL v[18];
@@ -47,8 +50,10 @@ static int F(to_device)(const int64_t n[4], const double *aos, void *dev, int nc
 static int F(from_device)(const int64_t n[4], const void *dev, double *aos, int nc) {
     geom g = make_geom(n);
     const REAL *L = (const REAL *)dev;
-    int64_t x[4];
-    for (x[0] = 0; x[0] < n[0]; ++x[0]) for (x[1] = 0; x[1] < n[1]; ++x[1]) for (x[2] = 0; x[2] < n[2]; ++x[2]) for (x[3] = 0; x[3] < n[3]; ++x[3]) {
+    YMD_PARALLEL_FOR
+    for (int64_t x0 = 0; x0 < n[0]; ++x0)
+    for (int64_t x1 = 0; x1 < n[1]; ++x1) for (int64_t x2 = 0; x2 < n[2]; ++x2) for (int64_t x3 = 0; x3 < n[3]; ++x3) {
+        const int64_t x[4] = {x0, x1, x2, x3};
         int64_t site = ((x[0] * n[1] + x[1]) * n[2] + x[2]) * n[3] + x[3];
         for (int mu = 0; mu < 4; ++mu) {
             REAL v[18];
@@ -347,13 +352,17 @@ static int F(su3_sweep)(const int64_t n[4], void *dev, const ymd_su3_params *P, 
     REAL *L = (REAL *)dev;
     uint64_t acc = 0, prop = 0;
     for (int mu = 0; mu < 4; ++mu)
-        for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p) {
+            /* the links of one sub-pass are independent: any split over
+             * threads gives the same links and the same integer counts */
+            YMD_PARALLEL_FOR_REDUCE(+ : acc, prop)
             for (int64_t s = 0; s < g.vh; ++s) {
                 int64_t x[4];
                 if (!site_of(&g, p, s, x)) continue;
                 F(su3_update_link)(L, &g, x, mu, P, &acc);
                 prop += (uint64_t)P->hits;
             }
+        }
     *proposals = prop;
     *accepts = acc;
     return YMO_OK;
@@ -373,8 +382,10 @@ static inline void F(gauss2)(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1,
 static int F(su3_hot)(const int64_t n[4], void *dev, uint64_t key) {
     geom g = make_geom(n);
     REAL *L = (REAL *)dev;
-    int64_t x[4];
-    for (x[0] = 0; x[0] < n[0]; ++x[0]) for (x[1] = 0; x[1] < n[1]; ++x[1]) for (x[2] = 0; x[2] < n[2]; ++x[2]) for (x[3] = 0; x[3] < n[3]; ++x[3]) {
+    YMD_PARALLEL_FOR
+    for (int64_t x0 = 0; x0 < n[0]; ++x0)
+    for (int64_t x1 = 0; x1 < n[1]; ++x1) for (int64_t x2 = 0; x2 < n[2]; ++x2) for (int64_t x3 = 0; x3 < n[3]; ++x3) {
+        const int64_t x[4] = {x0, x1, x2, x3};
         uint32_t site = lex_of(&g, x);
         for (int mu = 0; mu < 4; ++mu) {
             double gs[12];
@@ -446,8 +457,10 @@ static int F(su2_hb)(const int64_t n[4], void *dev, double beta, uint64_t key, u
     REAL *L = (REAL *)dev;
     uint64_t prop = 0, upd = 0;
     int status = YMO_OK;
+    int failed = 0;
     for (int mu = 0; mu < 4; ++mu)
-        for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p) {
+            YMD_PARALLEL_FOR_REDUCE(+ : prop, upd, failed)
             for (int64_t s = 0; s < g.vh; ++s) {
                 int64_t x[4];
                 if (!site_of(&g, p, s, x)) continue;
@@ -458,11 +471,13 @@ static int F(su2_hb)(const int64_t n[4], void *dev, double beta, uint64_t key, u
                 int64_t att = 0;
                 int ok = ymd_su2_heatbath_link(a, beta, key, lex_of(&g, x), mu, sweep, max_attempts, un, &att);
                 prop += (uint64_t)att;
-                if (!ok) { status = YMO_HEATBATH_ATTEMPT_LIMIT; continue; }
+                if (!ok) { failed += 1; continue; }
                 upd += 1;
                 REAL v[4] = {(REAL)un[0], (REAL)un[1], (REAL)un[2], (REAL)un[3]};
                 F(store_link)(L, &g, x, mu, 4, v);
             }
+        }
+    if (failed) status = YMO_HEATBATH_ATTEMPT_LIMIT;
     *proposals = prop;
     *updates = upd;
     return status;
@@ -473,7 +488,8 @@ static int F(su2_or)(const int64_t n[4], void *dev) {
     if (g.n[0] % 2 || g.n[1] % 2 || g.n[2] % 2 || g.n[3] % 2) return YMO_UNSUPPORTED;
     REAL *L = (REAL *)dev;
     for (int mu = 0; mu < 4; ++mu)
-        for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p) {
+            YMD_PARALLEL_FOR
             for (int64_t s = 0; s < g.vh; ++s) {
                 int64_t x[4];
                 if (!site_of(&g, p, s, x)) continue;
@@ -492,14 +508,17 @@ static int F(su2_or)(const int64_t n[4], void *dev) {
                 REAL v[4] = {(REAL)r[0], (REAL)r[1], (REAL)r[2], (REAL)r[3]};
                 F(store_link)(L, &g, x, mu, 4, v);
             }
+        }
     return YMO_OK;
 }
 
 static int F(su2_hot)(const int64_t n[4], void *dev, uint64_t key) {
     geom g = make_geom(n);
     REAL *L = (REAL *)dev;
-    int64_t x[4];
-    for (x[0] = 0; x[0] < n[0]; ++x[0]) for (x[1] = 0; x[1] < n[1]; ++x[1]) for (x[2] = 0; x[2] < n[2]; ++x[2]) for (x[3] = 0; x[3] < n[3]; ++x[3]) {
+    YMD_PARALLEL_FOR
+    for (int64_t x0 = 0; x0 < n[0]; ++x0)
+    for (int64_t x1 = 0; x1 < n[1]; ++x1) for (int64_t x2 = 0; x2 < n[2]; ++x2) for (int64_t x3 = 0; x3 < n[3]; ++x3) {
+        const int64_t x[4] = {x0, x1, x2, x3};
         uint32_t site = lex_of(&g, x);
         for (int mu = 0; mu < 4; ++mu) {
             double q[4];
