@@ -456,6 +456,16 @@ int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *mlp, double beta, u
  * [36..43] chart coordinates of the link, [44..46] chart phases,
  * [47..58] features, [59] frame ok, [60] chart ok, [61] log Haar Jacobian. */
 int32_t ym_su3_link_chart(const ym_lattice *lat, const int64_t x[4], int32_t mu, double *out62);
+/* Probe of one learned MH step (no reference counterpart; for tests): for
+ * the link (x, mu) of the lattice as it stands, the values k_su3_learned
+ * computes for it in sub-pass (mu, parity of x) of sweep `sweep` with the
+ * FP32 network, by the same device function: out[0..7] chart coordinates of
+ * the link, [8..15] mu, [16..23] sigma, [24..31] proposal eps', [32] log q(U),
+ * [33] log q(U'), [34] (beta/3) Re Tr((U' - U) A), [35] log MH ratio,
+ * [36] accept uniform, [37] proposal inside the chart, [38] accepted,
+ * [39] chart of the link ok, [40..57] U' (18 doubles, host layout). */
+int32_t ym_su3_learned_probe(const ym_lattice *lat, const ym_mlp *mlp, double beta, uint64_t key, uint64_t sweep,
+                             const int64_t x[4], int32_t mu, double *out58);
 
 /* ------------------------------------------------- host FP64 statistics */
 /* su2_ensemble.rs:375-463 diagnostics.  autocorr (optional) gets max_lag+1. */

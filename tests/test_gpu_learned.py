@@ -66,6 +66,77 @@ def test_link_chart_matches_oracle(ym, thermal):
     assert checked >= 20
 
 
+def test_learned_mh_step_matches_oracle_and_sweep(ym, thermal):
+    """The MH step of the learned sweep, link by link: the kernel's own
+    device function (ym_su3_learned_probe) against the numpy restatement --
+    chart of the link, network outputs, the proposed link exp(i X(eps')) W,
+    log q(U) and log q(U') with the Haar Jacobian, the action term, the log
+    ratio (1e-9) and the accept decision -- and then against what one learned
+    sweep (no Metropolis hits) stores for those links (links of sub-pass
+    (mu = 0, even sites), the first of the sweep, so their staples are the
+    ones the probe saw)."""
+    m = ym.learned.Mlp(seed=17, dataset_capacity=8 * 8 ** 4 * 2)
+    m.collect_pairs(thermal.clone())
+    m.train(epochs=3, batch=4096, lr=3e-3)
+    params = m.get_params().astype(np.float64)
+    lat = thermal.clone()
+    aos = lat.download()
+    n = lat.extents
+    beta, sweep = 6.0, 2000
+    rng = np.random.default_rng(5)
+    sites = []
+    while len(sites) < 40:
+        x = tuple(int(v) for v in rng.integers(0, 8, size=4))
+        if sum(x) % 2 == 0 and x not in sites:
+            sites.append(x)
+    probes = {}
+    n_acc = n_rej = 0
+    for x in sites:
+        pr = ym.learned.learned_probe(lat, m, beta, KEY, sweep, x, 0)
+        probes[x] = pr
+        if not pr.chart_ok:
+            continue
+        A = ym.learned.link_chart(lat, x, 0).staple
+        U = LO.reunitarize(site_links(aos, n, x, 0))
+        W, eps0, ph0 = LO.chart(U, A)
+        assert eps0 is not None
+        assert np.abs(pr.eps - eps0).max() < 1e-9
+        out = LO.mlp_forward(params, LO.features(A)[None, :])[0]
+        ls = np.clip(out[8:], LO.LS_MIN, LO.LS_MAX)
+        assert np.abs(pr.mu - out[:8]).max() < 1e-5
+        assert np.abs(pr.sigma / np.exp(ls) - 1.0).max() < 1e-5
+        if not pr.in_chart:
+            assert not pr.accepted
+            assert np.abs(np.linalg.eigvalsh(LO.from_coords(pr.eps_proposal))).max() > np.pi - 1e-5
+            n_rej += 1
+            continue
+        X1 = LO.from_coords(pr.eps_proposal)
+        Up = LO.expi(X1) @ W
+        assert np.abs(pr.proposal - Up).max() < 1e-10
+        ph1 = np.linalg.eigvalsh(X1)
+        lq0 = -0.5 * np.sum(((eps0 - pr.mu) / pr.sigma) ** 2) - LO.haar_log_jacobian(ph0)
+        lq1 = -0.5 * np.sum(((pr.eps_proposal - pr.mu) / pr.sigma) ** 2) - LO.haar_log_jacobian(ph1)
+        dre = beta / 3.0 * (np.trace(Up @ A).real - np.trace(U @ A).real)
+        assert abs(pr.log_q_current - lq0) < 1e-9 and abs(pr.log_q_proposal - lq1) < 1e-9
+        assert abs(pr.action_term - dre) < 1e-10
+        lr = dre + lq0 - lq1
+        assert abs(pr.log_ratio - lr) < 1e-9
+        want = lr >= 0.0 or pr.accept_uniform < np.exp(lr)
+        if abs(pr.accept_uniform - np.exp(min(lr, 0.0))) > 1e-8:
+            assert pr.accepted == want
+        n_acc += pr.accepted
+        n_rej += not pr.accepted
+    assert n_acc >= 5 and n_rej >= 1, (n_acc, n_rej)
+    # the sweep itself: accepted links become U', rejected ones keep U
+    ym.learned.learned_sweeps(lat, m, beta, KEY, sweep, 1, metro_hits=0)
+    after = lat.download()
+    for x, pr in probes.items():
+        new = site_links(after, n, x, 0)
+        old = LO.reunitarize(site_links(aos, n, x, 0))
+        want = LO.reunitarize(pr.proposal) if pr.accepted else old
+        assert np.abs(new - want).max() < 2e-6, (x, pr.accepted)
+
+
 def test_mlp_forward_fp32_and_bf16(ym):
     m = ym.learned.Mlp(seed=3)
     p = m.get_params().astype(np.float64)

@@ -297,6 +297,13 @@ int32_t sum_partials(ym_lattice *l, int nblk, int ncols, double *host_out) {
 
 }  // namespace
 
+namespace ym {
+void *lattice_aux(const ym_lattice *lc, size_t bytes) {
+    ym_lattice *l = const_cast<ym_lattice *>(lc);
+    return ensure_aux(l, bytes) == YM_OK ? l->aux : nullptr;
+}
+}  // namespace ym
+
 extern "C" {
 
 const char *ym_status_string(int32_t s) {

@@ -16,6 +16,8 @@ void *lattice_links(const ym_lattice *lat);
 unsigned long long *lattice_counters(const ym_lattice *lat);  // 4 device counters of the handle
 void lattice_aux_events(const ym_lattice *lat, hipEvent_t *e0, hipEvent_t *e1);
 hipStream_t lattice_stream(const ym_lattice *lat);
+// The handle's byte buffer, grown to at least `bytes` (nullptr if that fails).
+void *lattice_aux(const ym_lattice *lat, size_t bytes);
 
 void launch_su3_metropolis(int prec, void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
                            uint64_t key, uint64_t sweep, int reunit, unsigned long long *acc, hipStream_t st);

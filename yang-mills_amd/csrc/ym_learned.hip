@@ -189,6 +189,86 @@ __device__ __forceinline__ double unit_open(uint32_t lo, uint32_t hi) {
     return (double)((v >> 11) + 1ull) * 0x1p-53;
 }
 
+// Intermediate values of one MH step, for the single-link probe.
+struct MhRecord {
+    double mu[8], sig[8], epsn[8];
+    double lq0, lq1, dre, lr, uacc;
+    int in_chart, accepted;
+};
+
+// The learned independence proposal and its MH test for one link whose chart
+// (eps0, ph0, z0) in the frame W of its staple A succeeded: eps' from the
+// network outputs `out` and four Box-Muller Philox draws, U' = exp(i X(eps')) W,
+// log ratio (beta/3) Re Tr((U' - U) A) + log q(U) - log q(U').  Returns true
+// when accepted, with U' in Un.  Used by k_su3_learned and by the probe, so
+// both evaluate the same arithmetic.
+template <bool REC>
+__device__ __forceinline__ bool learned_mh(const float out[kOut], const double eps0[8], const double ph0[3],
+                                           const la::cx z0[3], const la::C3 &W, const la::C3 &A,
+                                           const M3<double> &u, double beta, uint64_t key, uint32_t site, int mu_dir,
+                                           uint64_t sweep, la::C3 &Un, MhRecord *rec) {
+    double mu[8], sig[8], isig[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const float ls = fminf(fmaxf(out[8 + a], kLogSigmaMin), kLogSigmaMax);
+        mu[a] = (double)out[a];
+        // sigma is whatever FP32 exp gives; the same value draws eps' and
+        // enters both densities of the MH ratio
+        sig[a] = (double)expf(ls);
+        isig[a] = 1.0 / sig[a];
+    }
+    double epsn[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t o[4];
+        draw(key, site, mu_dir, KIND_LEARNED, (uint32_t)k, sweep, o);
+        const double u1 = unit_open(o[0], o[1]), u2 = unit53(o[2], o[3]);
+        const double r = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos_turns(u2, sn, cs);
+        epsn[2 * k] = mu[2 * k] + sig[2 * k] * (r * cs);
+        epsn[2 * k + 1] = mu[2 * k + 1] + sig[2 * k + 1] * (r * sn);
+    }
+    uint32_t o[4];
+    draw(key, site, mu_dir, KIND_LEARNED, 4u, sweep, o);
+    const double uacc = unit53(o[0], o[1]);
+    // X' = V diag(x) V^dagger; proposals outside the chart (an eigenvalue
+    // beyond +-(pi - 1e-6)) are rejected: on the chart eps -> U' is one to
+    // one, so the chart coordinates of U' are eps' and its phases the
+    // eigenvalues x
+    double ph1[3];
+    la::cx z1[3];
+    const la::C3 E = la::expi_hermitian_fast(la::su3_from_coords(epsn), ph1, z1);
+    bool in_chart = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) in_chart = in_chart && fabs(ph1[k]) < 3.14159165358979;
+    bool acc = false;
+    double dre = 0.0, lq0 = 0.0, lq1 = 0.0, lr = 0.0;
+    if (in_chart) {
+        Un = la::c3_mul(E, W);
+        dre = (beta / 3.0) * (re_tr_mul(Un, A) - re_tr_mul(to_c3(u), A));
+        lq0 = log_q(eps0, ph0, z0, mu, isig);
+        lq1 = log_q(epsn, ph1, z1, mu, isig);
+        lr = dre + lq0 - lq1;
+        acc = lr >= 0.0 || uacc < exp(lr);
+    }
+    if constexpr (REC) {
+        for (int a = 0; a < 8; ++a) {
+            rec->mu[a] = mu[a];
+            rec->sig[a] = sig[a];
+            rec->epsn[a] = epsn[a];
+        }
+        rec->lq0 = lq0;
+        rec->lq1 = lq1;
+        rec->dre = dre;
+        rec->lr = lr;
+        rec->uacc = uacc;
+        rec->in_chart = in_chart ? 1 : 0;
+        rec->accepted = acc ? 1 : 0;
+    }
+    return acc;
+}
+
 // ------------------------------------------------- BF16 MFMA inference
 // One wavefront evaluates the MLP for its 64 links with
 // v_mfma_f32_32x32x16_bf16 (D 32x32 += A 32x16 * B 16x32, FP32
@@ -388,50 +468,11 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
     // phase 3 (per link): proposal, MH test, Metropolis hits, store
     if (live) {
         if (ok) {
-            double mu[8], sig[8], isig[8];
-#pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                const float ls = fminf(fmaxf(out[8 + a], kLogSigmaMin), kLogSigmaMax);
-                mu[a] = (double)out[a];
-                // sigma is whatever FP32 exp gives; the same value draws
-                // eps' and enters both densities of the MH ratio
-                sig[a] = (double)expf(ls);
-                isig[a] = 1.0 / sig[a];
-            }
-            double epsn[8];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                uint32_t o[4];
-                draw(key, site, MU, KIND_LEARNED, (uint32_t)k, sweep, o);
-                const double u1 = unit_open(o[0], o[1]), u2 = unit53(o[2], o[3]);
-                const double r = sqrt(-2.0 * log(u1));
-                double sn, cs;
-                sincos_turns(u2, sn, cs);
-                epsn[2 * k] = mu[2 * k] + sig[2 * k] * (r * cs);
-                epsn[2 * k + 1] = mu[2 * k + 1] + sig[2 * k + 1] * (r * sn);
-            }
-            uint32_t o[4];
-            draw(key, site, MU, KIND_LEARNED, 4u, sweep, o);
-            const double uacc = unit53(o[0], o[1]);
-            // X' = V diag(x) V^dagger; proposals outside the chart (an
-            // eigenvalue beyond +-(pi - 1e-6)) are rejected: on the chart
-            // eps -> U' is one to one, so the chart coordinates of U' are
-            // eps' and its phases the eigenvalues x
-            double ph1[3];
-            la::cx z1[3];
-            const la::C3 E = la::expi_hermitian_fast(la::su3_from_coords(epsn), ph1, z1);
-            bool in_chart = true;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) in_chart = in_chart && fabs(ph1[k]) < 3.14159165358979;
             tried = 1;
-            if (in_chart) {
-                const la::C3 Un = la::c3_mul(E, W);
-                const double lr = (beta / 3.0) * (re_tr_mul(Un, A) - re_tr_mul(to_c3(u), A)) +
-                                  log_q(eps0, ph0, z0, mu, isig) - log_q(epsn, ph1, z1, mu, isig);
-                if (lr >= 0.0 || uacc < exp(lr)) {
-                    accepted = 1;
-                    u = from_c3(Un);
-                }
+            la::C3 Un;
+            if (learned_mh<false>(out, eps0, ph0, z0, W, A, u, beta, key, site, MU, sweep, Un, nullptr)) {
+                accepted = 1;
+                u = from_c3(Un);
             }
         }
         if (metro_hits > 0) {
@@ -532,6 +573,60 @@ __global__ void k_learn_probe(const float2 *__restrict__ L, Geom g, int p, int s
     out[59] = okf;
     out[60] = okc;
     out[61] = okc ? la::haar_log_jacobian(ph, z) : 0.0;
+}
+
+// Probe of one learned MH step (tests): the link's chart, the FP32 network's
+// outputs, then learned_mh as k_su3_learned evaluates it.  out: [0..7] eps,
+// [8..15] mu, [16..23] sigma, [24..31] eps', [32] log q(U), [33] log q(U'),
+// [34] (beta/3) Re Tr((U' - U) A), [35] log ratio, [36] accept uniform,
+// [37] in chart, [38] accepted, [39] chart ok, [40..57] U'.
+template <int MU>
+__global__ void k_learn_mh_probe(const float2 *__restrict__ L, Geom g, int p, int s,
+                                 const float *__restrict__ params, double beta, uint64_t key, uint64_t sweep,
+                                 double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int x[4];
+    if (!site_of(g, p, s, x)) return;
+    M3<float> Af;
+    su3_staple_at<float, MU>(L, g, p, s, x, Af);
+    const M3<float> Uf = load3<float>(L, g.vh, MU, p, s);
+    la::C3 A, W;
+    M3<double> u;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A.m[i][j] = {(double)YM_RE(Af, i, j), (double)YM_IM(Af, i, j)};
+    for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
+    reunitarize3(u);
+    float feat[kIn] = {0};
+    double eps0[8] = {0}, ph0[3] = {0};
+    la::cx z0[3];
+    bool ok = false;
+    if (staple_frame(A, W, feat)) ok = chart_coords(to_c3(u), W, eps0, ph0, z0);
+    else
+        for (int i = 0; i < kIn; ++i) feat[i] = 0.0f;
+    float o[kOut];
+    mlp_forward<false>(params, feat, o);
+    for (int a = 0; a < 8; ++a) out[a] = eps0[a];
+    out[39] = ok ? 1.0 : 0.0;
+    if (!ok) return;
+    MhRecord rec;
+    la::C3 Un;
+    learned_mh<true>(o, eps0, ph0, z0, W, A, u, beta, key, (uint32_t)lex_of(g, x), MU, sweep, Un, &rec);
+    for (int a = 0; a < 8; ++a) {
+        out[8 + a] = rec.mu[a];
+        out[16 + a] = rec.sig[a];
+        out[24 + a] = rec.epsn[a];
+    }
+    out[32] = rec.lq0;
+    out[33] = rec.lq1;
+    out[34] = rec.dre;
+    out[35] = rec.lr;
+    out[36] = rec.uacc;
+    out[37] = rec.in_chart;
+    out[38] = rec.accepted;
+    if (rec.in_chart) {
+        const M3<double> Um = from_c3(Un);
+        for (int c = 0; c < 18; ++c) out[40 + c] = Um.v[c];
+    }
 }
 
 // ------------------------------------------------------------ training
@@ -1118,8 +1213,8 @@ int32_t ym_su3_link_chart(const ym_lattice *lat, const int64_t x[4], int32_t mu,
     const int p = (int)((x[0] + x[1] + x[2] + x[3]) & 1);
     const int64_t row = (x[0] * ext[1] + x[1]) * ext[2] + x[2];
     const int s = (int)((row * g.n3p + x[3]) >> 1);
-    double *d = nullptr;
-    if (hipMalloc((void **)&d, sizeof(double) * 62) != hipSuccess) return YM_ALLOCATION_FAILED;
+    double *d = (double *)ym::lattice_aux(lat, sizeof(double) * 62);
+    if (!d) return YM_ALLOCATION_FAILED;
     hipStream_t ls = ym::lattice_stream(lat);
     ym::launch_fill_zero(d, 62, ls);
     const float2 *L = (const float2 *)ym::lattice_links(lat);
@@ -1132,7 +1227,42 @@ int32_t ym_su3_link_chart(const ym_lattice *lat, const int64_t x[4], int32_t mu,
     st = hs(hipGetLastError());
     if (!st) st = hs(hipMemcpyAsync(out62, d, sizeof(double) * 62, hipMemcpyDeviceToHost, ls));
     if (!st) st = hs(hipStreamSynchronize(ls));
-    (void)hipFree(d);
+    return st;
+}
+
+int32_t ym_su3_learned_probe(const ym_lattice *lat, const ym_mlp *m, double beta, uint64_t key, uint64_t sweep,
+                             const int64_t x[4], int32_t mu, double *out58) {
+    if (!lat || !m || !x || !out58 || mu < 0 || mu > 3) return YM_INVALID_ARGUMENT;
+    if (!std::isfinite(beta) || beta < 0.0 || beta > 12.0) return YM_BETA_OUT_OF_RANGE;
+    int32_t group = 0, prec = 0;
+    int64_t ext[4];
+    int32_t st = ym::lattice_info(lat, &group, &prec, ext);
+    if (st) return st;
+    if (group != 3 || prec != 32) return YM_UNSUPPORTED;
+    for (int d = 0; d < 4; ++d) {
+        if (ext[d] % 2) return YM_UNSUPPORTED;
+        if (x[d] < 0 || x[d] >= ext[d]) return YM_COORDINATE_OUT_OF_RANGE;
+    }
+    const ym::Geom g = ym::make_geom(ext);
+    const int p = (int)((x[0] + x[1] + x[2] + x[3]) & 1);
+    const int64_t row = (x[0] * ext[1] + x[1]) * ext[2] + x[2];
+    const int s = (int)((row * g.n3p + x[3]) >> 1);
+    double *d = (double *)ym::lattice_aux(lat, sizeof(double) * 58);
+    if (!d) return YM_ALLOCATION_FAILED;
+    st = hs(hipStreamSynchronize(m->stream));
+    if (st) return st;
+    hipStream_t ls = ym::lattice_stream(lat);
+    ym::launch_fill_zero(d, 58, ls);
+    const float2 *L = (const float2 *)ym::lattice_links(lat);
+    switch (mu) {
+    case 0: hipLaunchKernelGGL(ym::k_learn_mh_probe<0>, dim3(1), dim3(64), 0, ls, L, g, p, s, m->params, beta, key, sweep, d); break;
+    case 1: hipLaunchKernelGGL(ym::k_learn_mh_probe<1>, dim3(1), dim3(64), 0, ls, L, g, p, s, m->params, beta, key, sweep, d); break;
+    case 2: hipLaunchKernelGGL(ym::k_learn_mh_probe<2>, dim3(1), dim3(64), 0, ls, L, g, p, s, m->params, beta, key, sweep, d); break;
+    default: hipLaunchKernelGGL(ym::k_learn_mh_probe<3>, dim3(1), dim3(64), 0, ls, L, g, p, s, m->params, beta, key, sweep, d); break;
+    }
+    st = hs(hipGetLastError());
+    if (!st) st = hs(hipMemcpyAsync(out58, d, sizeof(double) * 58, hipMemcpyDeviceToHost, ls));
+    if (!st) st = hs(hipStreamSynchronize(ls));
     return st;
 }
 

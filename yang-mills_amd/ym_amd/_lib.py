@@ -134,6 +134,7 @@ SIGNATURES = {
     "ym_su3_learned_sweeps": (i32, [C.c_void_p, C.c_void_p, dbl, u64, u64, i32, i32, dbl, i32, P(u64), P(u64),
                                     P(u64), P(flt)]),
     "ym_su3_link_chart": (i32, [C.c_void_p, P(i64), i32, P(dbl)]),
+    "ym_su3_learned_probe": (i32, [C.c_void_p, C.c_void_p, dbl, u64, u64, P(i64), i32, P(dbl)]),
     "ym_evaluate": (i32, [P(dbl), i64, P(dbl), i64, dbl, dbl, dbl, P(EfficiencyReport)]),
     "ym_stationarity": (i32, [P(dbl), i64, i64, P(StationarityReport)]),
     "ym_pool_chains": (i32, [P(dbl), i64, i64, i64, i64, P(dbl), P(dbl), P(dbl), P(dbl), P(PoolSummary)]),

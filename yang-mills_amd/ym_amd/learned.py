@@ -145,3 +145,31 @@ def link_chart(lattice, x, mu):
           "ym_su3_link_chart")
     return LinkChart(_c3(out[0:18]), _c3(out[18:36]), out[36:44].copy(), out[44:47].copy(), out[47:59].copy(),
                      bool(out[59]), bool(out[60]), float(out[61]))
+
+
+@dataclass
+class MhStep:
+    """One learned MH step as the sweep kernel evaluates it (ym_su3_learned_probe)."""
+    eps: np.ndarray            # chart coordinates of the link
+    mu: np.ndarray             # network means
+    sigma: np.ndarray          # network standard deviations
+    eps_proposal: np.ndarray   # proposed coordinates
+    log_q_current: float
+    log_q_proposal: float
+    action_term: float         # (beta/3) Re Tr((U' - U) A)
+    log_ratio: float
+    accept_uniform: float
+    in_chart: bool
+    accepted: bool
+    chart_ok: bool
+    proposal: np.ndarray       # (3, 3) complex U'
+
+
+def learned_probe(lattice, mlp, beta, key, sweep, x, mu):
+    out = np.zeros(58)
+    xs = (C.c_int64 * 4)(*[int(v) for v in x])
+    check(lib().ym_su3_learned_probe(lattice.handle, mlp.handle, float(beta), int(key), int(sweep), xs, int(mu),
+                                     out.ctypes.data_as(C.POINTER(C.c_double))), "ym_su3_learned_probe")
+    return MhStep(out[0:8].copy(), out[8:16].copy(), out[16:24].copy(), out[24:32].copy(), float(out[32]),
+                  float(out[33]), float(out[34]), float(out[35]), float(out[36]), bool(out[37]), bool(out[38]),
+                  bool(out[39]), _c3(out[40:58]))
