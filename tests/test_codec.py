@@ -158,3 +158,37 @@ def test_checkpoint_round_trip_and_corruption(ym, group, prec):
     assert L.ym_checkpoint_decode(raw[:-1], len(raw) - 1, dp(back), back.size) == 18
     assert L.ym_checkpoint_decode(raw[:50], 50, dp(back), back.size) == 18
     assert L.ym_checkpoint_decode(raw, len(raw), dp(back), back.size - 1) == 96
+
+
+def test_checkpoint_rejects_extents_whose_volume_wraps(ym):
+    """A 96-byte header with four extents of 2^32 (4 x 2^128 sites wraps to 0
+    links and an empty payload) and a matching FNV-1a checksum must not be
+    accepted; neither may a volume above the 2^30 sites a lattice can have."""
+    import ctypes as C
+    import struct
+    L = ym.lib()
+
+    def fnv(b, h=0xCBF29CE484222325):
+        for x in b:
+            h = ((h ^ x) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+        return h
+
+    good = C.create_string_buffer(96 + 16 * 4 * 4 * 8)
+    n = (2, 2, 2, 2)
+    aos = np.tile(np.array([1.0, 0.0, 0.0, 0.0]), 16 * 4)
+    ln = C.c_int64()
+    assert L.ym_checkpoint_encode(2, 64, (C.c_int64 * 4)(*n), aos.ctypes.data_as(C.POINTER(C.c_double)), 1, 2, good,
+                                  len(good), C.byref(ln)) == 0
+    head = bytearray(good.raw[:96])
+    for ext, links in (((1 << 32,) * 4, 0), (((1 << 15), (1 << 15), 2, 2), 4 << 32)):
+        h = bytearray(head)
+        h[24:56] = struct.pack("<4q", *ext)
+        h[72:88] = struct.pack("<QQ", links & 0xFFFFFFFFFFFFFFFF, (links * 32) & 0xFFFFFFFFFFFFFFFF)
+        h[88:96] = struct.pack("<Q", fnv(bytes(h[:88])))
+        g, p = C.c_int32(), C.c_int32()
+        e2 = (C.c_int64 * 4)()
+        k2, s2 = C.c_uint64(), C.c_uint64()
+        st = L.ym_checkpoint_header(bytes(h), 96, C.byref(g), C.byref(p), e2, C.byref(k2), C.byref(s2))
+        assert st != 0, ext
+        size = C.c_int64()
+        assert L.ym_checkpoint_bytes(2, 64, (C.c_int64 * 4)(*ext), C.byref(size)) == 2

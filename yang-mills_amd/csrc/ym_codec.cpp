@@ -121,6 +121,9 @@ int32_t su2_link_count(const int64_t n[4], uint64_t *links) {
     return YM_OK;
 }
 
+// sites of the largest lattice a handle can be created with (ym_lattice_create)
+constexpr uint64_t kCheckpointMaxSites = (uint64_t)1 << 30;
+
 uint64_t fnv1a64(const unsigned char *p, int64_t n, uint64_t h = 0xcbf29ce484222325ull) {
     for (int64_t i = 0; i < n; ++i) {
         h ^= p[i];
@@ -133,8 +136,14 @@ int32_t ckpt_sizes(int32_t group, int32_t precision, const int64_t n[4], uint64_
     if ((group != 2 && group != 3) || (precision != 32 && precision != 64) || !n) return YM_INVALID_ARGUMENT;
     for (int d = 0; d < 4; ++d)
         if (n[d] < 2) return YM_INVALID_EXTENT;
-    unsigned __int128 s = 4;
-    for (int d = 0; d < 4; ++d) s *= (unsigned __int128)n[d];
+    // checked after every factor (four extents of 2^32 would wrap a 128-bit
+    // product), and the volume capped at what a lattice handle can hold
+    unsigned __int128 v = 1;
+    for (int d = 0; d < 4; ++d) {
+        v *= (unsigned __int128)n[d];
+        if (v > (unsigned __int128)kCheckpointMaxSites) return YM_LATTICE_TOO_LARGE;
+    }
+    const unsigned __int128 s = 4 * v;
     const unsigned __int128 bytes = s * (group == 3 ? 18 : 4) * (precision / 8);
     if (bytes > (unsigned __int128)INT64_MAX / 2) return YM_LATTICE_SIZE_OVERFLOW;
     *links = (uint64_t)s;
