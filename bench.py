@@ -3,12 +3,14 @@
 
 One step = one full checkerboard sweep (mu = 0..3, parity 0/1: 8 sub-pass
 launches) of single-generator Metropolis over an SU(3) Wilson-action
-lattice, 8 proposals per link (the reference sweep's proposal set,
-su3_ensemble.rs:342-391), links stored FP32, accept test and
-reunitarisation FP64.  Default workload: 32^4, beta = 6.0, proposal angle
-0.35, cold start as the reference's run_chain (su3_ensemble.rs:412) followed
-by the warm-up sweeps; the links stay resident in HBM for the whole timed
-region.
+lattice, 10 proposals per link (generator h mod 8 for hit h: the reference
+sweep's 8 generators, then generators 0 and 1 again), links stored FP32,
+accept test and reunitarisation FP64.  Default workload: 32^4, beta = 6.0,
+proposal angle 0.35, cold start as the reference's run_chain
+(su3_ensemble.rs:412) followed by the warm-up sweeps; the links stay
+resident in HBM for the whole timed region.  The same line carries the
+32^4 8-proposal sweep (the reference's proposal set) and 24^4 at 10 hits,
+each with its own roofline.
 
 The second half of the metric, plaquette ESS/s of the learned proposal
 against plain Metropolis, is measured on SU(3) 16^3x32 (learned_vs_plain).
@@ -18,8 +20,10 @@ the dominant kernel (k_su3_metropolis_ws -- k_su3_metropolis on lattices of
 fewer than 256 x 256 links per sub-pass -- one launch per checkerboard sub-pass:
 algorithmic bytes per launch / mean launch duration, the device time of the
 timed region by one HIP event pair on the kernel's stream over its launch
-count) and the CPU baseline (the reference-exact restatement of
-metropolis_sweep timed on the host cores).
+count), the CPU baseline (the reference-exact restatement of
+metropolis_sweep on every host thread of the job, at 16^4 and 12^3x24) and
+the GPU's own algorithm on the host (the threaded FP64 device-order replay
+at 24^4 and 32^4).
 """
 import argparse
 import ctypes as C
@@ -47,29 +51,45 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds_target):
-    """Reference-exact metropolis_sweep (oracle restatement) on host threads."""
-    sys.path.insert(0, os.path.join(REPO, "tests"))
-    import numpy as np
+def job_threads():
+    """Host threads this job may use: the CPUs it is scheduled on, capped by
+    OMP_NUM_THREADS when the environment sets it (the GPU pool grants each
+    one-GPU job a share of the host's cores that way)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        avail = min(avail, int(env))
+    return max(1, avail)
 
+
+def cpu_baseline():
+    """Reference-exact metropolis_sweep (the oracle's restatement of
+    su3_ensemble.rs:342-391: lexicographic order, SplitMix64, checked FP64
+    products, Taylor exp) on every host thread of the job, one independent
+    chain per thread, at the reference's largest admissible SU(3) sizes:
+    half the threads on 16^4 (2^18 links, su3_lattice.rs:17), half on
+    12^3x24, one sweep each, run at the same time."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib as O
 
     L = O.lib()
-    cores = max(1, min(16, os.cpu_count() or 1))
-    n = (6, 6, 6, 6)
-    links_per_sweep = 4 * 6 ** 4
-    # one sweep of 6^4 takes ~1.1 s per core; size the sample to the target
-    sweeps = max(1, int(round(seconds_target / 1.2)))
-    done = [0] * cores
+    cores = job_threads()
+    geoms = [(16, 16, 16, 16), (12, 12, 12, 24)]
+    plan = [geoms[i % 2] for i in range(cores)]
+    secs = [0.0] * cores
 
     def work(i):
+        n = plan[i]
         links = O.su3_identity(n)
         rng = C.c_uint64(O.lib().ymo_derive_su3_chain_seed(0x2C02198105010002, i))
         p, a = C.c_uint64(), C.c_uint64()
-        for _ in range(sweeps):
-            st = L.ymo_su3_metropolis_sweep(O.ext(n), O.dptr(links), 6.0, 0.35, C.byref(rng), C.byref(p), C.byref(a))
-            assert st == 0
-            done[i] += links_per_sweep
+        t = time.perf_counter()
+        assert L.ymo_su3_metropolis_sweep(O.ext(n), O.dptr(links), 6.0, 0.35, C.byref(rng), C.byref(p),
+                                          C.byref(a)) == 0
+        secs[i] = time.perf_counter() - t
 
     t0 = time.perf_counter()
     th = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
@@ -78,62 +98,102 @@ def cpu_baseline(seconds_target):
     for t in th:
         t.join()
     dt = time.perf_counter() - t0
-    total = sum(done)
+    total = sum(4 * int(np.prod(n)) for n in plan)
+    per_geom = {}
+    for g in geoms:
+        idx = [i for i in range(cores) if plan[i] == g]
+        if idx:
+            per_geom["x".join(str(v) for v in g)] = {
+                "threads": len(idx), "per_core": float(np.mean([4 * np.prod(g) / secs[i] for i in idx]))}
     return {
         "value": total / dt,
         "unit": "link updates/s",
         "cores": cores,
         "per_core": total / dt / cores,
         "kind": "port",
-        "sample": (f"{cores} threads x {sweeps} reference-order sweeps of SU(3) 6^4 at beta=6.0 "
-                   f"(8 checked-product Metropolis proposals per link, SplitMix64), {total} link updates in {dt:.1f} s"),
+        "host_cpus": os.cpu_count(),
+        "by_lattice": per_geom,
+        "sample": (f"{cores} threads (all the job's host threads; host reports {os.cpu_count()} CPUs), one "
+                   f"reference-order sweep each of SU(3) 16^4 or 12^3x24 at beta=6.0 (8 checked-product Metropolis "
+                   f"proposals per link, SplitMix64), {total} link updates in {dt:.1f} s"),
     }
 
 
-def cpu_device_order(seconds_target):
-    """The same sweep as the GPU kernels (device-order replay in C, FP32
-    storage, Philox draws, FP64 accept, 8 hits) on host threads, one 8^4
-    chain per thread: the CPU rate of the algorithm the GPU runs, beside the
-    reference-order baseline."""
+def cpu_device_order_fp64(extents=(24, 32), hits=10):
+    """The GPU's algorithm on the host: the device-order replay (checkerboard
+    sub-passes, Philox draws, FP64 link storage, FP64 accept and
+    reunitarisation, 10 hits) with the links of each sub-pass on OpenMP
+    threads, one sweep of a hot-start lattice per extent."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
-    import numpy as np
-
     import oracle_lib as O
 
     L = O.lib()
-    cores = max(1, min(16, os.cpu_count() or 1))
-    n = (8, 8, 8, 8)
-    links_per_sweep = 4 * 8 ** 4
-    vh = L.ymd_slots_per_parity(O.ext(n))
-    done = [0] * cores
-    stop = time.perf_counter() + seconds_target
-
-    def work(i):
-        buf = np.zeros(8 * 18 * vh, dtype=np.float32)
-        key = O.lib().ymo_derive_su3_chain_seed(0x2C02198105010002, i)
-        assert L.ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), 32, key) == 0
+    threads = int(os.environ.get("OMP_NUM_THREADS") or job_threads())
+    rows = []
+    for e in extents:
+        n = (e,) * 4
+        vh = L.ymd_slots_per_parity(O.ext(n))
+        buf = np.zeros(8 * 18 * vh, dtype=np.float64)
+        key = O.lib().ymo_derive_su3_chain_seed(0x2C02198105010002, 0)
+        assert L.ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, key) == 0
+        prm = O.DevSu3Params(6.0, 0.35, hits, 1, key, 0)
         p, a = C.c_uint64(), C.c_uint64()
-        sweep = 0
-        while time.perf_counter() < stop:
-            prm = O.DevSu3Params(6.0, 0.35, 8, 1, key, sweep)
-            st = L.ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), 32, C.byref(prm),
-                                            C.byref(p), C.byref(a))
-            assert st == 0
-            done[i] += links_per_sweep
-            sweep += 1
+        t = time.perf_counter()
+        assert L.ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, C.byref(prm), C.byref(p),
+                                          C.byref(a)) == 0
+        dt = time.perf_counter() - t
+        rows.append({"extent": e, "seconds_per_sweep": dt, "link_updates_per_s": 4 * e ** 4 / dt})
+        del buf
+    return {"kind": "device-order replay, FP64 storage", "cores": threads, "unit": "link updates/s",
+            "hits_per_link": hits, "results": rows,
+            "sample": f"one hot-start sweep of SU(3) {' and '.join(f'{e}^4' for e in extents)} at beta=6.0, "
+                      f"{hits} hits, links of each checkerboard sub-pass on {threads} OpenMP threads"}
 
+
+def sweep_record(ym_amd, L, hits, beta, steps, warmup, hot_start, repeats=4):
+    """Autotune, warm up and time `steps` sweeps of SU(3) L^4 (FP32 storage):
+    updates/s by the host clock, the mean sub-pass launch time from one HIP
+    event pair over the region (on the lattice stream), the median of
+    `repeats` more regions, and the roofline of the sub-pass kernel."""
+    n = (L,) * 4
+    V = L ** 4
+    angle = 0.35
+    key = ym_amd.derive_su3_chain_seed(0x2C02198105010002, 0)
+    lat = ym_amd.Su3Lattice4(n, 32)
+    if hot_start:
+        lat.hot_start(key)
+    best, cand = lat.autotune(beta, angle, hits)
+    log(f"SU(3) {L}^4 hits={hits}: autotune {best} "
+        f"({', '.join(f'{m}/{w}: {t:.1f} us' for (m, w), t in cand.items())})")
+    lat.metropolis_sweeps(beta, angle, key, 0, warmup, hits)
+    # One HIP event pair on the lattice stream around the whole timed region
+    # (8 sub-pass launches per sweep, nothing else on the stream): the mean
+    # launch duration is the region's device time over the launch count, so
+    # it includes the gaps between back-to-back launches.  Events between
+    # the launches are left out because each one delays the next launch.
     t0 = time.perf_counter()
-    th = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    dt = time.perf_counter() - t0
-    total = sum(done)
-    return {"value": total / dt, "unit": "link updates/s", "cores": cores, "per_core": total / dt / cores,
-            "kind": "device-order replay",
-            "sample": f"{cores} threads x hot-start SU(3) 8^4 FP32 chains, beta=6.0, 8 hits, "
-                      f"{total} link updates in {dt:.1f} s"}
+    prop, acc, total_ms = lat.metropolis_sweeps_timed(beta, angle, key, warmup, steps, hits)
+    wall = time.perf_counter() - t0
+    rep = [total_ms / steps]
+    for r in range(repeats):
+        rep.append(lat.metropolis_sweeps_timed(beta, angle, key, warmup + (r + 1) * steps, steps, hits)[2] / steps)
+    avg_launch_s = total_ms / (8 * steps) / 1e3
+    bytes_per_launch = (V // 2) * BYTES_PER_UPDATE_FP32
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    per_link = best[0].endswith("/per-link") or (V // 2 + 255) // 256 < 256
+    med = sorted(rep)[len(rep) // 2]
+    return lat, {
+        "workload": f"SU(3) Wilson action {L}^4, beta={beta}, single-generator Metropolis, {hits} proposals/link/sweep",
+        "value": 4 * V * steps / wall, "unit": "link updates/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": wall * 1e3 / steps, "acceptance": acc / prop,
+        "device_time_ms_per_step": total_ms / steps, "repeat_device_ms_per_step": rep,
+        "median_repeat_link_updates_per_s": 4 * V / (med * 1e-3),
+        "autotuned_tile_mapping": {"mode": best[0], "strip": best[1]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(L, hits),
+                     "kernel": "k_su3_metropolis" if per_link else "k_su3_metropolis_ws",
+                     "bytes_per_launch": bytes_per_launch, "mean_launch_us": avg_launch_s * 1e6},
+    }
 
 
 def learned_vs_plain(chains=8, sweeps=2500, discard=500, therm=2000, equil=1000, configs=20, epochs=10):
@@ -309,9 +369,9 @@ def side_workload(name):
                 lat.heatbath_sweeps(2.3, key, s, 1)
                 lat.overrelax_sweeps(3)
             reps = 10
+            # one call of `reps` sweeps: no host synchronisation between them
             lat.timer_begin()
-            for s in range(reps):
-                lat.heatbath_sweeps(2.3, key, 5 + s, 1)
+            lat.heatbath_sweeps(2.3, key, 5, reps)
             hb = lat.timer_end() / reps
             lat.timer_begin()
             lat.overrelax_sweeps(3 * reps)
@@ -373,8 +433,8 @@ def side_workload(name):
     raise ValueError(name)
 
 
-def load_traffic(extent):
-    path = os.path.join(REPO, "profiles", f"su3_metropolis_{extent}x4_traffic.json")
+def load_traffic(extent, hits):
+    path = os.path.join(REPO, "profiles", f"su3_metropolis_{extent}x4_h{hits}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -388,12 +448,12 @@ def main():
     ap.add_argument("--workload", default="su3",
                     choices=["su3", "su2-sweeps", "su2-precision", "su3-precision", "ensemble", "plaquette"],
                     help="su3 (default): the SU(3) 32^4 sweep metric; the others print their own JSON line")
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--extent", type=int, default=32)
-    ap.add_argument("--hits", type=int, default=8)
+    ap.add_argument("--hits", type=int, default=10, help="proposals per link per sweep (config: 10 hits)")
     ap.add_argument("--beta", type=float, default=6.0)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-side-records", action="store_true", help="skip the 32^4 8-hit and 24^4 sweep records")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain ESS/s measurement")
     ap.add_argument("--learned-chains", type=int, default=8)
@@ -423,48 +483,28 @@ def main():
     L = args.extent
     n = (L, L, L, L)
     V = L ** 4
-    angle = 0.35
-    key = ym_amd.derive_su3_chain_seed(0x2C02198105010002, 0)
     start = "hot" if args.hot_start else "cold"
-    log(f"SU(3) {L}^4 FP32 {start} start, beta={args.beta}, hits={args.hits}")
-    lat = ym_amd.Su3Lattice4(n, 32)
-    if args.hot_start:
-        lat.hot_start(key)
-    best, cand = lat.autotune(args.beta, angle, args.hits)
-    log(f"autotune: {best} ({', '.join(f'{m}/{w}: {t:.1f} us' for (m, w), t in cand.items())})")
-    log(f"warmup {args.warmup} sweeps")
-    lat.metropolis_sweeps(args.beta, angle, key, 0, args.warmup, args.hits)
-    log(f"timed {args.steps} sweeps")
-    # One HIP event pair on the lattice stream around the whole timed region
-    # (8 sub-pass launches per sweep, nothing else on the stream): the mean
-    # launch duration is the region's device time over the launch count, so
-    # it includes the gaps between back-to-back launches.  Events between
-    # the launches are left out because each one delays the next launch.
-    launches = 8 * args.steps
-    t0 = time.perf_counter()
-    prop, acc, total_ms = lat.metropolis_sweeps_timed(args.beta, angle, key, args.warmup, args.steps, args.hits)
-    wall = time.perf_counter() - t0
-    # four more regions of the same length (the sweeps that follow), so the
-    # spread between repeats is reported beside the first region's value
-    repeats_ms = [total_ms / args.steps]
-    if not args.traffic_calibration:
-        for r in range(4):
-            repeats_ms.append(lat.metropolis_sweeps_timed(args.beta, angle, key, args.warmup + (r + 1) * args.steps,
-                                                          args.steps, args.hits)[2] / args.steps)
+    log(f"SU(3) {L}^4 FP32 {start} start, beta={args.beta}, hits={args.hits}: timed {args.steps} sweeps")
+    lat, head = sweep_record(ym_amd, L, args.hits, args.beta, args.steps, args.warmup, args.hot_start,
+                             repeats=0 if args.traffic_calibration else 4)
     if args.traffic_calibration:
         log(f"calibration: device copy of {lat.device_bytes()} bytes")
         lat.clone()
     plaq = lat.average_plaquette()
     unit_res, det_res = lat.link_residuals()
-    updates = 4 * V * args.steps
-    value = updates / wall
-    avg_launch_s = total_ms / launches / 1e3
-    bytes_per_launch = (V // 2) * BYTES_PER_UPDATE_FP32
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    per_link = best[0].endswith("/per-link") or (V // 2 + 255) // 256 < 256
-    kernel = "k_su3_metropolis" if per_link else "k_su3_metropolis_ws"
-    log(f"{value:.4e} link updates/s, {kernel} mean {avg_launch_s * 1e6:.1f} us, "
-        f"{achieved:.0f} GB/s algorithmic")
+    del lat
+    log(f"{head['value']:.4e} link updates/s, {head['roofline']['kernel']} mean "
+        f"{head['roofline']['mean_launch_us']:.1f} us, {head['roofline']['achieved']:.0f} GB/s algorithmic "
+        f"({100 * head['roofline']['frac']:.1f}% of peak)")
+    others = []
+    if not args.no_side_records:
+        for (Lx, hx) in ((32, 8), (24, 10)):
+            if (Lx, hx) == (L, args.hits):
+                continue
+            _, rec = sweep_record(ym_amd, Lx, hx, args.beta, args.steps, args.warmup, args.hot_start)
+            log(f"  {Lx}^4 hits={hx}: {rec['value']:.4e} link updates/s, {rec['roofline']['mean_launch_us']:.1f} us "
+                f"per launch, {100 * rec['roofline']['frac']:.1f}% of peak")
+            others.append(rec)
 
     learned = None
     if not args.no_learned:
@@ -480,19 +520,21 @@ def main():
 
     cpu = cpu_dev = None
     if not args.no_cpu_baseline:
-        log("cpu baseline")
-        cpu = cpu_baseline(args.cpu_seconds)
-        log("cpu device-order replay")
-        cpu_dev = cpu_device_order(min(8.0, args.cpu_seconds))
+        log("cpu baseline: reference-order sweeps of 16^4 and 12^3x24 on every job thread")
+        cpu = cpu_baseline()
+        log(f"  {cpu['value']:.4e} link updates/s on {cpu['cores']} threads ({cpu['per_core']:.0f} per thread)")
+        log("cpu device-order replay, FP64 storage, 24^4 and 32^4")
+        cpu_dev = cpu_device_order_fp64()
 
+    rf = head["roofline"]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "link updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -502,36 +544,28 @@ def main():
                 ", key derive_su3_chain_seed(0x2c02198105010002, 0)",
         "config": {
             "workload": f"SU(3) Wilson action {L}^4, beta={args.beta}, single-generator Metropolis, "
-                        f"{args.hits} proposals/link/sweep, checkerboard sweeps",
+                        f"{args.hits} proposals/link/sweep (generators h mod 8), checkerboard sweeps",
             "extents": list(n),
             "beta": args.beta,
-            "proposal_angle": angle,
+            "proposal_angle": 0.35,
             "hits_per_link": args.hits,
             "link_storage": "fp32 (18 reals)",
             "accept_test": "fp64",
             "reunitarisation": "fp64 per update",
-            "acceptance": acc / prop,
+            "acceptance": head["acceptance"],
             "final_plaquette": plaq,
             "unitarity_residual_fp32_storage": unit_res,
             "determinant_residual_fp32_storage": det_res,
-            "device_time_ms_per_step": total_ms / args.steps,
-            "repeat_device_ms_per_step": repeats_ms,
-            "median_repeat_link_updates_per_s": 4 * V / (sorted(repeats_ms)[len(repeats_ms) // 2] * 1e-3),
-            "autotuned_tile_mapping": {"mode": best[0], "strip": best[1]},
+            "device_time_ms_per_step": head["device_time_ms_per_step"],
+            "repeat_device_ms_per_step": head["repeat_device_ms_per_step"],
+            "median_repeat_link_updates_per_s": head["median_repeat_link_updates_per_s"],
+            "autotuned_tile_mapping": head["autotuned_tile_mapping"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(L),
-            "kernel": kernel,
-            "bytes_per_launch": bytes_per_launch,
-            "mean_launch_us": avg_launch_s * 1e6,
-        },
+        "roofline": {k: rf[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic", "kernel",
+                                        "bytes_per_launch", "mean_launch_us")},
+        "other_configs": others,
         "cpu_baseline": cpu,
-        "cpu_device_order_replay": cpu_dev,
+        "cpu_device_order_fp64": cpu_dev,
         "learned_vs_plain": learned,
     }
     print(json.dumps(out), flush=True)
