@@ -373,6 +373,9 @@ int ymd_su2_heatbath_sweep(const int64_t n[4], void *dev, int precision, double 
     return DISPATCH(precision, su2_hb_f32(n, dev, beta, key, sweep, max_attempts, proposals, updates),
                     su2_hb_f64(n, dev, beta, key, sweep, max_attempts, proposals, updates));
 }
+int ymd_su3_average_plaquette(const int64_t n[4], const void *dev, int precision, double *out) {
+    return DISPATCH(precision, su3_plaq_f32(n, dev, out), su3_plaq_f64(n, dev, out));
+}
 int ymd_su2_overrelax_sweep(const int64_t n[4], void *dev, int precision) {
     return DISPATCH(precision, su2_or_f32(n, dev), su2_or_f64(n, dev));
 }

@@ -55,6 +55,8 @@ int ymd_su2_hot_start(const int64_t n[4], void *dev, int precision, uint64_t key
 int ymd_su3_metropolis_sweep(const int64_t n[4], void *dev, int precision, const ymd_su3_params *p,
                              uint64_t *proposals, uint64_t *accepts);
 int ymd_su3_staple(const int64_t n[4], const void *dev, int precision, const int64_t x[4], int mu, double *out18);
+/* average plaquette of device-layout links, FP64 products, deterministic for any thread count */
+int ymd_su3_average_plaquette(const int64_t n[4], const void *dev, int precision, double *out);
 int ymd_su2_heatbath_sweep(const int64_t n[4], void *dev, int precision, double beta, uint64_t key, uint64_t sweep,
                            int64_t max_attempts, uint64_t *proposals, uint64_t *updates);
 int ymd_su2_overrelax_sweep(const int64_t n[4], void *dev, int precision);

@@ -368,6 +368,68 @@ static int F(su3_sweep)(const int64_t n[4], void *dev, const ymd_su3_params *P, 
     return YMO_OK;
 }
 
+/* ---------------------------------------------------------- plaquette */
+/* Average plaquette (1/6V) sum_x sum_{mu<nu} (1/3) Re Tr P_{mu nu}(x) of the
+ * stored links, products in FP64; one partial sum per x0 slice (threads over
+ * slices), the slices added in order, so the value does not depend on the
+ * thread count.  A measurement for CPU chains in statistical comparisons,
+ * not a replay of a device kernel. */
+static void F(m3d)(const REAL *L, const geom *g, const int64_t x[4], int mu, double m[18]) {
+    REAL v[18];
+    F(load_link)(L, g, x, mu, 18, v);
+    for (int c = 0; c < 18; ++c) m[c] = (double)v[c];
+}
+/* r = a b (dagger_b: a b^dagger) */
+static void F(m3mul)(const double *a, const double *b, int dagger_b, double *r) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double ar = a[(3 * i + k) * 2], ai = a[(3 * i + k) * 2 + 1];
+                double br = dagger_b ? b[(3 * j + k) * 2] : b[(3 * k + j) * 2];
+                double bi = dagger_b ? -b[(3 * j + k) * 2 + 1] : b[(3 * k + j) * 2 + 1];
+                re += ar * br - ai * bi;
+                im += ar * bi + ai * br;
+            }
+            r[(3 * i + j) * 2] = re;
+            r[(3 * i + j) * 2 + 1] = im;
+        }
+}
+static int F(su3_plaq)(const int64_t n[4], const void *dev, double *out) {
+    geom g = make_geom(n);
+    const REAL *L = (const REAL *)dev;
+    double *part = (double *)calloc((size_t)n[0], sizeof(double));
+    if (!part) return YMO_ALLOCATION_FAILED;
+    YMD_PARALLEL_FOR
+    for (int64_t x0 = 0; x0 < n[0]; ++x0) {
+        double acc = 0.0;
+        for (int64_t x1 = 0; x1 < n[1]; ++x1) for (int64_t x2 = 0; x2 < n[2]; ++x2) for (int64_t x3 = 0; x3 < n[3]; ++x3) {
+            const int64_t x[4] = {x0, x1, x2, x3};
+            for (int mu = 0; mu < 4; ++mu)
+                for (int nu = mu + 1; nu < 4; ++nu) {
+                    int64_t xm[4], xn[4];
+                    shift(&g, x, mu, +1, xm);
+                    shift(&g, x, nu, +1, xn);
+                    double a[18], b[18], c[18], d[18], t1[18], t2[18], t3[18];
+                    F(m3d)(L, &g, x, mu, a);
+                    F(m3d)(L, &g, xm, nu, b);
+                    F(m3d)(L, &g, xn, mu, c);
+                    F(m3d)(L, &g, x, nu, d);
+                    F(m3mul)(a, b, 0, t1);
+                    F(m3mul)(t1, c, 1, t2);
+                    F(m3mul)(t2, d, 1, t3);
+                    acc += (t3[0] + t3[8] + t3[16]) / 3.0;
+                }
+        }
+        part[x0] = acc;
+    }
+    double tot = 0.0;
+    for (int64_t x0 = 0; x0 < n[0]; ++x0) tot += part[x0];
+    free(part);
+    *out = tot / (6.0 * (double)(n[0] * n[1] * n[2] * n[3]));
+    return YMO_OK;
+}
+
 /* ---------------------------------------------------------- hot starts */
 /* Box-Muller from two 53-bit uniforms: r = sqrt(-2 log(1-u1)), angle 2 pi u2 */
 static inline void F(gauss2)(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1, double *g1, double *g2) {

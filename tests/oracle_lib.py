@@ -179,6 +179,7 @@ def _dev_signatures(L):
         "ymd_su2_hot_start": (C.c_int, [ip, C.c_void_p, C.c_int, u64]),
         "ymd_su3_metropolis_sweep": (C.c_int, [ip, C.c_void_p, C.c_int, P(DevSu3Params), P(u64), P(u64)]),
         "ymd_su3_staple": (C.c_int, [ip, C.c_void_p, C.c_int, ip, C.c_int, dp]),
+        "ymd_su3_average_plaquette": (C.c_int, [ip, C.c_void_p, C.c_int, dp]),
         "ymd_su2_heatbath_sweep": (C.c_int, [ip, C.c_void_p, C.c_int, dbl, u64, u64, i64, P(u64), P(u64)]),
         "ymd_su2_overrelax_sweep": (C.c_int, [ip, C.c_void_p, C.c_int]),
         "ymd_su2_heatbath_link": (C.c_int, [dp, dbl, u64, C.c_uint32, C.c_int, u64, i64, dp, ip]),

@@ -11,6 +11,7 @@ beta = 6 plaquette.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -184,3 +185,65 @@ def test_replayed_device_chain_matches_reference_plaquette():
     z = abs(pool.pooled_mean - 0.5992920042685975) / math.hypot(pool.pooled_se, 0.0008268453534905086)
     assert z < 2.0, (pool.pooled_mean, pool.pooled_se, z)
     assert abs(np.mean(acc) - 0.8675) < 0.005, acc
+
+
+def test_replay_plaquette_matches_reference_exact():
+    """ymd_su3_average_plaquette (the CPU chains' measurement in the level-(c)
+    comparisons) against the reference-exact average_plaquette: identity
+    gives 1, a hot-start FP64 configuration agrees to 1e-13."""
+    import ctypes as C
+    L = O.lib()
+    n = (4, 6, 4, 6)
+    vh = L.ymd_slots_per_parity(O.ext(n))
+    buf = np.zeros(8 * 18 * vh)
+    aos = O.su3_identity(n)
+    assert L.ymd_su3_to_device(O.ext(n), O.dptr(aos), buf.ctypes.data_as(C.c_void_p), 64) == 0
+    v = C.c_double()
+    assert L.ymd_su3_average_plaquette(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, C.byref(v)) == 0
+    assert v.value == 1.0
+    assert L.ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, 4242) == 0
+    p = O.DevSu3Params(beta=6.0, angle=0.35, hits=8, reunitarize=1, key=9, sweep=0)
+    a, b = C.c_uint64(), C.c_uint64()
+    assert L.ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, C.byref(p), C.byref(a),
+                                      C.byref(b)) == 0
+    assert L.ymd_su3_from_device(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, O.dptr(aos)) == 0
+    ref = C.c_double()
+    assert L.ymo_su3_average_plaquette(O.ext(n), O.dptr(aos), C.byref(ref)) == 0
+    assert L.ymd_su3_average_plaquette(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, C.byref(v)) == 0
+    assert abs(v.value - ref.value) <= 1e-13
+
+
+def test_replay_is_independent_of_the_thread_count():
+    """The replay runs the links of a sub-pass on OpenMP threads: 1 and 3
+    threads give the same links, counts and plaquette, bit for bit."""
+    import subprocess
+    import sys
+    code = r'''
+import ctypes as C, hashlib, sys
+import numpy as np
+sys.path.insert(0, "TESTS")
+import oracle_lib as O
+L = O.lib()
+n = (4, 4, 6, 8)
+vh = L.ymd_slots_per_parity(O.ext(n))
+b3 = np.zeros(8 * 18 * vh, dtype=np.float32)
+assert L.ymd_su3_hot_start(O.ext(n), b3.ctypes.data_as(C.c_void_p), 32, 5) == 0
+p = O.DevSu3Params(beta=6.0, angle=0.35, hits=10, reunitarize=1, key=3, sweep=1)
+a, c = C.c_uint64(), C.c_uint64()
+assert L.ymd_su3_metropolis_sweep(O.ext(n), b3.ctypes.data_as(C.c_void_p), 32, C.byref(p), C.byref(a), C.byref(c)) == 0
+b2 = np.zeros(8 * 4 * vh, dtype=np.float32)
+assert L.ymd_su2_hot_start(O.ext(n), b2.ctypes.data_as(C.c_void_p), 32, 5) == 0
+u, w = C.c_uint64(), C.c_uint64()
+assert L.ymd_su2_heatbath_sweep(O.ext(n), b2.ctypes.data_as(C.c_void_p), 32, 2.3, 3, 1, 10000, C.byref(u), C.byref(w)) == 0
+assert L.ymd_su2_overrelax_sweep(O.ext(n), b2.ctypes.data_as(C.c_void_p), 32) == 0
+v = C.c_double()
+assert L.ymd_su3_average_plaquette(O.ext(n), b3.ctypes.data_as(C.c_void_p), 32, C.byref(v)) == 0
+print(hashlib.sha1(b3.tobytes() + b2.tobytes()).hexdigest(), a.value, c.value, u.value, w.value, v.value.hex())
+'''.replace("TESTS", os.path.dirname(os.path.abspath(__file__)))
+    outs = []
+    for t in ("1", "3"):
+        env = dict(os.environ, OMP_NUM_THREADS=t)
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        outs.append(r.stdout.strip())
+    assert outs[0] == outs[1], outs

@@ -107,6 +107,70 @@ def test_level_c_against_reference_chains_in_the_same_job(ym):
     assert 0.56 <= cm <= 0.63  # the reference's external envelope (su3_benchmark.rs:23-28)
 
 
+def _su3_fp32_gpu_vs_fp64_cpu(ym, L, equil, chains, sweeps, hits=10):
+    """Level (c) at the benchmark lattice: one configuration equilibrated on
+    the GPU (FP32, `equil` sweeps from cold), then `chains` FP32 GPU chains
+    and `chains` FP64 CPU chains (the device-order replay with FP64 link
+    storage, threaded over each sub-pass) continue from it under their own
+    keys for `sweeps` sweeps, the plaquette after every sweep.  Returns the
+    per-chain mean plaquettes of both sides."""
+    import ctypes as C
+
+    import oracle_lib as O
+
+    n = (L,) * 4
+    master = 0x3C0FFEE000003200 + L
+    lat = ym.Su3Lattice4(n, 32)
+    lat.metropolis_sweeps(6.0, 0.35, ym.derive_su3_chain_seed(master, 0), 0, equil, hits)
+    gpu = []
+    for c in range(chains):
+        g = lat.clone()
+        key = ym.derive_su3_chain_seed(master, 100 + c)
+        pl = []
+        for s in range(sweeps):
+            g.metropolis_sweeps(6.0, 0.35, key, s, 1, hits)
+            pl.append(g.average_plaquette())
+        gpu.append(np.mean(pl))
+    # the FP64 chains start from the FP64 Gram-Schmidt projection of the same links
+    r64 = ym.Su3Lattice4(n, 64)
+    r64.upload(lat.download(), checked=False)
+    r64.reunitarize()
+    vh = O.lib().ymd_slots_per_parity(O.ext(n))
+    start = np.zeros(8 * 18 * vh)
+    assert O.lib().ymd_su3_to_device(O.ext(n), O.dptr(r64.download()), start.ctypes.data_as(C.c_void_p), 64) == 0
+    cpu = []
+    for c in range(chains):
+        buf = start.copy()
+        key = ym.derive_su3_chain_seed(master, 200 + c)
+        pl = []
+        for s in range(sweeps):
+            prm = O.DevSu3Params(beta=6.0, angle=0.35, hits=hits, reunitarize=1, key=key, sweep=s)
+            a, b = C.c_uint64(), C.c_uint64()
+            assert O.lib().ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, C.byref(prm),
+                                                   C.byref(a), C.byref(b)) == 0
+            v = C.c_double()
+            assert O.lib().ymd_su3_average_plaquette(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, C.byref(v)) == 0
+            pl.append(v.value)
+        cpu.append(np.mean(pl))
+    return gpu, cpu
+
+
+@pytest.mark.parametrize("L,equil,sweeps", [(16, 2000, 40), (32, 2000, 8)])
+def test_su3_level_c_fp32_gpu_against_fp64_cpu_chains(ym, L, equil, sweeps):
+    """Level (c) at 16^4 and at the benchmark's 32^4, beta = 6.0, 10 hits:
+    8 FP32 GPU chains and 8 FP64 CPU chains continuing from one
+    GPU-equilibrated configuration agree in the pooled plaquette within 2
+    sigma of the jackknife over chains."""
+    print(f"[level-c] SU(3) {L}^4: {equil} equilibration sweeps, 8 + 8 chains x {sweeps} sweeps", flush=True)
+    gpu, cpu = _su3_fp32_gpu_vs_fp64_cpu(ym, L, equil, 8, sweeps)
+    gm, gse = _jackknife_over_chains(gpu)
+    cm, cse = _jackknife_over_chains(cpu)
+    z = abs(gm - cm) / math.hypot(gse, cse)
+    print(f"[level-c] {L}^4 GPU FP32 {gm:.7f} +- {gse:.7f}, CPU FP64 {cm:.7f} +- {cse:.7f}, z = {z:.2f}", flush=True)
+    assert z < 2.0, (gm, gse, cm, cse, z)
+    assert 0.58 < gm < 0.61
+
+
 # docs/PUBLISHED_PLAQUETTE_BASELINE.md:109-126: two 10^4 chains at beta = 2.3,
 # 40 + 48 Markov sweeps (1 heat-bath + 1 over-relaxation), pooled mean and SE
 PUBLISHED_SU2_MEAN, PUBLISHED_SU2_SE = 0.6022919679191184, 0.0004173286094490
