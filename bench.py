@@ -386,6 +386,35 @@ def side_workload(name):
                          "heatbath_hbm_fraction": upd * 144 / (hb * 1e-3) / (HBM_PEAK_GBS * 1e9)})
         return {"workload": "SU(2) FP32 links, hot start, beta=2.3, checkerboard heat-bath / over-relaxation "
                             "(144 B per link update: 9 transfers of 16 B)", "results": rows}
+    if name == "stout":
+        rows = []
+        for ext in (24, 32):
+            n = (ext,) * 4
+            for prec in (32, 64):
+                src = ym_amd.Su3Lattice4(n, prec)
+                key = ym_amd.derive_su3_chain_seed(0x2C02198105010002, 3)
+                src.hot_start(key)
+                src.metropolis_sweeps(6.0, 0.35, key, 0, 20, 10)
+                dst = ym_amd.Su3Lattice4(n, prec)
+                ym_amd.lattice.lib().ym_su3_stout_spatial(dst.handle, src.handle, 0.1, 10)  # warm-up
+                dst.timer_begin()
+                st = ym_amd.lattice.lib().ym_su3_stout_spatial(dst.handle, src.handle, 0.1, 10)
+                ms = dst.timer_end()
+                assert st == 0
+                V = ext ** 4
+                # per level: the 3V spatial links read and written, the V temporal links copied
+                # (read + write), neighbours of the staples from the caches: 8V link transfers
+                per_level_bytes = 8 * V * (72 if prec == 32 else 144)
+                us = ms * 1e3 / 10
+                log(f"stout {ext}^4 FP{prec}: {us:.1f} us per level")
+                rows.append({"extent": ext, "precision": prec, "exp": "Cayley-Hamilton" if prec == 32 else "Taylor",
+                             "us_per_level": us, "spatial_links_per_s": 3 * V / (us * 1e-6),
+                             "algorithmic_bytes_per_level": per_level_bytes,
+                             "hbm_fraction": per_level_bytes / (us * 1e-6) / (HBM_PEAK_GBS * 1e9),
+                             "plaquette_after_10": dst.average_plaquette()})
+        return {"workload": "SU(3) spatial stout smearing, 10 levels at rho=0.1 (config 3), thermalised "
+                            "24^4 / 32^4, beta=6.0; device time by one event pair on the output lattice's stream",
+                "results": rows}
     if name == "su2-precision":
         rows = []
         for ext in (32, 48):
@@ -446,7 +475,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="su3",
-                    choices=["su3", "su2-sweeps", "su2-precision", "su3-precision", "ensemble", "plaquette"],
+                    choices=["su3", "su2-sweeps", "su2-precision", "su3-precision", "ensemble", "plaquette", "stout"],
                     help="su3 (default): the SU(3) 32^4 sweep metric; the others print their own JSON line")
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)

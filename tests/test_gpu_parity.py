@@ -454,6 +454,29 @@ def test_a1pp_fp64_vs_reference(ym):
         assert abs(pl[t] - p.value) <= REL and abs(rc[t] - r.value) <= REL
 
 
+@pytest.mark.parametrize("sweeps,levels", [(0, 1), (0, 3), (20, 1), (20, 4)])
+def test_stout_fp32_closed_form_against_fp64_series(ym, sweeps, levels):
+    """FP32 storage smears with exp(iQ) by Cayley-Hamilton (Morningstar-Peardon),
+    FP64 storage with the reference's scaled Taylor series (pinned to the
+    reference restatement at 1e-13).  On the same links -- a hot start, where
+    Q ranges widely and Tr Q^3 takes both signs, and a configuration after 20
+    sweeps -- the two agree link by link to FP32 rounding; temporal links
+    are copied unchanged."""
+    n = (8, 8, 8, 6)
+    lat = ym.Su3Lattice4(n, 32)
+    lat.hot_start(31)
+    if sweeps:
+        lat.metropolis_sweeps(6.0, 0.35, 5, 0, sweeps, 8)
+    links = lat.download()
+    twin = ym.Su3Lattice4(n, 64)
+    twin.upload(links, checked=False)
+    a = ym.spatial_stout_smear(lat, 0.1, levels).download().reshape(-1, 4, 18)
+    b = ym.spatial_stout_smear(twin, 0.1, levels).download().reshape(-1, 4, 18)
+    assert np.abs(a - b).max() < 1e-6 * (1 + levels), np.abs(a - b).max()
+    assert np.array_equal(a[:, 3], links.reshape(-1, 4, 18)[:, 3])
+    assert np.abs(a[:, :3] - links.reshape(-1, 4, 18)[:, :3]).max() > 0.01  # the spatial links moved
+
+
 def test_stout_frozen_cycle_fixture(ym):
     """glueball-spatial-stout-phase3b1-v1.json: one level at rho = 0.1."""
     n = (3, 3, 3, 3)

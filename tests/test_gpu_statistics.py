@@ -228,6 +228,9 @@ def test_su2_level_c_against_reference_chains_and_published_mean(ym, prec):
     z = abs(gm - cm) / math.hypot(gse, cse)
     assert z < 2.0, (gm, gse, cm, cse, z)
     zp = abs(gm - PUBLISHED_SU2_MEAN) / math.hypot(gse, PUBLISHED_SU2_SE)
+    print(f"[level-c] SU(2) 10^4 FP{prec} GPU {gm:.6f} +- {gse:.6f}, CPU reference-order {cm:.6f} +- {cse:.6f}, "
+          f"z = {z:.2f}; published {PUBLISHED_SU2_MEAN:.6f}, z = {zp:.2f}; acceptance {np.mean(acc):.4f} vs "
+          f"{np.mean([c[1] for c in cpu]):.4f}", flush=True)
     assert zp < 2.0, (gm, gse, zp)
     assert abs(np.mean(acc) - np.mean([c[1] for c in cpu])) < 0.003, (np.mean(acc), [c[1] for c in cpu])
     assert 0.59 <= gm <= 0.62  # the published source-resolution envelope (su2_benchmark.rs:23-24)

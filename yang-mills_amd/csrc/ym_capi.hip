@@ -43,6 +43,32 @@ void launch_fill_zero_i32(int *p, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_fill_zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
 }
 
+// >= one slot per wave of 256-thread blocks over Vh slots (the finest grid of
+// the sweep kernels) and >= the 256 blocks of the wave-specialised kernel
+int64_t tally_slots(const Geom &g) { return std::max<int64_t>(((int64_t)g.vh + 255) / 256 * 4, 1024); }
+
+// One block: out[0] = sum of the n slots, eight independent loads in flight
+// per thread.
+__global__ __launch_bounds__(1024) void k_tally_sum(const unsigned long long *__restrict__ t, int64_t n,
+                                                    unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long red[16];
+    unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t i = threadIdx.x;
+    for (; i + 7 * 1024 < n; i += 8 * 1024) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] += t[i + j * 1024];
+    }
+    for (; i < n; i += 1024) v[0] += t[i];
+    unsigned long long w = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w += v[j];
+    w = block_sum_u64<1024>(w, red);
+    if (threadIdx.x == 0) out[0] = w;
+}
+void launch_tally_sum(const unsigned long long *tally, int64_t n, unsigned long long *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_tally_sum, dim3(1), dim3(1024), 0, st, tally, n, out);
+}
+
 int64_t stage_sites_for(int64_t volume) { return std::min<int64_t>(volume, (int64_t)1 << 16); }
 
 int32_t upload_aos_staged(int group, int prec, void *links, const Geom &g, const double *aos, double *stage,
@@ -178,6 +204,7 @@ struct ym_lattice {
     double *dres = nullptr;  // small result buffer (64 doubles)
     double *hres = nullptr;  // pinned host copy of dres (64 doubles)
     unsigned long long *dctr = nullptr;  // 4 counters
+    unsigned long long *tally = nullptr; // tally_slots(g) per-block counts of the sweep kernels
     int *dstatus = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t aux0 = nullptr, aux1 = nullptr;  // timing of the learned sweeps
@@ -203,6 +230,7 @@ int32_t lattice_info(const ym_lattice *l, int32_t *group, int32_t *prec, int64_t
 }
 void *lattice_links(const ym_lattice *l) { return l->links; }
 unsigned long long *lattice_counters(const ym_lattice *l) { return l->dctr; }
+unsigned long long *lattice_tally(const ym_lattice *l) { return l->tally; }
 void lattice_aux_events(const ym_lattice *l, hipEvent_t *e0, hipEvent_t *e1) {
     *e0 = l->aux0;
     *e1 = l->aux1;
@@ -443,6 +471,8 @@ int32_t ym_lattice_create(int32_t group, const int64_t extents[4], int32_t preci
     if (!st && hipHostMalloc((void **)&l->hres, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         st = YM_ALLOCATION_FAILED;
     if (!st && hipMalloc((void **)&l->dctr, 4 * sizeof(unsigned long long)) != hipSuccess) st = YM_ALLOCATION_FAILED;
+    if (!st && hipMalloc((void **)&l->tally, (size_t)ym::tally_slots(l->g) * sizeof(unsigned long long)) != hipSuccess)
+        st = YM_ALLOCATION_FAILED;
     if (!st && hipMalloc((void **)&l->dstatus, sizeof(int)) != hipSuccess) st = YM_ALLOCATION_FAILED;
     if (!st && (hipEventCreate(&l->ev0) != hipSuccess || hipEventCreate(&l->ev1) != hipSuccess)) st = YM_HIP_ERROR;
     if (!st && (hipEventCreate(&l->aux0) != hipSuccess || hipEventCreate(&l->aux1) != hipSuccess)) st = YM_HIP_ERROR;
@@ -463,6 +493,7 @@ int32_t ym_lattice_destroy(ym_lattice *l) {
     if (l->dres) (void)hipFree(l->dres);
     if (l->hres) (void)hipHostFree(l->hres);
     if (l->dctr) (void)hipFree(l->dctr);
+    if (l->tally) (void)hipFree(l->tally);
     if (l->dstatus) (void)hipFree(l->dstatus);
     if (l->ev0) (void)hipEventDestroy(l->ev0);
     if (l->ev1) (void)hipEventDestroy(l->ev1);
@@ -539,12 +570,14 @@ int32_t ym_su3_metropolis_sweeps(ym_lattice *l, double beta, double angle, int32
     if (!std::isfinite(angle) || angle < 1.0e-8 || angle > 1.0) return YM_ANGLE_OUT_OF_RANGE;
     if (!all_even(l)) return YM_UNSUPPORTED;
     const int reunit = (l->prec == 32) ? 1 : (reunitarize != 0);
-    ym::launch_fill_zero_u64(l->dctr, 1, l->stream);
+    const int64_t nt = ym::tally_slots(l->g);
+    ym::launch_fill_zero_u64(l->tally, (size_t)nt, l->stream);
     for (int k = 0; k < n_sweeps; ++k)
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p)
                 ym::launch_su3_metropolis(l->prec, l->links, l->g, mu, p, beta, angle, hits, key,
-                                          first_sweep + (uint64_t)k, reunit, l->dctr, l->stream);
+                                          first_sweep + (uint64_t)k, reunit, l->tally, l->stream);
+    ym::launch_tally_sum(l->tally, nt, l->dctr, l->stream);
     YM_HIP(hipGetLastError());
     unsigned long long acc = 0;
     YM_HIP(hipMemcpyAsync(&acc, l->dctr, sizeof(acc), hipMemcpyDeviceToHost, l->stream));
@@ -590,7 +623,7 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
             for (int mu = 0; mu < 4; ++mu)
                 for (int p = 0; p < 2; ++p)
                     ym::launch_su3_metropolis(l->prec, scratch, g, mu, p, beta, angle, hits, 0x5EEDull, (uint64_t)k,
-                                              1, l->dctr, l->stream);
+                                              1, l->tally, l->stream);
         }
         float ms = 0.0f;
         if (!st) st = hip_status(hipEventRecord(l->ev1, l->stream));
@@ -621,7 +654,7 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
             for (int mu = 0; mu < 4; ++mu)
                 for (int p = 0; p < 2; ++p)
                     ym::launch_su3_metropolis(l->prec, scratch, g, mu, p, beta, angle, hits, 0x5EEDull, (uint64_t)k,
-                                              1, l->dctr, l->stream);
+                                              1, l->tally, l->stream);
         }
         float ms = 0.0f;
         if (!st) st = hip_status(hipEventRecord(l->ev1, l->stream));
@@ -662,16 +695,18 @@ int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *l, double beta, double ang
     for (auto &e : ev)
         if (hipEventCreate(&e) != hipSuccess) st = YM_HIP_ERROR;
     if (!st) {
-        ym::launch_fill_zero_u64(l->dctr, 1, l->stream);
+        const int64_t nt = ym::tally_slots(l->g);
+        ym::launch_fill_zero_u64(l->tally, (size_t)nt, l->stream);
         int k = 0;
         for (int s = 0; s < n_sweeps; ++s)
             for (int mu = 0; mu < 4; ++mu)
                 for (int p = 0; p < 2; ++p, ++k) {
                     if (per_launch || k == 0) (void)hipEventRecord(ev[per_launch ? 2 * k : 0], l->stream);
                     ym::launch_su3_metropolis(l->prec, l->links, l->g, mu, p, beta, angle, hits, key,
-                                              first_sweep + (uint64_t)s, reunit, l->dctr, l->stream);
+                                              first_sweep + (uint64_t)s, reunit, l->tally, l->stream);
                     if (per_launch || k == nl - 1) (void)hipEventRecord(ev[per_launch ? 2 * k + 1 : 1], l->stream);
                 }
+        ym::launch_tally_sum(l->tally, nt, l->dctr, l->stream);
         unsigned long long acc = 0;
         st = hip_status(hipGetLastError());
         if (!st) st = hip_status(hipMemcpyAsync(&acc, l->dctr, sizeof(acc), hipMemcpyDeviceToHost, l->stream));
@@ -702,13 +737,16 @@ int32_t ym_su2_heatbath_sweeps(ym_lattice *l, double beta, uint64_t key, uint64_
     if (!std::isfinite(beta) || beta < 0.0 || beta > 8.0) return YM_BETA_OUT_OF_RANGE;
     if (max_attempts <= 0) return YM_ZERO_ATTEMPT_LIMIT;
     if (!all_even(l)) return YM_UNSUPPORTED;
+    const int64_t nt = ym::tally_slots(l->g);
+    ym::launch_fill_zero_u64(l->tally, (size_t)nt, l->stream);
     ym::launch_fill_zero_u64(l->dctr, 2, l->stream);
     ym::launch_fill_zero_i32(l->dstatus, 1, l->stream);
     for (int k = 0; k < n_sweeps; ++k)
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p)
                 ym::launch_su2_heatbath(l->prec, l->links, l->g, mu, p, beta, key, first_sweep + (uint64_t)k,
-                                        max_attempts, l->dctr, l->dstatus, l->stream);
+                                        max_attempts, l->tally, l->dctr, l->dstatus, l->stream);
+    ym::launch_tally_sum(l->tally, nt, l->dctr, l->stream);
     YM_HIP(hipGetLastError());
     unsigned long long ctr[2] = {0, 0};
     int status = 0;
@@ -716,8 +754,9 @@ int32_t ym_su2_heatbath_sweeps(ym_lattice *l, double beta, uint64_t key, uint64_
     YM_HIP(hipMemcpyAsync(&status, l->dstatus, sizeof(int), hipMemcpyDeviceToHost, l->stream));
     int32_t st = finish(l);
     if (st) return st;
+    // counter 1 holds the links that exhausted the attempt limit
     if (proposals) *proposals += ctr[0];
-    if (updates) *updates += ctr[1];
+    if (updates) *updates += (uint64_t)n_sweeps * 4ull * (uint64_t)l->g.volume - ctr[1];
     return status ? YM_HEATBATH_ATTEMPT_LIMIT : YM_OK;
 }
 
@@ -980,6 +1019,15 @@ int32_t ym_su3_stout_spatial(ym_lattice *dst, const ym_lattice *src, double rho,
         if (e) return e;
     }
     void *tmp = dst->spare;
+    // The kernels write the spatial links only: the temporal (mu = 3) groups,
+    // unchanged by smearing, are copied once into each output buffer (groups
+    // (3, 0) and (3, 1) are the last quarter of the tiled field).
+    const size_t quarter = src->link_bytes / 4;
+    YM_HIP(hipMemcpyAsync((char *)dst->links + 3 * quarter, (const char *)src->links + 3 * quarter, quarter,
+                          hipMemcpyDeviceToDevice, dst->stream));
+    if (levels > 1)
+        YM_HIP(hipMemcpyAsync((char *)tmp + 3 * quarter, (const char *)src->links + 3 * quarter, quarter,
+                              hipMemcpyDeviceToDevice, dst->stream));
     // Level k writes dst when (levels-1-k) is even, tmp otherwise, so the
     // buffers alternate and the last level lands in dst.
     const void *cur = src->links;
@@ -1179,17 +1227,20 @@ int32_t ym_su2_markov_sweeps(ym_lattice *l, double beta, uint64_t key, uint64_t 
     if (overrelaxation_sweeps_per_heatbath < 0 || overrelaxation_sweeps_per_heatbath > 16) return YM_TOO_MANY_OR;
     if (max_attempts <= 0) return YM_ZERO_ATTEMPT_LIMIT;
     if (!all_even(l)) return YM_UNSUPPORTED;
+    const int64_t nt = ym::tally_slots(l->g);
+    ym::launch_fill_zero_u64(l->tally, (size_t)nt, l->stream);
     ym::launch_fill_zero_u64(l->dctr, 2, l->stream);
     ym::launch_fill_zero_i32(l->dstatus, 1, l->stream);
     for (int k = 0; k < n_sweeps; ++k) {
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p)
                 ym::launch_su2_heatbath(l->prec, l->links, l->g, mu, p, beta, key, first_sweep + (uint64_t)k,
-                                        max_attempts, l->dctr, l->dstatus, l->stream);
+                                        max_attempts, l->tally, l->dctr, l->dstatus, l->stream);
         for (int o = 0; o < overrelaxation_sweeps_per_heatbath; ++o)
             for (int mu = 0; mu < 4; ++mu)
                 for (int p = 0; p < 2; ++p) ym::launch_su2_overrelax(l->prec, l->links, l->g, mu, p, l->stream);
     }
+    ym::launch_tally_sum(l->tally, nt, l->dctr, l->stream);
     YM_HIP(hipGetLastError());
     unsigned long long ctr[2] = {0, 0};
     int status = 0;
@@ -1197,8 +1248,9 @@ int32_t ym_su2_markov_sweeps(ym_lattice *l, double beta, uint64_t key, uint64_t 
     YM_HIP(hipMemcpyAsync(&status, l->dstatus, sizeof(int), hipMemcpyDeviceToHost, l->stream));
     const int32_t st = finish(l);
     if (st) return st;
+    // counter 1 holds the links that exhausted the attempt limit
     if (proposals) *proposals += ctr[0];
-    if (updates) *updates += ctr[1];
+    if (updates) *updates += (uint64_t)n_sweeps * 4ull * (uint64_t)l->g.volume - ctr[1];
     return status ? YM_HEATBATH_ATTEMPT_LIMIT : YM_OK;
 }
 

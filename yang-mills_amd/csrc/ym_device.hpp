@@ -358,10 +358,17 @@ __device__ __forceinline__ double fmaR(double a, double b, double c) { return __
 __host__ __device__ __forceinline__ int ntile_of(int vh) { return (vh + 63) >> 6; }
 __host__ __device__ __forceinline__ size_t su3_pairs(int vh) { return (size_t)8 * ntile_of(vh) * 576; }
 
+// Addresses are a group base (mu, parity: uniform across a wavefront in every
+// kernel here, so it is formed once in scalar registers) plus a 32-bit
+// element index inside the group (a group holds at most 2^23 tiles of 576
+// pairs, below 2^32), so no per-lane 64-bit multiply is needed.
 template <typename Real>
 __device__ __forceinline__ const typename Vec2<Real>::type *link3_ptr(const typename Vec2<Real>::type *L, int vh, int mu,
                                                                      int par, int s) {
-    return L + ((size_t)(mu * 2 + par) * ntile_of(vh) + (size_t)(s >> 6)) * 576 + (s & 63);
+    const typename Vec2<Real>::type *grp = L + (size_t)((mu * 2 + par) * ntile_of(vh)) * 576;
+    const uint32_t tile = (uint32_t)(s >> 6);
+    __builtin_assume(tile < (1u << 23));  // slots < 2^29 (at most 2^30 sites)
+    return grp + (tile * 576u + (uint32_t)(s & 63));
 }
 template <typename Real>
 __device__ __forceinline__ M3<Real> load3(const typename Vec2<Real>::type *__restrict__ L, int vh, int mu, int par, int s) {
@@ -396,9 +403,10 @@ __device__ __forceinline__ void store3(typename Vec2<Real>::type *__restrict__ L
 // storage keeps the 9 runs of double2 (already 16 bytes per lane).
 template <>
 __device__ __forceinline__ M3<float> load3<float>(const float2 *__restrict__ L, int vh, int mu, int par, int s) {
-    const size_t tile = (size_t)(mu * 2 + par) * ntile_of(vh) + (size_t)(s >> 6);
-    const int lane = s & 63;
-    const float4 *q = reinterpret_cast<const float4 *>(L) + tile * 288 + lane;
+    const float2 *grp = L + (size_t)((mu * 2 + par) * ntile_of(vh)) * 576;
+    const uint32_t tile = (uint32_t)(s >> 6), lane = (uint32_t)(s & 63);
+    __builtin_assume(tile < (1u << 23));  // slots < 2^29 (at most 2^30 sites)
+    const float4 *q = reinterpret_cast<const float4 *>(grp) + (tile * 288u + lane);
     M3<float> m;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -408,19 +416,20 @@ __device__ __forceinline__ M3<float> load3<float>(const float2 *__restrict__ L, 
         m.v[4 * k + 2] = t.z;
         m.v[4 * k + 3] = t.w;
     }
-    const float2 t = L[tile * 576 + 512 + lane];
+    const float2 t = grp[tile * 576u + 512u + lane];
     m.v[16] = t.x;
     m.v[17] = t.y;
     return m;
 }
 template <>
 __device__ __forceinline__ void store3<float>(float2 *__restrict__ L, int vh, int mu, int par, int s, const M3<float> &m) {
-    const size_t tile = (size_t)(mu * 2 + par) * ntile_of(vh) + (size_t)(s >> 6);
-    const int lane = s & 63;
-    float4 *q = reinterpret_cast<float4 *>(L) + tile * 288 + lane;
+    float2 *grp = L + (size_t)((mu * 2 + par) * ntile_of(vh)) * 576;
+    const uint32_t tile = (uint32_t)(s >> 6), lane = (uint32_t)(s & 63);
+    __builtin_assume(tile < (1u << 23));  // slots < 2^29 (at most 2^30 sites)
+    float4 *q = reinterpret_cast<float4 *>(grp) + (tile * 288u + lane);
 #pragma unroll
     for (int k = 0; k < 4; ++k) q[k * 64] = make_float4(m.v[4 * k], m.v[4 * k + 1], m.v[4 * k + 2], m.v[4 * k + 3]);
-    L[tile * 576 + 512 + lane] = make_float2(m.v[16], m.v[17]);
+    grp[tile * 576u + 512u + lane] = make_float2(m.v[16], m.v[17]);
 }
 
 // T = X Y^dagger

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::
                                                            int p, double nbt, double angle, int hits,
                                                            const uint64_t *__restrict__ ctr, int sweep_k, int reunit,
                                                            unsigned long long *__restrict__ acc) {
-    __shared__ LdsMat<Real> abuf[2], ubuf[2];
+    constexpr int NB = 2;
+    __shared__ LdsMat<Real> abuf[NB], ubuf[NB];
     const bool producer = threadIdx.x < 256;
     const int lane = threadIdx.x & 255;
     const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
@@ -118,30 +119,35 @@ __global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::
             M3<Real> A;
             su3_staple_at<Real, MU>(L, g, p, s, x, A);
             const M3<Real> U = load3<Real>(L, g.vh, MU, p, s);
-            abuf[i & 1].put(lane, A);
-            ubuf[i & 1].put(lane, U);
+            abuf[i % NB].put(lane, A);
+            ubuf[i % NB].put(lane, U);
         }
     };
-    if (producer && nsteps > 0) produce(0);
-    __syncthreads();
-    for (int i = 0; i < nsteps; ++i) {
-        if (producer) {
-            if (i + 1 < nsteps) produce(i + 1);
-        } else {
-            const int T = tile_at(i);
-            const int c = tile_chain[T];
-            const Geom g = cd[c].g;
-            typename Vec2<Real>::type *L = links + cd[c].link_off;
-            const int s = tile_slot(g, T - begin[c], lane);
-            unsigned long long nacc = 0;
-            int x[4];
-            if (s < g.vh && site_of(g, p, s, x)) {
-                const M3<Real> A = abuf[i & 1].get(lane), U = ubuf[i & 1].get(lane);
-                nacc = su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, cd[c].key, sweep, reunit);
-            }
+    auto consume = [&](int i) {
+        const int T = tile_at(i);
+        const int c = tile_chain[T];
+        const Geom g = cd[c].g;
+        typename Vec2<Real>::type *L = links + cd[c].link_off;
+        const int s = tile_slot(g, T - begin[c], lane);
+        unsigned long long nacc = 0;
+        int x[4];
+        if (s < g.vh && site_of(g, p, s, x)) {
+            const M3<Real> A = abuf[i % NB].get(lane), U = ubuf[i % NB].get(lane);
+            nacc = su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, cd[c].key, sweep, reunit);
+        }
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) nacc += __shfl_down(nacc, off, 64);
-            if ((lane & 63) == 0 && nacc) atomicAdd(acc + c, nacc);
+        for (int off = 32; off > 0; off >>= 1) nacc += __shfl_down(nacc, off, 64);
+        if ((lane & 63) == 0 && nacc) atomicAdd(acc + c, nacc);
+    };
+    // Phase t of the block: the producers stage tile t while the consumers
+    // update tile t - 1 (n + 1 phases for n tiles); produce and consume have
+    // one call site each, so the kernel carries one copy of their code.
+    const int nphase = nsteps == 0 ? 0 : nsteps + 1;
+    for (int t = 0; t < nphase; ++t) {
+        if (producer) {
+            if (t < nsteps) produce(t);
+        } else if (t > 0) {
+            consume(t - 1);
         }
         __syncthreads();
     }

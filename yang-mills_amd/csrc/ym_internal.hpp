@@ -14,13 +14,14 @@ struct Geom;
 int32_t lattice_info(const ym_lattice *lat, int32_t *group, int32_t *prec, int64_t ext[4]);
 void *lattice_links(const ym_lattice *lat);
 unsigned long long *lattice_counters(const ym_lattice *lat);  // 4 device counters of the handle
+unsigned long long *lattice_tally(const ym_lattice *lat);     // tally_slots(g) slots of the handle
 void lattice_aux_events(const ym_lattice *lat, hipEvent_t *e0, hipEvent_t *e1);
 hipStream_t lattice_stream(const ym_lattice *lat);
 // The handle's byte buffer, grown to at least `bytes` (nullptr if that fails).
 void *lattice_aux(const ym_lattice *lat, size_t bytes);
 
 void launch_su3_metropolis(int prec, void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
-                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *acc, hipStream_t st);
+                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, hipStream_t st);
 void launch_su3_hot(int prec, void *links, const Geom &g, uint64_t key, hipStream_t st);
 void launch_su3_staple_probe(int prec, const void *links, const Geom &g, const int x[4], int mu, double *out, hipStream_t st);
 void launch_su3_reunit(int prec, void *links, const Geom &g, hipStream_t st);
@@ -33,8 +34,10 @@ void launch_su3_to_aos_range(int prec, const void *links, const Geom &g, double 
                              hipStream_t st);
 void launch_su3_identity(int prec, void *links, const Geom &g, hipStream_t st);
 
+// attempts go to the tally, links that exhaust max_attempts to counters[1]
 void launch_su2_heatbath(int prec, void *links, const Geom &g, int mu, int p, double beta, uint64_t key, uint64_t sweep,
-                         int max_attempts, unsigned long long *counters, int *status, hipStream_t st);
+                         int max_attempts, unsigned long long *tally, unsigned long long *counters, int *status,
+                         hipStream_t st);
 void launch_su2_overrelax(int prec, void *links, const Geom &g, int mu, int p, hipStream_t st);
 void launch_su2_hot(int prec, void *links, const Geom &g, uint64_t key, hipStream_t st);
 void launch_su2_from_aos(int prec, void *links, const Geom &g, const double *aos, hipStream_t st);
@@ -64,6 +67,16 @@ void launch_max_partials(const double *part, int n, int ncols, int stride, doubl
 void launch_fill_zero(double *p, size_t n, hipStream_t st);
 void launch_fill_zero_u64(unsigned long long *p, size_t n, hipStream_t st);
 void launch_fill_zero_i32(int *p, size_t n, hipStream_t st);
+
+// Counts of the sweep kernels (accepted proposals, heat-bath attempts) go to
+// a tally of per-block slots: block b (wave b for the heat-bath sampler) of a
+// launch adds its count to slot b with a plain load and store.  One writer
+// per slot per launch, launches ordered on one stream: no atomics, which on
+// one address cost the sweep kernels tens of microseconds per launch.
+// tally_slots(g) slots cover every grid those launchers use on g;
+// launch_tally_sum stores the sum of n slots to out[0].
+int64_t tally_slots(const Geom &g);
+void launch_tally_sum(const unsigned long long *tally, int64_t n, unsigned long long *out, hipStream_t st);
 
 // Host AoS image (reference layout) <-> tiled field of one lattice through a
 // device staging buffer of `stage_sites` sites: the image is moved in

@@ -500,10 +500,11 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
     const unsigned long long t0 = block_sum_u64<256>(tried, red);
     const unsigned long long t1 = block_sum_u64<256>(accepted, red);
     const unsigned long long t2 = block_sum_u64<256>(macc, red);
+    // tally slots: column j of block b at counts[j * gridDim.x + b]
     if (threadIdx.x == 0) {
-        if (t0) atomicAdd(counts + 0, t0);
-        if (t1) atomicAdd(counts + 1, t1);
-        if (t2) atomicAdd(counts + 2, t2);
+        counts[blockIdx.x] += t0;
+        counts[gridDim.x + blockIdx.x] += t1;
+        counts[2 * gridDim.x + blockIdx.x] += t2;
     }
 }
 
@@ -1152,21 +1153,22 @@ int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *m, double beta, uin
     // several lattices (own streams, own host threads) may sweep with one
     // network at the same time.
     unsigned long long *cnt = ym::lattice_counters(lat);
+    unsigned long long *tally = ym::lattice_tally(lat);  // >= 3 nb slots
     hipEvent_t e0, e1;
     ym::lattice_aux_events(lat, &e0, &e1);
+    const int nb = (g.vh + 255) / 256;
     st = hs(hipStreamSynchronize(m->stream));
     if (!st) {
-        ym::launch_fill_zero_u64(cnt, 3, ls);
+        ym::launch_fill_zero_u64(tally, 3 * (size_t)nb, ls);
         st = hs(hipEventRecord(e0, ls));
     }
-    const int nb = (g.vh + 255) / 256;
     for (int k = 0; k < n_sweeps && !st; ++k)
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p) {
                 const uint64_t sw = first_sweep + (uint64_t)k;
 #define YM_LEARNED_LAUNCH(MU, BF)                                                                                   \
     hipLaunchKernelGGL((ym::k_su3_learned<MU, BF>), dim3(nb), dim3(256), 0, ls, L, g, p, m->params, m->packed, \
-                       beta, angle, metro_hits, key, sw, cnt)
+                       beta, angle, metro_hits, key, sw, tally)
                 if (bf16) {
                     switch (mu) {
                     case 0: YM_LEARNED_LAUNCH(0, true); break;
@@ -1186,6 +1188,7 @@ int32_t ym_su3_learned_sweeps(ym_lattice *lat, const ym_mlp *m, double beta, uin
             }
     if (!st) st = hs(hipGetLastError());
     if (!st) st = hs(hipEventRecord(e1, ls));
+    for (int j = 0; j < 3 && !st; ++j) ym::launch_tally_sum(tally + (size_t)j * nb, nb, cnt + j, ls);
     unsigned long long h[3] = {0, 0, 0};
     if (!st) st = hs(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, ls));
     if (!st) st = hs(hipStreamSynchronize(ls));

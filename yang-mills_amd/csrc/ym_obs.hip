@@ -333,8 +333,10 @@ __global__ __launch_bounds__(128) void k_stout(const typename Vec2<Real>::type *
     const int s = blockIdx.x * 128 + threadIdx.x;
     if (s < g.vh) stout_link<Real>(in, out, g, rho, blockIdx.y, s, blockIdx.z);
 }
+// Spatial directions only (grid.z = 3): the temporal links are the same at
+// every level, so the caller copies them into the output buffers once.
 void launch_stout(int prec, const void *in, void *out, const Geom &g, double rho, hipStream_t st) {
-    dim3 grid((g.vh + 127) / 128, 2, 4);
+    dim3 grid((g.vh + 127) / 128, 2, 3);
     if (prec == 32) hipLaunchKernelGGL((k_stout<float>), grid, dim3(128), 0, st, (const float2 *)in, (float2 *)out, g, rho);
     else hipLaunchKernelGGL((k_stout<double>), grid, dim3(128), 0, st, (const double2 *)in, (double2 *)out, g, rho);
 }

@@ -215,6 +215,88 @@ __device__ __forceinline__ M3<double> expm_i_hermitian(const M3<double> &Q) {
     return sum;
 }
 
+// exp(iQ) of a traceless Hermitian 3 x 3 matrix by the Cayley-Hamilton
+// theorem (Morningstar & Peardon, Phys. Rev. D 69, 054501 (2004), section
+// III): exp(iQ) = f0 I + f1 Q + f2 Q^2 with c0 = det Q = Tr(Q^3)/3,
+// c1 = Tr(Q^2)/2, theta = arccos(|c0| / c0max) / 3, c0max = 2 (c1/3)^(3/2),
+// u = sqrt(c1/3) cos theta, w = sqrt(c1) sin theta and
+//   h0 = (u^2 - w^2) e^{2iu} + e^{-iu} (8 u^2 cos w + 2iu (3u^2 + w^2) xi0(w)),
+//   h1 = 2u e^{2iu} - e^{-iu} (2u cos w - i (3u^2 - w^2) xi0(w)),
+//   h2 = e^{2iu} - e^{-iu} (cos w + 3iu xi0(w)),
+//   f_j = h_j / (9u^2 - w^2),  xi0(w) = sin(w) / w;
+// for c0 < 0 the f_j of |c0| give f_j(c0) = (-1)^j conj(f_j(|c0|)).  With
+// |c0| the angle stays in [0, pi/6], so 9u^2 - w^2 >= 8u^2 and only Q = 0
+// makes the denominator vanish (then exp(iQ) = 1 + iQ - Q^2/2 to the order
+// that matters).  FP64 scalars; one matrix product (Q^2).
+__device__ __forceinline__ M3<double> expm_i_hermitian_ch(const M3<double> &Q) {
+    const M3<double> Q2 = mul_nn(Q, Q);
+    double trq3 = 0.0, trq2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        trq2 += YM_RE(Q2, i, i);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) trq3 += YM_RE(Q, i, k) * YM_RE(Q2, k, i) - YM_IM(Q, i, k) * YM_IM(Q2, k, i);
+    }
+    const double c0 = trq3 / 3.0, c1 = 0.5 * trq2;
+    double f0r, f0i, f1r, f1i, f2r, f2i;
+    if (!(c1 > 1.0e-30)) {
+        f0r = 1.0; f0i = 0.0; f1r = 0.0; f1i = 1.0; f2r = -0.5; f2i = 0.0;
+    } else {
+        const double neg = c0 < 0.0;
+        const double s13 = sqrt(c1 / 3.0);
+        const double c0max = 2.0 * (c1 / 3.0) * s13;
+        const double ratio = fmin(fabs(c0) / c0max, 1.0);
+        const double theta = acos(ratio) / 3.0;
+        double st, ct;
+        sincos(theta, &st, &ct);
+        const double u = s13 * ct, w = sqrt(c1) * st;
+        const double u2 = u * u, w2 = w * w;
+        double sw, cw, su, cu, s2u, c2u;
+        sincos(w, &sw, &cw);
+        sincos(u, &su, &cu);
+        sincos(2.0 * u, &s2u, &c2u);
+        const double xi0 = fabs(w) < 0.05 ? 1.0 - w2 / 6.0 * (1.0 - w2 / 20.0 * (1.0 - w2 / 42.0)) : sw / w;
+        // e^{-iu} = cu - i su, e^{2iu} = c2u + i s2u
+        // h0
+        const double a0r = 8.0 * u2 * cw, a0i = 2.0 * u * (3.0 * u2 + w2) * xi0;
+        const double h0r = (u2 - w2) * c2u + (cu * a0r + su * a0i);
+        const double h0i = (u2 - w2) * s2u + (cu * a0i - su * a0r);
+        // h1 = 2u e^{2iu} - e^{-iu} (2u cos w - i (3u^2 - w^2) xi0)
+        const double a1r = 2.0 * u * cw, a1i = -(3.0 * u2 - w2) * xi0;
+        const double h1r = 2.0 * u * c2u - (cu * a1r + su * a1i);
+        const double h1i = 2.0 * u * s2u - (cu * a1i - su * a1r);
+        // h2 = e^{2iu} - e^{-iu} (cos w + 3iu xi0)
+        const double a2r = cw, a2i = 3.0 * u * xi0;
+        const double h2r = c2u - (cu * a2r + su * a2i);
+        const double h2i = s2u - (cu * a2i - su * a2r);
+        const double inv = 1.0 / (9.0 * u2 - w2);
+        f0r = h0r * inv; f0i = h0i * inv;
+        f1r = h1r * inv; f1i = h1i * inv;
+        f2r = h2r * inv; f2i = h2i * inv;
+        if (neg) {
+            f0i = -f0i;
+            f1r = -f1r;
+            f2i = -f2i;
+        }
+    }
+    M3<double> E;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double qr = YM_RE(Q, i, j), qi = YM_IM(Q, i, j), pr = YM_RE(Q2, i, j), pi = YM_IM(Q2, i, j);
+            double er = f1r * qr - f1i * qi + f2r * pr - f2i * pi;
+            double ei = f1r * qi + f1i * qr + f2r * pi + f2i * pr;
+            if (i == j) {
+                er += f0r;
+                ei += f0i;
+            }
+            YM_RE(E, i, j) = er;
+            YM_IM(E, i, j) = ei;
+        }
+    return E;
+}
+
 // One stout step of the link (mu, parity p, slot s < vh) from `in` into `out`.
 template <typename Real>
 __device__ __forceinline__ void stout_link(const typename Vec2<Real>::type *__restrict__ in,
@@ -273,7 +355,12 @@ __device__ __forceinline__ void stout_link(const typename Vec2<Real>::type *__re
             YM_RE(Q, i, j) = qr;
             YM_IM(Q, i, j) = qi;
         }
-    const M3<double> R = expm_i_hermitian(Q);
+    // FP64 storage keeps the reference's scaled Taylor series (the
+    // rotation of smear_one_level, glueball_smearing.rs:236-238, is
+    // Su3::from_algebra_coefficients -> matrix_exponential, su3.rs:395-450)
+    // for parity to 1e-13; FP32 storage uses the closed form, accurate far
+    // below its rounding at a fraction of the series' cost
+    const M3<double> R = sizeof(Real) == 8 ? expm_i_hermitian(Q) : expm_i_hermitian_ch(Q);
     const M3<double> Un = mul_nn(R, U);
     M3<Real> o;
 #pragma unroll

@@ -76,39 +76,47 @@ __device__ __forceinline__ bool hb_attempt(uint64_t key, uint32_t site, int mu, 
     return u2 * u2 <= __builtin_fma(-r0, r0, 1.0);
 }
 
-// The rejection loop is run by the whole wavefront together.  Each round the
-// P links still without an accepted attempt get k = 64 / 2^ceil(log2 P) lanes
-// each (a power of two, so lane -> (link, offset) is a shift and a mask);
-// lane o of link r evaluates attempt next_r + o.  A link takes the lowest
-// accepted attempt index of the round (or moves next on by k), so every link
-// ends with exactly the attempt, r0 and proposal count of its own sequential
-// loop, while a wave needs about (total attempts) / 64 rounds instead of the
-// largest attempt count among its 64 links.  The pending links' parameters
-// and the accepted r0 pass through per-wave LDS tables indexed by rank.
+// Attempts per lane and round of the cooperative rejection loop.
+constexpr int kHbAttemptsPerLane = 2;
+
+// One heat-bath sub-pass: thread = link, staple from the neighbours, then the
+// rejection loop run by the whole wavefront together.  Each round the P links
+// still without an accepted attempt get k = 64 / 2^ceil(log2 P) lanes each (a
+// power of two, so lane -> (link, offset) is a shift and a mask), and lane o
+// of link r evaluates the A = kHbAttemptsPerLane attempts next_r + A o ...
+// next_r + A o + A - 1 (independent Philox streams: A dependency chains in
+// flight per lane).  A link takes the lowest accepted attempt index of the
+// round (or moves next on by A k), so every link ends with exactly the
+// attempt, r0 and proposal count of its own sequential loop, while a wave
+// needs about (total attempts) / (64 A) rounds instead of the largest
+// attempt count among its 64 links.  The pending links' parameters and the
+// accepted attempt pass through per-wave LDS tables indexed by rank.  No
+// block-wide barrier: a wave leaves as soon as its own links are done.  Its
+// attempt count goes to its own tally slot; links that exhaust the attempt
+// limit (rare) are counted by an atomic.
 template <typename Real, int MU>
 __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type *__restrict__ L, Geom g, int p,
                                                       double beta, uint64_t key, uint64_t sweep, int max_attempts,
+                                                      unsigned long long *__restrict__ tally,
                                                       unsigned long long *__restrict__ counters,
                                                       int *__restrict__ status) {
-    __shared__ unsigned long long red[4];
-    __shared__ double2 tab_par[4][64];  // (1/alpha, tm) of pending link `rank`
-    __shared__ int2 tab_idx[4][64];     // (site, next attempt) of pending link `rank`
-    __shared__ double tab_r0[4][64];    // r0 of the first accepted attempt of link `rank`
+    constexpr int A = kHbAttemptsPerLane, NW = 4;
+    __shared__ double2 tab_par[NW][64];  // (1/alpha, tm) of pending link `rank`
+    __shared__ int2 tab_idx[NW][64];     // (site, next attempt) of pending link `rank`
+    __shared__ double tab_r0[NW][64];    // r0 of the first accepted attempt of link `rank`
+    __shared__ int tab_first[NW][64];    // its offset from next
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // parity-1 sub-passes walk the tiles backwards (the field of a 48^4
-    // lattice is larger than the memory-side cache; see k_su3_metropolis_ws)
     const int tile = (p & 1) ? xcd_tile_rev(blockIdx.x, gridDim.x) : xcd_tile(blockIdx.x, gridDim.x);
     const int s = tile * 256 + threadIdx.x;
-    unsigned long long nprop = 0, nupd = 0;
     int x[4] = {0, 0, 0, 0};
     const bool has = s < g.vh && site_of(g, p, s, x);
     Q4<double> a;
     double norm = 0.0, inv_alpha = 0.0, tm = 0.0;
     uint32_t site = 0;
     if (has) {
-        const Q4<Real> Af = su2_staple<Real, MU>(L, g, x, p, s);
+        const Q4<Real> A4 = su2_staple<Real, MU>(L, g, x, p, s);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) a.c[c] = (double)Af.c[c];
+        for (int c = 0; c < 4; ++c) a.c[c] = (double)A4.c[c];
         norm = __builtin_sqrt(qnorm2(a));
         const double alpha = beta * norm;
         site = (uint32_t)lex_of(g, x);
@@ -118,7 +126,8 @@ __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type 
         }
     }
     bool pending = has, ok = false;
-    int next = 0;
+    int next = 0;  // < max_attempts while pending
+    unsigned long long nprop = 0;
     double r0 = 0.0;
     const unsigned long long below = (1ull << lane) - 1ull;
     while (true) {
@@ -136,16 +145,34 @@ __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type 
         const int r = lane >> sh, o = lane & (k - 1);
         bool hit = false;
         double tr0 = 0.0;
+        int first = 0;
         if (r < P) {
             const double2 par = tab_par[wave][r];
             const int2 idx = tab_idx[wave][r];
-            const int att = idx.y + o;
-            hit = att < max_attempts && hb_attempt(key, (uint32_t)idx.x, MU, att, sweep, par.x, par.y, tr0);
+            const int left = max_attempts - idx.y;  // > 0: attempts idx.y + j exist for j < left
+            double ra[A];
+            bool ha[A];
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                const int off = A * o + j;
+                ra[j] = 0.0;
+                ha[j] = off < left && hb_attempt(key, (uint32_t)idx.x, MU, idx.y + off, sweep, par.x, par.y, ra[j]);
+            }
+#pragma unroll
+            for (int j = A - 1; j >= 0; --j)
+                if (ha[j]) {
+                    hit = true;
+                    tr0 = ra[j];
+                    first = A * o + j;
+                }
         }
         const unsigned long long am = __ballot(hit);
-        // the first accepted lane of each link's group publishes its r0
+        // the first accepted lane of each link's group publishes its attempt
         const unsigned long long grp = (k == 64) ? ~0ull : (((1ull << k) - 1ull) << (r << sh));
-        if (hit && (am & grp & below) == 0ull) tab_r0[wave][r] = tr0;
+        if (hit && (am & grp & below) == 0ull) {
+            tab_r0[wave][r] = tr0;
+            tab_first[wave][r] = first;
+        }
         wave_lds_sync();
         if (pending) {
             const unsigned long long mine = (k == 64) ? am : ((am >> (rank << sh)) & ((1ull << k) - 1ull));
@@ -153,54 +180,52 @@ __global__ __launch_bounds__(256) void k_su2_heatbath(typename Vec4<Real>::type 
                 ok = true;
                 pending = false;
                 r0 = tab_r0[wave][rank];
-                nprop = (unsigned long long)(next + __ffsll((long long)mine));
+                nprop = (unsigned long long)next + (unsigned long long)tab_first[wave][rank] + 1ull;
+            } else if (next >= max_attempts - A * k) {
+                pending = false;
+                nprop = (unsigned long long)max_attempts;
             } else {
-                next += k;
-                if (next >= max_attempts) {
-                    pending = false;
-                    nprop = (unsigned long long)max_attempts;
-                }
+                next += A * k;
             }
         }
         wave_lds_sync();
     }
-    if (has) {
-        if (!ok) {
-            atomicOr(status, 1);
+    if (has && ok) {
+        const double radial = __builtin_sqrt(fmax(__builtin_fma(-r0, r0, 1.0), 0.0));
+        uint32_t o[4];
+        draw(key, site, MU, KIND_HB_DIRECTION, 0, sweep, o);
+        const double z = 2.0 * unit53(o[0], o[1]) - 1.0;
+        double sn, cs;
+        sincos_turns(unit53(o[2], o[3]), sn, cs);
+        const double planar = __builtin_sqrt(fmax(__builtin_fma(-z, z, 1.0), 0.0));
+        const double rp = radial * planar;
+        Q4<double> R;
+        R.c[0] = r0; R.c[1] = rp * cs; R.c[2] = rp * sn; R.c[3] = radial * z;
+        Q4<double> un;
+        if (norm == 0.0) {
+            un = R;
         } else {
-            nupd = 1;
-            const double radial = __builtin_sqrt(fmax(__builtin_fma(-r0, r0, 1.0), 0.0));
-            uint32_t o[4];
-            draw(key, site, MU, KIND_HB_DIRECTION, 0, sweep, o);
-            const double z = 2.0 * unit53(o[0], o[1]) - 1.0;
-            double sn, cs;
-            sincos_turns(unit53(o[2], o[3]), sn, cs);
-            const double planar = __builtin_sqrt(fmax(__builtin_fma(-z, z, 1.0), 0.0));
-            const double rp = radial * planar;
-            Q4<double> R;
-            R.c[0] = r0; R.c[1] = rp * cs; R.c[2] = rp * sn; R.c[3] = radial * z;
-            Q4<double> un;
-            if (norm == 0.0) {
-                un = R;
-            } else {
-                const double inv_norm = 1.0 / norm;
-                Q4<double> ad;
-                ad.c[0] = a.c[0] * inv_norm; ad.c[1] = -(a.c[1] * inv_norm);
-                ad.c[2] = -(a.c[2] * inv_norm); ad.c[3] = -(a.c[3] * inv_norm);
-                un = qmul(R, ad);
-            }
-            qnormalize(un);
-            Q4<Real> v;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) v.c[c] = (Real)un.c[c];
-            store2<Real>(L, g.vh, MU, p, s, v);
+            const double inv_norm = 1.0 / norm;
+            Q4<double> ad;
+            ad.c[0] = a.c[0] * inv_norm; ad.c[1] = -(a.c[1] * inv_norm);
+            ad.c[2] = -(a.c[2] * inv_norm); ad.c[3] = -(a.c[3] * inv_norm);
+            un = qmul(R, ad);
         }
+        qnormalize(un);
+        Q4<Real> v;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v.c[c] = (Real)un.c[c];
+        store2<Real>(L, g.vh, MU, p, s, v);
     }
-    const unsigned long long tp = block_sum_u64<256>(nprop, red);
-    const unsigned long long tu = block_sum_u64<256>(nupd, red);
-    if (threadIdx.x == 0) {
-        if (tp) atomicAdd(&counters[0], tp);
-        if (tu) atomicAdd(&counters[1], tu);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nprop += __shfl_down(nprop, off, 64);
+    const unsigned long long failed = __ballot(has && !ok);
+    if (lane == 0) {
+        tally[blockIdx.x * NW + wave] += nprop;
+        if (failed) {
+            atomicAdd(&counters[1], (unsigned long long)__popcll(failed));
+            atomicOr(status, 1);
+        }
     }
 }
 
@@ -231,16 +256,20 @@ __global__ __launch_bounds__(256) void k_su2_overrelax(typename Vec4<Real>::type
 
 template <typename Real>
 static void hb_t(void *links, const Geom &g, int mu, int p, double beta, uint64_t key, uint64_t sweep, int max_att,
-                 unsigned long long *ctr, int *status, hipStream_t st) {
+                 unsigned long long *tally, unsigned long long *ctr, int *status, hipStream_t st) {
     using V4 = typename Vec4<Real>::type;
     const int nb = (g.vh + 255) / 256;
     V4 *L = (V4 *)links;
+#define YM_SU2_HB(M)                                                                                     \
+    hipLaunchKernelGGL((k_su2_heatbath<Real, M>), dim3(nb), dim3(256), 0, st, L, g, p, beta, key, sweep, \
+                       max_att, tally, ctr, status)
     switch (mu) {
-    case 0: hipLaunchKernelGGL((k_su2_heatbath<Real, 0>), dim3(nb), dim3(256), 0, st, L, g, p, beta, key, sweep, max_att, ctr, status); break;
-    case 1: hipLaunchKernelGGL((k_su2_heatbath<Real, 1>), dim3(nb), dim3(256), 0, st, L, g, p, beta, key, sweep, max_att, ctr, status); break;
-    case 2: hipLaunchKernelGGL((k_su2_heatbath<Real, 2>), dim3(nb), dim3(256), 0, st, L, g, p, beta, key, sweep, max_att, ctr, status); break;
-    default: hipLaunchKernelGGL((k_su2_heatbath<Real, 3>), dim3(nb), dim3(256), 0, st, L, g, p, beta, key, sweep, max_att, ctr, status); break;
+    case 0: YM_SU2_HB(0); break;
+    case 1: YM_SU2_HB(1); break;
+    case 2: YM_SU2_HB(2); break;
+    default: YM_SU2_HB(3); break;
     }
+#undef YM_SU2_HB
 }
 template <typename Real>
 static void or_t(void *links, const Geom &g, int mu, int p, hipStream_t st) {
@@ -255,9 +284,10 @@ static void or_t(void *links, const Geom &g, int mu, int p, hipStream_t st) {
     }
 }
 void launch_su2_heatbath(int prec, void *links, const Geom &g, int mu, int p, double beta, uint64_t key, uint64_t sweep,
-                         int max_attempts, unsigned long long *counters, int *status, hipStream_t st) {
-    if (prec == 32) hb_t<float>(links, g, mu, p, beta, key, sweep, max_attempts, counters, status, st);
-    else hb_t<double>(links, g, mu, p, beta, key, sweep, max_attempts, counters, status, st);
+                         int max_attempts, unsigned long long *tally, unsigned long long *counters, int *status,
+                         hipStream_t st) {
+    if (prec == 32) hb_t<float>(links, g, mu, p, beta, key, sweep, max_attempts, tally, counters, status, st);
+    else hb_t<double>(links, g, mu, p, beta, key, sweep, max_attempts, tally, counters, status, st);
 }
 void launch_su2_overrelax(int prec, void *links, const Geom &g, int mu, int p, hipStream_t st) {
     if (prec == 32) or_t<float>(links, g, mu, p, st);

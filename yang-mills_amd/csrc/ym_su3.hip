@@ -20,12 +20,13 @@
 namespace ym {
 
 // One sub-pass (MU, p) in one kernel: staple then update per thread; the
-// accepted-proposal count is reduced per block.
+// accepted-proposal count is reduced per block and added to the block's
+// tally slot.
 template <typename Real, int MU>
 __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::type *__restrict__ L, Geom g, int p,
                                                         double nbt, double angle, int hits, uint64_t key,
                                                         uint64_t sweep, int reunit,
-                                                        unsigned long long *__restrict__ acc_out) {
+                                                        unsigned long long *__restrict__ tally) {
     __shared__ unsigned long long red[4];
     const int s = tile_slot(g, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
     unsigned long long nacc = 0;
@@ -36,7 +37,7 @@ __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::typ
         nacc = su3_update_at<Real, MU>(L, g, p, s, x, A, nbt, angle, hits, key, sweep, reunit);
     }
     unsigned long long tot = block_sum_u64<256>(nacc, red);
-    if (threadIdx.x == 0 && tot) atomicAdd(acc_out, tot);
+    if (threadIdx.x == 0) tally[blockIdx.x] += tot;
 }
 
 // Wave-specialised sub-pass for lattices of at least 256 tiles: one block of
@@ -56,12 +57,16 @@ __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::typ
 // larger than the 256 MB cache: the same direction every time evicts each
 // line just before it is needed again).  Tiles are independent, so the
 // direction changes timing only.
+// (A three-buffer schedule, with the producers two tiles ahead and the
+// idle waves of the first and last phase helping, measured slower on 32^4
+// and 24^4: the kernel is throughput-bound, not bound by those phases.)
 template <typename Real, int MU>
 __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::type *__restrict__ L, Geom g, int p,
                                                            double nbt, double angle, int hits, uint64_t key,
                                                            uint64_t sweep, int reunit, int ntiles,
-                                                           unsigned long long *__restrict__ acc_out) {
-    __shared__ LdsMat<Real> abuf[2], ubuf[2];
+                                                           unsigned long long *__restrict__ tally) {
+    constexpr int NB = 2;
+    __shared__ LdsMat<Real> abuf[NB], ubuf[NB];
     __shared__ unsigned long long red[8];
     const bool producer = threadIdx.x < 256;
     const int lane = threadIdx.x & 255;
@@ -80,32 +85,37 @@ __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::
             M3<Real> A;
             su3_staple_at<Real, MU>(L, g, p, s, x, A);
             const M3<Real> U = load3<Real>(L, g.vh, MU, p, s);
-            abuf[i & 1].put(lane, A);
-            ubuf[i & 1].put(lane, U);
+            abuf[i % NB].put(lane, A);
+            ubuf[i % NB].put(lane, U);
         }
     };
-    if (producer && nsteps > 0) produce(0);
-    __syncthreads();
-    for (int i = 0; i < nsteps; ++i) {
+    auto consume = [&](int i) {
+        const int s = tile_slot(g, tile_at(i), lane);
+        int x[4];
+        if (s < g.vh && site_of(g, p, s, x)) {
+            const M3<Real> A = abuf[i % NB].get(lane), U = ubuf[i % NB].get(lane);
+            nacc += su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, key, sweep, reunit);
+        }
+    };
+    // Phase t of the block: the producers stage tile t while the consumers
+    // update tile t - 1 (n + 1 phases for n tiles); produce and consume have
+    // one call site each, so the kernel carries one copy of their code.
+    const int nphase = nsteps == 0 ? 0 : nsteps + 1;
+    for (int t = 0; t < nphase; ++t) {
         if (producer) {
-            if (i + 1 < nsteps) produce(i + 1);
-        } else {
-            const int s = tile_slot(g, tile_at(i), lane);
-            int x[4];
-            if (s < g.vh && site_of(g, p, s, x)) {
-                const M3<Real> A = abuf[i & 1].get(lane), U = ubuf[i & 1].get(lane);
-                nacc += su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, key, sweep, reunit);
-            }
+            if (t < nsteps) produce(t);
+        } else if (t > 0) {
+            consume(t - 1);
         }
         __syncthreads();
     }
     const unsigned long long tot = block_sum_u64<512>(nacc, red);
-    if (threadIdx.x == 0 && tot) atomicAdd(acc_out, tot);
+    if (threadIdx.x == 0) tally[blockIdx.x] += tot;
 }
 
 template <typename Real>
 static void launch_metro_t(void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
-                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *acc, hipStream_t st) {
+                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, hipStream_t st) {
     using V2 = typename Vec2<Real>::type;
     const double nbt = -(beta / 3.0);
     const int nb = (g.vh + 255) / 256;
@@ -114,24 +124,24 @@ static void launch_metro_t(void *links, const Geom &g, int mu, int p, double bet
         // one 8-wave block per CU
         const int grid = 256;
         switch (mu) {
-        case 0: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 0>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, acc); break;
-        case 1: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 1>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, acc); break;
-        case 2: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 2>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, acc); break;
-        default: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 3>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, acc); break;
+        case 0: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 0>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally); break;
+        case 1: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 1>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally); break;
+        case 2: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 2>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally); break;
+        default: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 3>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally); break;
         }
         return;
     }
     switch (mu) {
-    case 0: hipLaunchKernelGGL((k_su3_metropolis<Real, 0>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
-    case 1: hipLaunchKernelGGL((k_su3_metropolis<Real, 1>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
-    case 2: hipLaunchKernelGGL((k_su3_metropolis<Real, 2>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
-    default: hipLaunchKernelGGL((k_su3_metropolis<Real, 3>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, acc); break;
+    case 0: hipLaunchKernelGGL((k_su3_metropolis<Real, 0>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, tally); break;
+    case 1: hipLaunchKernelGGL((k_su3_metropolis<Real, 1>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, tally); break;
+    case 2: hipLaunchKernelGGL((k_su3_metropolis<Real, 2>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, tally); break;
+    default: hipLaunchKernelGGL((k_su3_metropolis<Real, 3>), dim3(nb), dim3(256), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, tally); break;
     }
 }
 void launch_su3_metropolis(int prec, void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
-                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *acc, hipStream_t st) {
-    if (prec == 32) launch_metro_t<float>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, acc, st);
-    else launch_metro_t<double>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, acc, st);
+                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, hipStream_t st) {
+    if (prec == 32) launch_metro_t<float>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, tally, st);
+    else launch_metro_t<double>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, tally, st);
 }
 
 // ----------------------------------------------------------- hot start
