@@ -116,6 +116,7 @@ enum {
     YM_WRONG_CONFIGURATION = 90,      /* benchmark gate: schedule does not allow the split */
     YM_STATIONARITY_FAILED = 91,      /* benchmark gate: constant half or zero spread */
     YM_WORKSPACE_EXCEEDED = 95,       /* input larger than an allocation-free entry point's fixed workspace */
+    YM_KERNEL_HANDOFF_TIMEOUT = 94,   /* a sweep kernel's bounded wave-to-wave wait expired (no reference counterpart) */
     YM_INVALID_ARGUMENT = 96,         /* null pointer, wrong group/precision, size mismatch */
     YM_NO_DEVICE = 97,
     YM_HIP_ERROR = 98,

@@ -419,6 +419,7 @@ const char *ym_status_string(int32_t s) {
     case YM_SU3_NON_FINITE_MATRIX: return "SU(3) link has a non-finite entry";
     case YM_SU3_NON_UNITARY: return "SU(3) link violates the unitarity tolerance";
     case YM_SU3_DETERMINANT_NOT_ONE: return "SU(3) link determinant differs from one beyond the tolerance";
+    case YM_KERNEL_HANDOFF_TIMEOUT: return "a wave of a sweep kernel waited past its bound for its partner wave";
     case YM_WORKSPACE_EXCEEDED: return "input exceeds the fixed device workspace of an allocation-free entry point";
     case YM_INVALID_ARGUMENT: return "invalid argument";
     case YM_NO_DEVICE: return "no HIP device";
@@ -572,17 +573,21 @@ int32_t ym_su3_metropolis_sweeps(ym_lattice *l, double beta, double angle, int32
     const int reunit = (l->prec == 32) ? 1 : (reunitarize != 0);
     const int64_t nt = ym::tally_slots(l->g);
     ym::launch_fill_zero_u64(l->tally, (size_t)nt, l->stream);
+    ym::launch_fill_zero_i32(l->dstatus, 1, l->stream);
     for (int k = 0; k < n_sweeps; ++k)
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p)
                 ym::launch_su3_metropolis(l->prec, l->links, l->g, mu, p, beta, angle, hits, key,
-                                          first_sweep + (uint64_t)k, reunit, l->tally, l->stream);
+                                          first_sweep + (uint64_t)k, reunit, l->tally, l->dstatus, l->stream);
     ym::launch_tally_sum(l->tally, nt, l->dctr, l->stream);
     YM_HIP(hipGetLastError());
     unsigned long long acc = 0;
+    int status = 0;
     YM_HIP(hipMemcpyAsync(&acc, l->dctr, sizeof(acc), hipMemcpyDeviceToHost, l->stream));
+    YM_HIP(hipMemcpyAsync(&status, l->dstatus, sizeof(int), hipMemcpyDeviceToHost, l->stream));
     int32_t st = finish(l);
     if (st) return st;
+    if (status) return YM_KERNEL_HANDOFF_TIMEOUT;
     if (proposals) *proposals += (uint64_t)n_sweeps * (uint64_t)hits * 4ull * (uint64_t)l->g.volume;
     if (accepts) *accepts += acc;
     return YM_OK;
@@ -623,7 +628,7 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
             for (int mu = 0; mu < 4; ++mu)
                 for (int p = 0; p < 2; ++p)
                     ym::launch_su3_metropolis(l->prec, scratch, g, mu, p, beta, angle, hits, 0x5EEDull, (uint64_t)k,
-                                              1, l->tally, l->stream);
+                                              1, l->tally, l->dstatus, l->stream);
         }
         float ms = 0.0f;
         if (!st) st = hip_status(hipEventRecord(l->ev1, l->stream));
@@ -654,7 +659,7 @@ int32_t ym_su3_autotune(ym_lattice *l, double beta, double angle, int32_t hits, 
             for (int mu = 0; mu < 4; ++mu)
                 for (int p = 0; p < 2; ++p)
                     ym::launch_su3_metropolis(l->prec, scratch, g, mu, p, beta, angle, hits, 0x5EEDull, (uint64_t)k,
-                                              1, l->tally, l->stream);
+                                              1, l->tally, l->dstatus, l->stream);
         }
         float ms = 0.0f;
         if (!st) st = hip_status(hipEventRecord(l->ev1, l->stream));
@@ -697,20 +702,25 @@ int32_t ym_su3_metropolis_sweeps_profiled(ym_lattice *l, double beta, double ang
     if (!st) {
         const int64_t nt = ym::tally_slots(l->g);
         ym::launch_fill_zero_u64(l->tally, (size_t)nt, l->stream);
+        ym::launch_fill_zero_i32(l->dstatus, 1, l->stream);
         int k = 0;
         for (int s = 0; s < n_sweeps; ++s)
             for (int mu = 0; mu < 4; ++mu)
                 for (int p = 0; p < 2; ++p, ++k) {
                     if (per_launch || k == 0) (void)hipEventRecord(ev[per_launch ? 2 * k : 0], l->stream);
                     ym::launch_su3_metropolis(l->prec, l->links, l->g, mu, p, beta, angle, hits, key,
-                                              first_sweep + (uint64_t)s, reunit, l->tally, l->stream);
+                                              first_sweep + (uint64_t)s, reunit, l->tally, l->dstatus,
+                                              l->stream);
                     if (per_launch || k == nl - 1) (void)hipEventRecord(ev[per_launch ? 2 * k + 1 : 1], l->stream);
                 }
         ym::launch_tally_sum(l->tally, nt, l->dctr, l->stream);
         unsigned long long acc = 0;
+        int status = 0;
         st = hip_status(hipGetLastError());
         if (!st) st = hip_status(hipMemcpyAsync(&acc, l->dctr, sizeof(acc), hipMemcpyDeviceToHost, l->stream));
+        if (!st) st = hip_status(hipMemcpyAsync(&status, l->dstatus, sizeof(int), hipMemcpyDeviceToHost, l->stream));
         if (!st) st = finish(l);
+        if (!st && status) st = YM_KERNEL_HANDOFF_TIMEOUT;
         if (!st) {
             float sum = 0.f, ms = 0.f;
             for (int i = 0; per_launch && i < nl && !st; ++i) {

@@ -85,30 +85,34 @@ __device__ __forceinline__ double exp_neg(double d) {
     return __builtin_ldexp(p, (int)k);
 }
 
-// sin x and cos x - 1, |x| <= pi/4.
+// sin x and cos x - 1, |x| <= pi/4: Taylor series to x^19 / x^20 by Horner
+// in y = -x^2, so every coefficient is a positive 1/n!, the same constants as
+// exp_neg's (one register copy each where the compiler keeps them live).
+// Rounding is odd-symmetric, so this is bit for bit the series in x^2 with
+// alternating coefficients.
 __device__ __forceinline__ void sincos_small(double x, double &s, double &cm1) {
-    double x2 = x * x;
-    double ps = -0x1.2f49b46814157p-57;
-    ps = __builtin_fma(ps, x2, 0x1.952c77030ad4ap-49);
-    ps = __builtin_fma(ps, x2, -0x1.ae7f3e733b81fp-41);
-    ps = __builtin_fma(ps, x2, 0x1.6124613a86d09p-33);
-    ps = __builtin_fma(ps, x2, -0x1.ae64567f544e4p-26);
-    ps = __builtin_fma(ps, x2, 0x1.71de3a556c734p-19);
-    ps = __builtin_fma(ps, x2, -0x1.a01a01a01a01ap-13);
-    ps = __builtin_fma(ps, x2, 0x1.1111111111111p-7);
-    ps = __builtin_fma(ps, x2, -0x1.5555555555555p-3);
-    s = __builtin_fma(x * x2, ps, x);
-    double pc = 0x1.e542ba4020225p-62;
-    pc = __builtin_fma(pc, x2, -0x1.6827863b97d97p-53);
-    pc = __builtin_fma(pc, x2, 0x1.ae7f3e733b81fp-45);
-    pc = __builtin_fma(pc, x2, -0x1.93974a8c07c9dp-37);
-    pc = __builtin_fma(pc, x2, 0x1.1eed8eff8d898p-29);
-    pc = __builtin_fma(pc, x2, -0x1.27e4fb7789f5cp-22);
-    pc = __builtin_fma(pc, x2, 0x1.a01a01a01a01ap-16);
-    pc = __builtin_fma(pc, x2, -0x1.6c16c16c16c17p-10);
-    pc = __builtin_fma(pc, x2, 0x1.5555555555555p-5);
-    pc = __builtin_fma(pc, x2, -0x1p-1);
-    cm1 = x2 * pc;
+    const double y = -(x * x);
+    double ps = 0x1.2f49b46814157p-57;  // 1/19!
+    ps = __builtin_fma(ps, y, 0x1.952c77030ad4ap-49);
+    ps = __builtin_fma(ps, y, 0x1.ae7f3e733b81fp-41);
+    ps = __builtin_fma(ps, y, 0x1.6124613a86d09p-33);
+    ps = __builtin_fma(ps, y, 0x1.ae64567f544e4p-26);
+    ps = __builtin_fma(ps, y, 0x1.71de3a556c734p-19);
+    ps = __builtin_fma(ps, y, 0x1.a01a01a01a01ap-13);
+    ps = __builtin_fma(ps, y, 0x1.1111111111111p-7);
+    ps = __builtin_fma(ps, y, 0x1.5555555555555p-3);
+    s = __builtin_fma(x * y, ps, x);
+    double pc = 0x1.e542ba4020225p-62;  // 1/20!
+    pc = __builtin_fma(pc, y, 0x1.6827863b97d97p-53);
+    pc = __builtin_fma(pc, y, 0x1.ae7f3e733b81fp-45);
+    pc = __builtin_fma(pc, y, 0x1.93974a8c07c9dp-37);
+    pc = __builtin_fma(pc, y, 0x1.1eed8eff8d898p-29);
+    pc = __builtin_fma(pc, y, 0x1.27e4fb7789f5cp-22);
+    pc = __builtin_fma(pc, y, 0x1.a01a01a01a01ap-16);
+    pc = __builtin_fma(pc, y, 0x1.6c16c16c16c17p-10);
+    pc = __builtin_fma(pc, y, 0x1.5555555555555p-5);
+    pc = __builtin_fma(pc, y, 0x1p-1);
+    cm1 = y * pc;
 }
 
 // sin(2 pi t), cos(2 pi t) for t in [0, 1).
@@ -292,6 +296,30 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Point-to-point handoff between two waves of one block through an LDS
+// flag holding a count.  post: every earlier LDS access of the wave is
+// complete (release fence, workgroup scope, LDS only: the wave's global
+// stores stay in flight), then one lane stores the count.
+// wait: spin (s_sleep between reads) until the flag reaches `target`, then
+// acquire.  The partner wave of the same block is always resident and never
+// waits on the waiting wave for that flag, so the spin ends; a bound of
+// 2^24 reads (about a second) turns a logic error into status bit 2 instead
+// of a hang.
+__device__ __forceinline__ void wave_handoff_post(int *flag, int count, bool leader) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    if (leader) __hip_atomic_store(flag, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void wave_handoff_wait(const int *flag, int target, int *status) {
+    for (int it = 0; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target; ++it) {
+        if (it >= (1 << 24)) {
+            atomicOr(status, 2);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Logical block index with consecutive tiles kept on one XCD: the hardware

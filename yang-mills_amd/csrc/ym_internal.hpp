@@ -21,7 +21,8 @@ hipStream_t lattice_stream(const ym_lattice *lat);
 void *lattice_aux(const ym_lattice *lat, size_t bytes);
 
 void launch_su3_metropolis(int prec, void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
-                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, hipStream_t st);
+                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, int *status,
+                           hipStream_t st);
 void launch_su3_hot(int prec, void *links, const Geom &g, uint64_t key, hipStream_t st);
 void launch_su3_staple_probe(int prec, const void *links, const Geom &g, const int x[4], int mu, double *out, hipStream_t st);
 void launch_su3_reunit(int prec, void *links, const Geom &g, hipStream_t st);

@@ -41,73 +41,82 @@ __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::typ
 }
 
 // Wave-specialised sub-pass for lattices of at least 256 tiles: one block of
-// 8 wavefronts per CU walks a sequence of 256-slot tiles; waves 0-3 compute
-// the staples of tile i+1 into LDS while waves 4-7 run the FP64 hits of tile
-// i from the staples of the previous step, so the neighbour loads of one
-// tile overlap the accept/reject arithmetic of the other.  Per link the
-// arithmetic is that of k_su3_metropolis, so results are identical.  Tiles
-// are split into 8 contiguous ranges, one per XCD (blocks are dealt
+// 8 wavefronts per CU walks a sequence of 256-slot tiles.  Producer wave k
+// (0-3) computes the staples (and loads the links) of slots 64k..64k+63 of
+// each tile into an LDS double buffer; consumer wave k + 4 runs the FP64
+// hits of the same 64 slots.  The two waves of a pair hand buffers over by
+// LDS flags (wave_handoff_wait / wave_handoff_post), not by a block barrier:
+// a producer fills buffer b for tile i once its consumer has read the
+// buffer's previous tile i - 2 into registers, and the consumer releases a
+// buffer right after reading it, so a producer runs up to two tiles ahead
+// of its consumer and no pair waits for the slowest of the four.  Per link
+// the arithmetic is that of k_su3_metropolis, so results are identical.
+// Tiles are split into 8 contiguous ranges, one per XCD (blocks are dealt
 // round-robin over the XCDs), and the blocks of an XCD advance through its
 // range side by side, so the tiles in flight on an XCD are neighbours.  The
-// producers also stage the link being updated, so the hit waves touch global
-// memory only for the final store.  Every sub-pass reads the whole link
-// field; sub-passes of parity 1 walk each XCD's range backwards, so a
+// producers also stage the link being updated, so the hit waves touch
+// global memory only for the final store.  Every sub-pass reads the whole
+// link field; sub-passes of parity 1 walk each XCD's range backwards, so a
 // sub-pass starts on the tiles its predecessor touched last, which are still
 // in that XCD's L2 and in the memory-side cache (a 302 MB field at 32^4 is
 // larger than the 256 MB cache: the same direction every time evicts each
 // line just before it is needed again).  Tiles are independent, so the
 // direction changes timing only.
-// (A three-buffer schedule, with the producers two tiles ahead and the
-// idle waves of the first and last phase helping, measured slower on 32^4
-// and 24^4: the kernel is throughput-bound, not bound by those phases.)
 template <typename Real, int MU>
 __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::type *__restrict__ L, Geom g, int p,
                                                            double nbt, double angle, int hits, uint64_t key,
                                                            uint64_t sweep, int reunit, int ntiles,
-                                                           unsigned long long *__restrict__ tally) {
+                                                           unsigned long long *__restrict__ tally,
+                                                           int *__restrict__ status) {
     constexpr int NB = 2;
     __shared__ LdsMat<Real> abuf[NB], ubuf[NB];
     __shared__ unsigned long long red[8];
+    __shared__ int ready[NB][4], freed[NB][4];  // tile index + 1 last filled / last read, per pair
     const bool producer = threadIdx.x < 256;
-    const int lane = threadIdx.x & 255;
+    const int lane = threadIdx.x & 255, pair = lane >> 6;
+    const bool leader = (threadIdx.x & 63) == 0;
+    if (threadIdx.x < 4 * NB) {
+        (&ready[0][0])[threadIdx.x] = 0;
+        (&freed[0][0])[threadIdx.x] = 0;
+    }
+    __syncthreads();
     const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
     const int nbx = ((int)gridDim.x - 1 - xcd) / 8 + 1;
     const int t0 = (int)(((int64_t)ntiles * xcd) / 8), t1 = (int)(((int64_t)ntiles * (xcd + 1)) / 8);
     const int nsteps = (t1 - t0 - j + nbx - 1) / nbx;
     unsigned long long nacc = 0;
-    // producer: staple of the link in slot `lane` of tile i into buffer i & 1
     const bool rev = (p & 1) != 0;
     auto tile_at = [&](int i) { return rev ? t1 - 1 - (j + i * nbx) : t0 + j + i * nbx; };
-    auto produce = [&](int i) {
-        const int s = tile_slot(g, tile_at(i), lane);
-        int x[4];
-        if (s < g.vh && site_of(g, p, s, x)) {
-            M3<Real> A;
-            su3_staple_at<Real, MU>(L, g, p, s, x, A);
-            const M3<Real> U = load3<Real>(L, g.vh, MU, p, s);
-            abuf[i % NB].put(lane, A);
-            ubuf[i % NB].put(lane, U);
+    if (producer) {
+        for (int i = 0; i < nsteps; ++i) {
+            const int b = i % NB;
+            if (i >= NB) wave_handoff_wait(&freed[b][pair], i - NB + 1, status);
+            const int s = tile_slot(g, tile_at(i), lane);
+            int x[4];
+            if (s < g.vh && site_of(g, p, s, x)) {
+                M3<Real> A;
+                su3_staple_at<Real, MU>(L, g, p, s, x, A);
+                const M3<Real> U = load3<Real>(L, g.vh, MU, p, s);
+                abuf[b].put(lane, A);
+                ubuf[b].put(lane, U);
+            }
+            wave_handoff_post(&ready[b][pair], i + 1, leader);
         }
-    };
-    auto consume = [&](int i) {
-        const int s = tile_slot(g, tile_at(i), lane);
-        int x[4];
-        if (s < g.vh && site_of(g, p, s, x)) {
-            const M3<Real> A = abuf[i % NB].get(lane), U = ubuf[i % NB].get(lane);
-            nacc += su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, key, sweep, reunit);
+    } else {
+        for (int i = 0; i < nsteps; ++i) {
+            const int b = i % NB;
+            wave_handoff_wait(&ready[b][pair], i + 1, status);
+            const int s = tile_slot(g, tile_at(i), lane);
+            int x[4];
+            const bool has = s < g.vh && site_of(g, p, s, x);
+            M3<Real> A, U;
+            if (has) {
+                A = abuf[b].get(lane);
+                U = ubuf[b].get(lane);
+            }
+            wave_handoff_post(&freed[b][pair], i + 1, leader);
+            if (has) nacc += su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, key, sweep, reunit);
         }
-    };
-    // Phase t of the block: the producers stage tile t while the consumers
-    // update tile t - 1 (n + 1 phases for n tiles); produce and consume have
-    // one call site each, so the kernel carries one copy of their code.
-    const int nphase = nsteps == 0 ? 0 : nsteps + 1;
-    for (int t = 0; t < nphase; ++t) {
-        if (producer) {
-            if (t < nsteps) produce(t);
-        } else if (t > 0) {
-            consume(t - 1);
-        }
-        __syncthreads();
     }
     const unsigned long long tot = block_sum_u64<512>(nacc, red);
     if (threadIdx.x == 0) tally[blockIdx.x] += tot;
@@ -115,7 +124,8 @@ __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::
 
 template <typename Real>
 static void launch_metro_t(void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
-                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, hipStream_t st) {
+                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, int *status,
+                           hipStream_t st) {
     using V2 = typename Vec2<Real>::type;
     const double nbt = -(beta / 3.0);
     const int nb = (g.vh + 255) / 256;
@@ -124,10 +134,10 @@ static void launch_metro_t(void *links, const Geom &g, int mu, int p, double bet
         // one 8-wave block per CU
         const int grid = 256;
         switch (mu) {
-        case 0: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 0>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally); break;
-        case 1: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 1>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally); break;
-        case 2: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 2>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally); break;
-        default: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 3>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally); break;
+        case 0: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 0>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status); break;
+        case 1: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 1>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status); break;
+        case 2: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 2>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status); break;
+        default: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 3>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status); break;
         }
         return;
     }
@@ -139,9 +149,10 @@ static void launch_metro_t(void *links, const Geom &g, int mu, int p, double bet
     }
 }
 void launch_su3_metropolis(int prec, void *links, const Geom &g, int mu, int p, double beta, double angle, int hits,
-                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, hipStream_t st) {
-    if (prec == 32) launch_metro_t<float>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, tally, st);
-    else launch_metro_t<double>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, tally, st);
+                           uint64_t key, uint64_t sweep, int reunit, unsigned long long *tally, int *status,
+                           hipStream_t st) {
+    if (prec == 32) launch_metro_t<float>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, tally, status, st);
+    else launch_metro_t<double>(links, g, mu, p, beta, angle, hits, key, sweep, reunit, tally, status, st);
 }
 
 // ----------------------------------------------------------- hot start
