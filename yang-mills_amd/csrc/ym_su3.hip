@@ -68,7 +68,7 @@ __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::
                                                            uint64_t sweep, int reunit, int ntiles,
                                                            unsigned long long *__restrict__ tally,
                                                            int *__restrict__ status) {
-    constexpr int NB = 2;
+    constexpr int NB = 2;  // three or four FP32 buffers measured 2-3% slower
     __shared__ LdsMat<Real> abuf[NB], ubuf[NB];
     __shared__ unsigned long long red[8];
     __shared__ int ready[NB][4], freed[NB][4];  // tile index + 1 last filled / last read, per pair
