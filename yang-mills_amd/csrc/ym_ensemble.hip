@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_ens_plaquette(const typename Vec2<Real>
                                                        const int *__restrict__ begin, int nch,
                                                        double *__restrict__ part) {
     __shared__ double red[4];
-    const int b = blockIdx.x;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);  // contiguous tile ranges per XCD
     const int c = chain_of(begin, nch, b);
     const Geom g = cd[c].g;
     const int nt = (g.vh + 255) / 256;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(512) void k_ens_plaquette_pp(const typename Vec2<Re
                                                           const int *__restrict__ begin,
                                                           double *__restrict__ part) {
     __shared__ double red[8];
-    const int b = blockIdx.x;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);  // contiguous tile ranges per XCD
     const int c = tile_chain[b];
     const Geom g = cd[c].g;
     const int nt = (g.vh + 255) / 256;
@@ -207,22 +207,47 @@ __global__ __launch_bounds__(512) void k_ens_plaquette_pp(const typename Vec2<Re
     }
 }
 
-// One stout level for every chain: begin[c+1]-begin[c] = 8 * ceil(vh/128)
-// (mu major, then parity, then 128-slot block).
+// One stout level of the spatial links of every chain.  begin[c] is the
+// prefix table of 8 * ceil(vh/128) blocks per chain, so chain c owns the 6 k
+// spatial blocks from 6 begin[c] / 8 on: 128-slot block major, then the six
+// (mu < 3, parity) groups.  Logical blocks go to the XCDs in contiguous
+// ranges, so a chain's blocks, and the neighbours they share, stay in one
+// XCD's L2.  The temporal links are the same at every level:
+// k_ens_copy_temporal puts them into both output buffers once.
 template <typename Real>
 __global__ __launch_bounds__(128) void k_ens_stout(const typename Vec2<Real>::type *__restrict__ in,
                                                    typename Vec2<Real>::type *__restrict__ out,
                                                    const ChainDesc *__restrict__ cd, const int *__restrict__ begin,
                                                    int nch, double rho) {
-    const int b = blockIdx.x;
-    const int c = chain_of(begin, nch, b);
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int c = chain_of(begin, nch, 8 * (b / 6));
     const Geom g = cd[c].g;
-    const int nsb = (g.vh + 127) / 128;
-    const int lb = b - begin[c];
-    const int sb = lb % nsb, p = (lb / nsb) & 1, mu = lb / (2 * nsb);
+    const int lb = b - 6 * (begin[c] >> 3);
+    const int sb = lb / 6, r = lb - 6 * sb;
     const int s = sb * 128 + threadIdx.x;
     const uint64_t off = cd[c].link_off;
-    if (s < g.vh) stout_link<Real>(in + off, out + off, g, rho, p, s, mu);
+    if (s < g.vh) stout_link<Real>(in + off, out + off, g, rho, r & 1, s, r >> 1);
+}
+
+// The temporal (mu = 3) links of every chain from `in` into out0 and out1:
+// 2 * ceil(vh/128) blocks per chain (block major, then parity).
+template <typename Real>
+__global__ __launch_bounds__(128) void k_ens_copy_temporal(const typename Vec2<Real>::type *__restrict__ in,
+                                                           typename Vec2<Real>::type *__restrict__ out0,
+                                                           typename Vec2<Real>::type *__restrict__ out1,
+                                                           const ChainDesc *__restrict__ cd,
+                                                           const int *__restrict__ begin, int nch) {
+    const int b = blockIdx.x;
+    const int c = chain_of(begin, nch, 8 * (b >> 1));
+    const Geom g = cd[c].g;
+    const int lb = b - 2 * (begin[c] >> 3);
+    const int s = (lb >> 1) * 128 + threadIdx.x, p = lb & 1;
+    const uint64_t off = cd[c].link_off;
+    if (s < g.vh) {
+        const M3<Real> u = load3<Real>(in + off, g.vh, 3, p, s);
+        store3<Real>(out0 + off, g.vh, 3, p, s, u);
+        store3<Real>(out1 + off, g.vh, 3, p, s, u);
+    }
 }
 
 // Per-site A1++ values of every chain (a1pp_slot): both parities of one
@@ -234,7 +259,7 @@ __global__ __launch_bounds__(512) void k_ens_a1pp_sites(const typename Vec2<Real
                                                         const int *__restrict__ tiles,
                                                         const int *__restrict__ tile_chain,
                                                         double *__restrict__ sv) {
-    const int b = blockIdx.x;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);  // contiguous tile ranges per XCD
     const int c = tile_chain[b];
     const Geom g = cd[c].g;
     a1pp_slot<Real>(links + cd[c].link_off, g, threadIdx.x >> 8, (b - tiles[c]) * 256 + (threadIdx.x & 255),
@@ -405,11 +430,14 @@ void enqueue_measure(ym_ensemble *E, int levels, double rho, double *out_plaq, d
     hipLaunchKernelGGL(ym::k_ens_reduce_plaq, dim3(E->nch), dim3(256), 0, E->stream, E->part, E->d_cd, E->d_plaq,
                        E->nch, E->d_ctr, out_plaq);
     const V2 *cur = L;
+    if (levels > 0)
+        hipLaunchKernelGGL((ym::k_ens_copy_temporal<Real>), dim3(E->h_stout[E->nch] / 4), dim3(128), 0, E->stream, L,
+                           (V2 *)E->tmp[0], (V2 *)E->tmp[1], E->d_cd, E->d_stout, E->nch);
     for (int lev = 0; lev <= levels; ++lev) {
         if (lev > 0) {
             V2 *dst = (V2 *)E->tmp[(lev - 1) & 1];
-            hipLaunchKernelGGL((ym::k_ens_stout<Real>), dim3(E->h_stout[E->nch]), dim3(128), 0, E->stream, cur, dst,
-                               E->d_cd, E->d_stout, E->nch, rho);
+            hipLaunchKernelGGL((ym::k_ens_stout<Real>), dim3(E->h_stout[E->nch] / 8 * 6), dim3(128), 0, E->stream,
+                               cur, dst, E->d_cd, E->d_stout, E->nch, rho);
             cur = dst;
         }
         hipLaunchKernelGGL((ym::k_ens_a1pp_sites<Real>), dim3(E->h_metro[E->nch]), dim3(512), 0, E->stream, cur,

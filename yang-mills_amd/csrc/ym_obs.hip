@@ -20,12 +20,13 @@ template <class G>
 __global__ __launch_bounds__(256) void k_plaquette(const typename G::Store *__restrict__ L, Geom g,
                                                    double *__restrict__ part) {
     __shared__ double red[4];
-    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);  // contiguous tile ranges per XCD
+    const int s = blk * 256 + threadIdx.x;
     const int p = blockIdx.y;
     double acc = 0.0;
     if (s < g.vh) acc = plaquette_site<G>(L, g, p, s);
     const double t = block_sum<256>(acc, red);
-    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = t;
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blk] = t;
 }
 
 // The same partials with both parities of 256 consecutive slots (a tile) in
@@ -289,7 +290,8 @@ int launch_polyakov(int group, int prec, const void *links, const Geom &g, int d
 template <typename Real>
 __global__ __launch_bounds__(512) void k_a1pp_sites(const typename Vec2<Real>::type *__restrict__ L, Geom g,
                                                     double *__restrict__ sv) {
-    a1pp_slot<Real>(L, g, threadIdx.x >> 8, blockIdx.x * 256 + (threadIdx.x & 255), sv);
+    // contiguous tile ranges per XCD: neighbouring tiles share one L2
+    a1pp_slot<Real>(L, g, threadIdx.x >> 8, xcd_tile(blockIdx.x, gridDim.x) * 256 + (threadIdx.x & 255), sv);
 }
 __global__ __launch_bounds__(256) void k_a1pp(const double *__restrict__ sv, Geom g, double *__restrict__ part) {
     __shared__ double red[4];
@@ -330,13 +332,19 @@ int launch_a1pp(int prec, const void *links, const Geom &g, double *sv, double *
 template <typename Real>
 __global__ __launch_bounds__(128) void k_stout(const typename Vec2<Real>::type *__restrict__ in,
                                                typename Vec2<Real>::type *__restrict__ out, Geom g, double rho) {
-    const int s = blockIdx.x * 128 + threadIdx.x;
-    if (s < g.vh) stout_link<Real>(in, out, g, rho, blockIdx.y, s, blockIdx.z);
+    // logical block: 128-slot block major, then the six (direction, parity)
+    // groups; each XCD takes a contiguous range, so the x2- and x1-neighbours
+    // of a block (16 and N2 N3 / 2 slots away) and the other groups of the
+    // same slots are fetched into that XCD's L2 once
+    const int lb = xcd_tile(blockIdx.x, gridDim.x);
+    const int sb = lb / 6, r = lb - 6 * sb;
+    const int s = sb * 128 + threadIdx.x;
+    if (s < g.vh) stout_link<Real>(in, out, g, rho, r & 1, s, r >> 1);
 }
-// Spatial directions only (grid.z = 3): the temporal links are the same at
-// every level, so the caller copies them into the output buffers once.
+// Spatial directions only: the temporal links are the same at every level,
+// so the caller copies them into the output buffers once.
 void launch_stout(int prec, const void *in, void *out, const Geom &g, double rho, hipStream_t st) {
-    dim3 grid((g.vh + 127) / 128, 2, 3);
+    dim3 grid(((g.vh + 127) / 128) * 6);
     if (prec == 32) hipLaunchKernelGGL((k_stout<float>), grid, dim3(128), 0, st, (const float2 *)in, (float2 *)out, g, rho);
     else hipLaunchKernelGGL((k_stout<double>), grid, dim3(128), 0, st, (const double2 *)in, (double2 *)out, g, rho);
 }
