@@ -6,6 +6,27 @@
 
 namespace ym {
 
+// The Metropolis decision dS <= 0 or u < exp_neg(dS), exactly that FP64 rule,
+// without evaluating exp_neg for almost every proposal.  a = 2^(-dS log2 e)
+// by the hardware's FP32 exp2 (v_exp_f32) differs from exp_neg(dS) by less
+// than tol = 2^-12 a + 2^-126: the FP32 rounding of the exponent t (|t| <=
+// 149 wherever a > 0) moves a by at most 149 * 2^-24 * ln 2 < 2^-17
+// relative, v_exp_f32 adds about 1 ulp of FP32 (2^-23), exp_neg is within
+// 2 ulp of FP64, and below 2^-126 only the absolute term matters.  So
+// u < a - tol decides accept and u >= a + tol decides reject exactly as
+// exp_neg would; only u inside that band (probability < 2^-10 per
+// proposal) takes the FP64 path.  Decisions, counts and links are
+// therefore bit for bit those of the plain rule.
+__device__ __forceinline__ bool metropolis_accept(double dS, double u) {
+    if (!(dS > 0.0)) return true;
+    const float t = (float)(dS * -0x1.71547652b82fep+0);
+    const double a = (double)__builtin_amdgcn_exp2f(t);
+    const double tol = __builtin_fma(a, 0x1p-12, 0x1p-126);
+    if (u < a - tol) return true;
+    if (u >= a + tol) return false;
+    return u < exp_neg(dS);
+}
+
 // One single-generator proposal on the FP64 link u with M = u A (staple A):
 // generator GEN = h mod 8, angle from the reference's paired grid, accept iff
 // dS <= 0 or u < exp(-dS), then u <- R u and M <- R M, where a rejected
@@ -45,7 +66,7 @@ __device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t k
         dre = __builtin_fma(cm1, t1, sn * t2);
     }
     const double dS = nbt * dre;
-    const bool accept = (dS <= 0.0) || (uacc < exp_neg(dS));
+    const bool accept = metropolis_accept(dS, uacc);
     nacc += accept ? 1ull : 0ull;
     const double c = accept ? 1.0 + cm1 : 1.0;
     const double sa = accept ? sn : 0.0;
