@@ -102,6 +102,27 @@ void ymd_sincos_small(double x, double *s, double *cm1) {
     *cm1 = x2 * pc;
 }
 
+/* sin x and cos x - 1 for |x| <= 0.29, the device's sincos_short: the series
+ * in y = -x^2 with positive 1/n! coefficients, stopped at x^13 / x^12.  The
+ * SU(3) hit uses it for proposal angles <= 0.5. */
+void ymd_sincos_short(double x, double *s, double *cm1) {
+    double y = -(x * x);
+    double ps = 0x1.6124613a86d09p-33;         /* 1/13! */
+    ps = fma(ps, y, 0x1.ae64567f544e4p-26);    /* 1/11! */
+    ps = fma(ps, y, 0x1.71de3a556c734p-19);    /* 1/9!  */
+    ps = fma(ps, y, 0x1.a01a01a01a01ap-13);    /* 1/7!  */
+    ps = fma(ps, y, 0x1.1111111111111p-7);     /* 1/5!  */
+    ps = fma(ps, y, 0x1.5555555555555p-3);     /* 1/3!  */
+    *s = fma(x * y, ps, x);
+    double pc = 0x1.1eed8eff8d898p-29;         /* 1/12! */
+    pc = fma(pc, y, 0x1.27e4fb7789f5cp-22);    /* 1/10! */
+    pc = fma(pc, y, 0x1.a01a01a01a01ap-16);    /* 1/8!  */
+    pc = fma(pc, y, 0x1.6c16c16c16c17p-10);    /* 1/6!  */
+    pc = fma(pc, y, 0x1.5555555555555p-5);     /* 1/4!  */
+    pc = fma(pc, y, 0x1p-1);                   /* 1/2!  */
+    *cm1 = y * pc;
+}
+
 /* sin(2 pi t), cos(2 pi t): exact quarter-turn reduction t' = t - q/4 with
  * q = rint(4t), then the Taylor kernel on x = 2 pi t' (|x| <= pi/4). */
 void ymd_sincos_turns(double t, double *s, double *c) {
@@ -323,6 +344,13 @@ int ymd_su2_heatbath_samples(const double a[4], double beta, uint64_t key, int m
 #define YMD_PARALLEL_FOR
 #define YMD_PARALLEL_FOR_REDUCE(...)
 #endif
+
+/* The SU(3) hit's sincos, as the device's sincos_hit: the short series for
+ * proposal angles <= 0.5, else the full one. */
+static void sincos_hit_angle(double angle, double x, double *s, double *cm1) {
+    if (angle <= 0.5) ymd_sincos_short(x, s, cm1);
+    else ymd_sincos_small(x, s, cm1);
+}
 
 #define REAL float
 #define RSUF f32

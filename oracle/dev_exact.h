@@ -39,6 +39,7 @@ typedef struct {
 void ymd_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double ymd_exp_neg(double d);
 void ymd_sincos_small(double x, double *s, double *cm1);
+void ymd_sincos_short(double x, double *s, double *cm1);
 double ymd_log(double w);
 double ymd_log1p(double y);
 double ymd_expm1(double x);

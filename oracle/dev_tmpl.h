@@ -298,14 +298,14 @@ static void F(su3_update_link)(REAL *L, const geom *g, const int64_t x[4], int m
         double s, cm1, s2 = 0.0, cm2 = 0.0, dre;
         if (gen == 7) {
             double psi = theta * 0x1.279a74590331cp-2;
-            ymd_sincos_small(psi, &s, &cm1);
-            ymd_sincos_small(-2.0 * psi, &s2, &cm2);
+            sincos_hit_angle(P->angle, psi, &s, &cm1);
+            sincos_hit_angle(P->angle, -2.0 * psi, &s2, &cm2);
             double t1 = RE(M, 0, 0) + RE(M, 1, 1), t2 = IM(M, 0, 0) + IM(M, 1, 1);
             double d1 = fma(cm1, t1, -(s * t2));
             double d2 = fma(cm2, RE(M, 2, 2), -(s2 * IM(M, 2, 2)));
             dre = d1 + d2;
         } else {
-            ymd_sincos_small(theta * 0.5, &s, &cm1);
+            sincos_hit_angle(P->angle, theta * 0.5, &s, &cm1);
             if (gen == 2) {
                 double t1 = RE(M, 0, 0) + RE(M, 1, 1), t2 = IM(M, 0, 0) - IM(M, 1, 1);
                 dre = fma(cm1, t1, -(s * t2));

@@ -166,6 +166,7 @@ def _dev_signatures(L):
         "ymd_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "ymd_exp_neg": (dbl, [dbl]),
         "ymd_sincos_small": (None, [dbl, dp, dp]),
+        "ymd_sincos_short": (None, [dbl, dp, dp]),
         "ymd_log": (dbl, [dbl]),
         "ymd_log1p": (dbl, [dbl]),
         "ymd_expm1": (dbl, [dbl]),

@@ -48,6 +48,15 @@ def test_fp64_kernels_against_libm():
         L.ymd_sincos_small(x, C.byref(s), C.byref(c))
         worst["sin"] = max(worst.get("sin", 0), abs(s.value - math.sin(x)) / math.ulp(1.0))
         worst["cos"] = max(worst.get("cos", 0), abs(1.0 + c.value - math.cos(x)) / math.ulp(1.0))
+    for x in np.linspace(-0.2887, 0.2887, 4001):  # the short series' range (proposal angles <= 0.5)
+        if x == 0.0:
+            continue
+        # relative to sin x and to cos x - 1 = -2 sin^2(x/2) (no cancellation), in ulps of each
+        L.ymd_sincos_short(x, C.byref(s), C.byref(c))
+        worst["sin_short"] = max(worst.get("sin_short", 0), ulps(s.value, math.sin(x)))
+        worst["cm1_short"] = max(worst.get("cm1_short", 0), ulps(c.value, -2.0 * math.sin(x / 2) ** 2))
+        L.ymd_sincos_small(x, C.byref(s), C.byref(c))
+        worst["cm1"] = max(worst.get("cm1", 0), ulps(c.value, -2.0 * math.sin(x / 2) ** 2))
     for t in np.linspace(0, 1, 4001, endpoint=False):
         L.ymd_sincos_turns(t, C.byref(s), C.byref(c))
         worst["turns"] = max(worst.get("turns", 0), abs(s.value - math.sin(2 * math.pi * t)) / math.ulp(1.0),

@@ -86,15 +86,18 @@ def test_su2_hot_start_matches_replay(ym, prec):
 
 
 # ------------------------------------------------ decision-for-decision chains
-@pytest.mark.parametrize("prec,n,beta,hits", [
-    (32, (4, 4, 4, 4), 6.0, 8),
-    (32, (2, 4, 6, 4), 5.7, 10),
-    (64, (4, 4, 4, 4), 6.0, 8),
-    (64, (4, 2, 2, 8), 0.0, 3),
+# angle <= 0.5 runs the short sincos series, larger angles the full one
+@pytest.mark.parametrize("prec,n,beta,hits,angle", [
+    (32, (4, 4, 4, 4), 6.0, 8, 0.35),
+    (32, (2, 4, 6, 4), 5.7, 10, 0.35),
+    (64, (4, 4, 4, 4), 6.0, 8, 0.35),
+    (64, (4, 2, 2, 8), 0.0, 3, 0.35),
+    (64, (4, 4, 4, 4), 5.7, 10, 0.5),
+    (32, (4, 4, 4, 4), 6.0, 10, 0.8),
+    (64, (2, 4, 4, 4), 6.0, 8, 1.0),
 ])
-def test_su3_metropolis_matches_replay_bitwise(ym, prec, n, beta, hits):
+def test_su3_metropolis_matches_replay_bitwise(ym, prec, n, beta, hits, angle):
     key = ym.derive_su3_chain_seed(0x2c02198105010002, 1)
-    angle = 0.35
     lat = ym.Su3Lattice4(n, prec)
     lat.hot_start(key ^ 0x55)
     buf = dev_buffer(n, 18, prec)

@@ -115,6 +115,28 @@ __device__ __forceinline__ void sincos_small(double x, double &s, double &cm1) {
     cm1 = y * pc;
 }
 
+// sin x and cos x - 1 for |x| <= 0.29 (the proposal rotations of angles up
+// to 0.5): the same series in y = -x^2 stopped at x^13 / x^12, whose first
+// omitted terms are below 2.2e-20 (sin, relative to x) and 7.7e-18
+// (cos - 1, relative to x^2 / 2) on that range.
+__device__ __forceinline__ void sincos_short(double x, double &s, double &cm1) {
+    const double y = -(x * x);
+    double ps = 0x1.6124613a86d09p-33;  // 1/13!
+    ps = __builtin_fma(ps, y, 0x1.ae64567f544e4p-26);
+    ps = __builtin_fma(ps, y, 0x1.71de3a556c734p-19);
+    ps = __builtin_fma(ps, y, 0x1.a01a01a01a01ap-13);
+    ps = __builtin_fma(ps, y, 0x1.1111111111111p-7);
+    ps = __builtin_fma(ps, y, 0x1.5555555555555p-3);
+    s = __builtin_fma(x * y, ps, x);
+    double pc = 0x1.1eed8eff8d898p-29;  // 1/12!
+    pc = __builtin_fma(pc, y, 0x1.27e4fb7789f5cp-22);
+    pc = __builtin_fma(pc, y, 0x1.a01a01a01a01ap-16);
+    pc = __builtin_fma(pc, y, 0x1.6c16c16c16c17p-10);
+    pc = __builtin_fma(pc, y, 0x1.5555555555555p-5);
+    pc = __builtin_fma(pc, y, 0x1p-1);
+    cm1 = y * pc;
+}
+
 // sin(2 pi t), cos(2 pi t) for t in [0, 1).
 __device__ __forceinline__ void sincos_turns(double t, double &s, double &c) {
     double q = __builtin_rint(4.0 * t);

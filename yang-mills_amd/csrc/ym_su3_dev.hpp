@@ -27,6 +27,16 @@ __device__ __forceinline__ bool metropolis_accept(double dS, double u) {
     return u < exp_neg(dS);
 }
 
+// Proposal angles up to this bound use the short sincos series: every
+// argument of a rotation is then at most 0.5 / 2 (generators 0-6) or
+// 2 * 0.5 / (2 sqrt 3) (generator 7) in magnitude, inside its 0.29 range.
+constexpr double kShortSeriesMaxAngle = 0.5;
+
+__device__ __forceinline__ void sincos_hit(double x, bool short_series, double &s, double &cm1) {
+    if (short_series) sincos_short(x, s, cm1);
+    else sincos_small(x, s, cm1);
+}
+
 // One single-generator proposal on the FP64 link u with M = u A (staple A):
 // generator GEN = h mod 8, angle from the reference's paired grid, accept iff
 // dS <= 0 or u < exp(-dS), then u <- R u and M <- R M, where a rejected
@@ -41,26 +51,27 @@ __device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t k
     draw(key, site, MU, KIND_SU3_HIT, (uint32_t)h, sweep, o);
     const double theta = angle_from_raw(o[0], o[1], angle);
     const double uacc = unit53(o[2], o[3]);
+    const bool short_series = angle <= kShortSeriesMaxAngle;  // uniform over the grid
     double sn, cm1, s2 = 0.0, cm2 = 0.0, dre;
     if constexpr (GEN == 7) {
         const double psi = theta * 0x1.279a74590331cp-2;
-        sincos_small(psi, sn, cm1);
-        sincos_small(-2.0 * psi, s2, cm2);
+        sincos_hit(psi, short_series, sn, cm1);
+        sincos_hit(-2.0 * psi, short_series, s2, cm2);
         const double t1 = YM_RE(M, 0, 0) + YM_RE(M, 1, 1), t2 = YM_IM(M, 0, 0) + YM_IM(M, 1, 1);
         const double d1 = __builtin_fma(cm1, t1, -(sn * t2));
         const double d2 = __builtin_fma(cm2, YM_RE(M, 2, 2), -(s2 * YM_IM(M, 2, 2)));
         dre = d1 + d2;
     } else if constexpr (GEN == 2) {
-        sincos_small(theta * 0.5, sn, cm1);
+        sincos_hit(theta * 0.5, short_series, sn, cm1);
         const double t1 = YM_RE(M, 0, 0) + YM_RE(M, 1, 1), t2 = YM_IM(M, 0, 0) - YM_IM(M, 1, 1);
         dre = __builtin_fma(cm1, t1, -(sn * t2));
     } else if constexpr (GEN == 0 || GEN == 3 || GEN == 5) {
-        sincos_small(theta * 0.5, sn, cm1);
+        sincos_hit(theta * 0.5, short_series, sn, cm1);
         const double t1 = YM_RE(M, bi, bi) + YM_RE(M, bj, bj);
         const double t2 = YM_IM(M, bi, bj) + YM_IM(M, bj, bi);
         dre = __builtin_fma(cm1, t1, -(sn * t2));
     } else {
-        sincos_small(theta * 0.5, sn, cm1);
+        sincos_hit(theta * 0.5, short_series, sn, cm1);
         const double t1 = YM_RE(M, bi, bi) + YM_RE(M, bj, bj);
         const double t2 = YM_RE(M, bj, bi) - YM_RE(M, bi, bj);
         dre = __builtin_fma(cm1, t1, sn * t2);
