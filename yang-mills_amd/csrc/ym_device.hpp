@@ -23,6 +23,15 @@
 namespace ym {
 
 // ------------------------------------------------------------------ RNG
+// a ^ b ^ c in one gfx950 three-input bit operation (truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// Round 0 keeps the plain xors: with the counter words that are uniform over
+// a wavefront (direction, kind, index, sweep) half of its work is scalar.
+// From round 1 on every word varies per lane and each pair of xors is one
+// v_bitop3_b32.
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                               uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll
@@ -30,9 +39,9 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
         uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n0 = r == 0 ? (uint32_t)(p1 >> 32) ^ c1 ^ k0 : xor3((uint32_t)(p1 >> 32), c1, k0);
         uint32_t n1 = (uint32_t)p1;
-        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n2 = r == 0 ? (uint32_t)(p0 >> 32) ^ c3 ^ k1 : xor3((uint32_t)(p0 >> 32), c3, k1);
         uint32_t n3 = (uint32_t)p0;
         c0 = n0; c1 = n1; c2 = n2; c3 = n3;
     }
@@ -54,10 +63,16 @@ __device__ __forceinline__ double unit53(uint32_t lo, uint32_t hi) {
 
 // Reference angle grid (su3_ensemble.rs:199-213): the high bit selects the
 // sign, the low 52 bits an odd multiple of 2^-53 of the proposal angle.
+// The odd multiple (2m + 1) 2^-53, m the low 52 bits, is formed without an
+// integer conversion: the double with exponent 0 and mantissa m is
+// 1 + m 2^-52, subtracting 1 is exact (Sterbenz), and adding 2^-53 is exact
+// because (2m + 1) 2^-53 is an integer below 2^53 times a power of two.  So
+// angle * unit is the one rounding of angle * (2m + 1) 2^-53, the same
+// value as (angle * (2m + 1)) * 2^-53.
 __device__ __forceinline__ double angle_from_raw(uint32_t lo, uint32_t hi, double proposal_angle) {
-    uint64_t raw = ((uint64_t)hi << 32) | lo;
-    uint64_t odd = (raw & ((1ull << 52) - 1ull)) * 2ull + 1ull;
-    double magnitude = proposal_angle * (double)odd * 0x1p-53;
+    const uint64_t bits = ((uint64_t)((hi & 0xFFFFFu) | 0x3FF00000u) << 32) | lo;
+    const double unit = (__builtin_bit_cast(double, bits) - 1.0) + 0x1p-53;
+    const double magnitude = proposal_angle * unit;
     return (hi >> 31) ? magnitude : -magnitude;
 }
 
