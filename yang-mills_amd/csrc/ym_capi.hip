@@ -43,6 +43,22 @@ void launch_fill_zero_i32(int *p, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_fill_zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
 }
 
+// Device copy of a link field, 16 bytes per lane (a wavefront moves one
+// contiguous 1 KB block per instruction, the access width of the link loads
+// of the sweep kernels), grid-stride over n16 16-byte words.  Link fields are
+// multiples of 16 bytes.  It also serves as the known-size reference that
+// calibrates the L2 <-> fabric byte counters.
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+void launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t st) {
+    const size_t n16 = bytes / 16;
+    if (!n16) return;
+    const size_t blocks = std::min<size_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16);
+}
+
 // >= one slot per wave of 256-thread blocks over Vh slots (the finest grid of
 // the sweep kernels) and >= the 256 blocks of the wave-specialised kernel
 int64_t tally_slots(const Geom &g) { return std::max<int64_t>(((int64_t)g.vh + 255) / 256 * 4, 1024); }
@@ -559,7 +575,8 @@ int32_t ym_lattice_copy(ym_lattice *dst, const ym_lattice *src) {
     for (int d = 0; d < 4; ++d)
         if (dst->n[d] != src->n[d]) return YM_INVALID_ARGUMENT;
     YM_HIP(hipStreamSynchronize(src->stream));
-    YM_HIP(hipMemcpyAsync(dst->links, src->links, src->link_bytes, hipMemcpyDeviceToDevice, dst->stream));
+    ym::launch_copy16(src->links, dst->links, src->link_bytes, dst->stream);
+    YM_HIP(hipGetLastError());
     return finish(dst);
 }
 

@@ -68,6 +68,7 @@ void launch_max_partials(const double *part, int n, int ncols, int stride, doubl
 void launch_fill_zero(double *p, size_t n, hipStream_t st);
 void launch_fill_zero_u64(unsigned long long *p, size_t n, hipStream_t st);
 void launch_fill_zero_i32(int *p, size_t n, hipStream_t st);
+void launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t st);
 
 // Counts of the sweep kernels (accepted proposals, heat-bath attempts) go to
 // a tally of per-block slots: block b (wave b for the heat-bath sampler) of a
