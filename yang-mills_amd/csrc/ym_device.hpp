@@ -497,6 +497,48 @@ __device__ __forceinline__ void store3<float>(float2 *__restrict__ L, int vh, in
     grp[tile * 576u + 512u + lane] = make_float2(m.v[16], m.v[17]);
 }
 
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+// The same FP32 accesses with the non-temporal hint, for the link a sub-pass
+// updates: it is read once and written once and no staple of that sub-pass
+// reads it, so it should not displace the neighbour links that are reused.
+__device__ __forceinline__ M3<float> load3_nt(const float2 *__restrict__ L, int vh, int mu, int par, int s) {
+    const float2 *grp = L + (size_t)((mu * 2 + par) * ntile_of(vh)) * 576;
+    const uint32_t tile = (uint32_t)(s >> 6), lane = (uint32_t)(s & 63);
+    __builtin_assume(tile < (1u << 23));
+    const float4 *q = reinterpret_cast<const float4 *>(grp) + (tile * 288u + lane);
+    M3<float> m;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const nt_f4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(q + k * 64));
+        m.v[4 * k] = t.x;
+        m.v[4 * k + 1] = t.y;
+        m.v[4 * k + 2] = t.z;
+        m.v[4 * k + 3] = t.w;
+    }
+    const nt_f2 t = __builtin_nontemporal_load(reinterpret_cast<const nt_f2 *>(grp + tile * 576u + 512u + lane));
+    m.v[16] = t.x;
+    m.v[17] = t.y;
+    return m;
+}
+__device__ __forceinline__ void store3_nt(float2 *__restrict__ L, int vh, int mu, int par, int s, const M3<float> &m) {
+    float2 *grp = L + (size_t)((mu * 2 + par) * ntile_of(vh)) * 576;
+    const uint32_t tile = (uint32_t)(s >> 6), lane = (uint32_t)(s & 63);
+    __builtin_assume(tile < (1u << 23));
+    float4 *q = reinterpret_cast<float4 *>(grp) + (tile * 288u + lane);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        __builtin_nontemporal_store(nt_f4{m.v[4 * k], m.v[4 * k + 1], m.v[4 * k + 2], m.v[4 * k + 3]},
+                                    reinterpret_cast<nt_f4 *>(q + k * 64));
+    __builtin_nontemporal_store(nt_f2{m.v[16], m.v[17]}, reinterpret_cast<nt_f2 *>(grp + tile * 576u + 512u + lane));
+}
+__device__ __forceinline__ M3<double> load3_nt(const double2 *__restrict__ L, int vh, int mu, int par, int s) {
+    return load3<double>(L, vh, mu, par, s);
+}
+__device__ __forceinline__ void store3_nt(double2 *__restrict__ L, int vh, int mu, int par, int s, const M3<double> &m) {
+    store3<double>(L, vh, mu, par, s, m);
+}
+
 // T = X Y^dagger
 template <typename Real>
 __device__ __forceinline__ void mul_nd(const M3<Real> &X, const M3<Real> &Y, M3<Real> &T) {

@@ -242,7 +242,7 @@ __device__ __forceinline__ unsigned long long su3_update_from(typename Vec2<Real
     M3<Real> Uo;
 #pragma unroll
     for (int c = 0; c < 18; ++c) Uo.v[c] = (Real)u.v[c];
-    store3<Real>(L, g.vh, MU, p, s, Uo);
+    store3_nt(L, g.vh, MU, p, s, Uo);
     return nacc;
 }
 
