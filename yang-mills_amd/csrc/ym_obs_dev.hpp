@@ -297,14 +297,20 @@ __device__ __forceinline__ M3<double> expm_i_hermitian_ch(const M3<double> &Q) {
     return E;
 }
 
-// One stout step of the link (mu, parity p, slot s < vh) from `in` into `out`.
-template <typename Real>
-__device__ __forceinline__ void stout_link(const typename Vec2<Real>::type *__restrict__ in,
-                                           typename Vec2<Real>::type *__restrict__ out, const Geom &g, double rho,
-                                           int p, int s, int mu) {
+// One stout step of the spatial link (MU, parity p, slot s < vh) from `in`
+// into `out`.  The direction is a template parameter so that every
+// neighbour index is a compile-time choice: with a runtime direction the
+// site arrays are indexed dynamically, which puts them in scratch memory
+// when the geometry comes from memory (the batched kernel's per-chain
+// table) instead of kernel arguments.
+template <typename Real, int MU>
+__device__ __forceinline__ void stout_link_dir(const typename Vec2<Real>::type *__restrict__ in,
+                                               typename Vec2<Real>::type *__restrict__ out, const Geom &g, double rho,
+                                               int p, int s) {
     using G = G3<Real>;
+    constexpr int mu = MU;
     int x[4];
-    if (!site_of(g, p, s, x) || mu == 3) {
+    if (!site_of(g, p, s, x)) {
         store3<Real>(out, g.vh, mu, p, s, load3<Real>(in, g.vh, mu, p, s));
         return;
     }
@@ -313,6 +319,7 @@ __device__ __forceinline__ void stout_link(const typename Vec2<Real>::type *__re
     for (int c = 0; c < 18; ++c) C.v[c] = 0.0;
     int xpm[4];
     shifted(g, x, mu, +1, xpm);
+#pragma unroll
     for (int nu = 0; nu < 3; ++nu) {
         if (nu == mu) continue;
         int xpn[4], xmn[4], xmnpm[4];
@@ -366,6 +373,20 @@ __device__ __forceinline__ void stout_link(const typename Vec2<Real>::type *__re
 #pragma unroll
     for (int c = 0; c < 18; ++c) o.v[c] = (Real)Un.v[c];
     store3<Real>(out, g.vh, mu, p, s, o);
+}
+
+// The stout step of link (mu, p, s): spatial directions through
+// stout_link_dir, the temporal one copied unchanged.
+template <typename Real>
+__device__ __forceinline__ void stout_link(const typename Vec2<Real>::type *__restrict__ in,
+                                           typename Vec2<Real>::type *__restrict__ out, const Geom &g, double rho,
+                                           int p, int s, int mu) {
+    switch (mu) {
+    case 0: stout_link_dir<Real, 0>(in, out, g, rho, p, s); break;
+    case 1: stout_link_dir<Real, 1>(in, out, g, rho, p, s); break;
+    case 2: stout_link_dir<Real, 2>(in, out, g, rho, p, s); break;
+    default: store3<Real>(out, g.vh, mu, p, s, load3<Real>(in, g.vh, mu, p, s)); break;
+    }
 }
 
 }  // namespace ym
