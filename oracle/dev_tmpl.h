@@ -140,7 +140,84 @@ static void F(acc_nn)(REAL *A, const REAL *T, const REAL *Z) {
     }
 }
 
-/* A = sum_{nu != mu} [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)] */
+/* FP32 staple from the first two rows of each neighbour (the device reads
+ * rows 0 and 1 only): row 2 = conj(row 0 x row 1), rebuilt where a product
+ * needs it.  Entry (2, c) = conj(a_j b_k - a_k b_j), (c, j, k) cyclic. */
+static void F(recon_row2)(REAL *M) {
+    for (int c = 0; c < 3; ++c) {
+        int j = (c + 1) % 3, k = (c + 2) % 3;
+        REAL re = RE(M, 0, j) * RE(M, 1, k);
+        re = RFMA(-IM(M, 0, j), IM(M, 1, k), re);
+        re = RFMA(-RE(M, 0, k), RE(M, 1, j), re);
+        re = RFMA(IM(M, 0, k), IM(M, 1, j), re);
+        REAL im = RE(M, 0, j) * IM(M, 1, k);
+        im = RFMA(IM(M, 0, j), RE(M, 1, k), im);
+        im = RFMA(-RE(M, 0, k), IM(M, 1, j), im);
+        im = RFMA(-IM(M, 0, k), RE(M, 1, j), im);
+        RE(M, 2, c) = re;
+        IM(M, 2, c) = -im;
+    }
+}
+/* rows 0 and 1 of T = X Y */
+static void F(mul_nn_rows01)(const REAL *X, const REAL *Y, REAL *T) {
+    for (int i = 0; i < 2; ++i) for (int j = 0; j < 3; ++j) {
+        REAL re = RE(X, i, 0) * RE(Y, 0, j);
+        re = RFMA(-IM(X, i, 0), IM(Y, 0, j), re);
+        re = RFMA(RE(X, i, 1), RE(Y, 1, j), re);
+        re = RFMA(-IM(X, i, 1), IM(Y, 1, j), re);
+        re = RFMA(RE(X, i, 2), RE(Y, 2, j), re);
+        re = RFMA(-IM(X, i, 2), IM(Y, 2, j), re);
+        REAL im = RE(X, i, 0) * IM(Y, 0, j);
+        im = RFMA(IM(X, i, 0), RE(Y, 0, j), im);
+        im = RFMA(RE(X, i, 1), IM(Y, 1, j), im);
+        im = RFMA(IM(X, i, 1), RE(Y, 1, j), im);
+        im = RFMA(RE(X, i, 2), IM(Y, 2, j), im);
+        im = RFMA(IM(X, i, 2), RE(Y, 2, j), im);
+        RE(T, i, j) = re;
+        IM(T, i, j) = im;
+    }
+}
+/* rows 0 and 1 of T = X Y^dagger (mul_nd's chain) */
+static void F(mul_nd_rows01)(const REAL *X, const REAL *Y, REAL *T) {
+    for (int i = 0; i < 2; ++i) for (int j = 0; j < 3; ++j) {
+        REAL re = RE(X, i, 0) * RE(Y, j, 0);
+        re = RFMA(IM(X, i, 0), IM(Y, j, 0), re);
+        re = RFMA(RE(X, i, 1), RE(Y, j, 1), re);
+        re = RFMA(IM(X, i, 1), IM(Y, j, 1), re);
+        re = RFMA(RE(X, i, 2), RE(Y, j, 2), re);
+        re = RFMA(IM(X, i, 2), IM(Y, j, 2), re);
+        REAL im = IM(X, i, 0) * RE(Y, j, 0);
+        im = RFMA(-RE(X, i, 0), IM(Y, j, 0), im);
+        im = RFMA(IM(X, i, 1), RE(Y, j, 1), im);
+        im = RFMA(-RE(X, i, 1), IM(Y, j, 1), im);
+        im = RFMA(IM(X, i, 2), RE(Y, j, 2), im);
+        im = RFMA(-RE(X, i, 2), IM(Y, j, 2), im);
+        RE(T, i, j) = re;
+        IM(T, i, j) = im;
+    }
+}
+/* A += W^dagger Z */
+static void F(acc_dn)(REAL *A, const REAL *W, const REAL *Z) {
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+        REAL re = RE(A, i, j), im = IM(A, i, j);
+        for (int k = 0; k < 3; ++k) {
+            re = RFMA(RE(W, k, i), RE(Z, k, j), re);
+            re = RFMA(IM(W, k, i), IM(Z, k, j), re);
+        }
+        for (int k = 0; k < 3; ++k) {
+            im = RFMA(RE(W, k, i), IM(Z, k, j), im);
+            im = RFMA(-IM(W, k, i), RE(Z, k, j), im);
+        }
+        RE(A, i, j) = re;
+        IM(A, i, j) = im;
+    }
+}
+
+/* A = sum_{nu != mu} [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)].
+ * FP64 storage: full products (mul_nd / acc_nd forward, mul_dd / acc_nn
+ * backward).  FP32 storage: forward X (Z Y)^+ with Y, V = Z Y and P = X V^+
+ * completed by recon_row2, A += P; backward (Y X)^+ Z with X, W = Y X and Z
+ * completed, A += W^+ Z. */
 static void F(su3_staple)(const REAL *L, const geom *g, const int64_t x[4], int mu, REAL *A) {
     for (int c = 0; c < 18; ++c) A[c] = 0;
     for (int nu = 0; nu < 4; ++nu) {
@@ -151,6 +228,27 @@ static void F(su3_staple)(const REAL *L, const geom *g, const int64_t x[4], int 
         shift(g, x, nu, -1, xmn);
         shift(g, xpm, nu, -1, xpmmn);
         REAL X[18], Y[18], Z[18], T[18];
+        if (sizeof(REAL) == 4) {
+            REAL V[18], P[18];
+            F(load_link)(L, g, xpm, nu, 18, X);
+            F(load_link)(L, g, xpn, mu, 18, Y);
+            F(load_link)(L, g, x, nu, 18, Z);
+            F(recon_row2)(Y);
+            F(mul_nn_rows01)(Z, Y, V);
+            F(recon_row2)(V);
+            F(mul_nd_rows01)(X, V, P);
+            F(recon_row2)(P);
+            for (int c = 0; c < 18; ++c) A[c] = A[c] + P[c];
+            F(load_link)(L, g, xpmmn, nu, 18, X);
+            F(load_link)(L, g, xmn, mu, 18, Y);
+            F(load_link)(L, g, xmn, nu, 18, Z);
+            F(recon_row2)(X);
+            F(mul_nn_rows01)(Y, X, T);
+            F(recon_row2)(T);
+            F(recon_row2)(Z);
+            F(acc_dn)(A, T, Z);
+            continue;
+        }
         F(load_link)(L, g, xpm, nu, 18, X);
         F(load_link)(L, g, xpn, mu, 18, Y);
         F(mul_nd)(X, Y, T);

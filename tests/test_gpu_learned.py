@@ -239,12 +239,17 @@ def test_learned_sweeps_share_one_network_across_threads(ym, thermal):
 
 def test_learned_chain_targets_wilson_action(ym, thermal):
     """Plaquette of learned chains (1 learned proposal + 1 Metropolis hit per
-    link) against plain Metropolis chains (8 hits).  Every chain is
-    thermalised on its own from a cold start with 500 plain sweeps, then runs
-    1200 sweeps of its kind with the first 200 discarded.  The positive-window
-    standard error underestimates the spread of short plain chains (the
-    reference's own four-chain summary reaches a pairwise z of 3.55,
-    su3_confirmation_reference.rs), hence the bound z < 4."""
+    link) against plain Metropolis chains (8 hits): 8 chains of each kind,
+    each thermalised on its own from a cold start with 1000 plain sweeps,
+    then 1200 sweeps of its kind with the first 200 discarded.  The standard
+    error of each kind is the larger of the reference's positive-window
+    pooled error and the spread of the 8 chain means: on 1000-sweep plain
+    chains at 8^4 the positive window underestimates the spread (the
+    plaquette's slow modes outlast the window; the reference's own
+    four-chain summary reaches a pairwise z of 3.55,
+    su3_confirmation_reference.rs), hence also the bound z < 4.  (Eight
+    chains of 4000 sweeps of each kind give 0.594275 +- 0.000090 plain and
+    0.594232 +- 0.000027 learned, z = 0.46.)"""
     m = ym.learned.Mlp(seed=11, dataset_capacity=8 * 8 ** 4 * 4)
     lat = thermal.clone()
     for k in range(4):
@@ -252,11 +257,11 @@ def test_learned_chain_targets_wilson_action(ym, thermal):
         m.collect_pairs(lat)
     m.train(epochs=8, batch=8192, lr=3e-3)
     series = {"plain": [], "learned": []}
-    for chain in range(4):
+    for chain in range(8):
         for kind in series:
             key = KEY + 17 * chain + (1 if kind == "learned" else 0)
             l = ym.Su3Lattice4((8, 8, 8, 8), 32)
-            l.metropolis_sweeps(6.0, 0.35, key, 0, 500, 8)
+            l.metropolis_sweeps(6.0, 0.35, key, 0, 1000, 8)
             vals = []
             for s in range(1200):
                 if kind == "plain":
@@ -267,10 +272,13 @@ def test_learned_chain_targets_wilson_action(ym, thermal):
             series[kind].append(np.array(vals[200:]))
     mean_se = {}
     for kind, rows in series.items():
-        p = ym.stats.pool_chains(np.array(rows), 100)
-        mean_se[kind] = (p.pooled_mean, p.pooled_se)
+        rows = np.array(rows)
+        p = ym.stats.pool_chains(rows, 100)
+        between = rows.mean(axis=1).std(ddof=1) / np.sqrt(len(rows))
+        mean_se[kind] = (p.pooled_mean, max(p.pooled_se, between))
     (m1, s1), (m2, s2) = mean_se["plain"], mean_se["learned"]
     z = abs(m1 - m2) / np.sqrt(s1 ** 2 + s2 ** 2)
+    print(f"[learned] 8^4 plain {m1:.6f} +- {s1:.6f}, learned {m2:.6f} +- {s2:.6f}, z = {z:.2f}")
     assert z < 4.0, (mean_se, z)
 
 

@@ -630,6 +630,123 @@ __device__ __forceinline__ void acc_nn(M3<Real> &A, const M3<Real> &T, const M3<
         }
 }
 
+// ---- FP32 staple from the first two rows of each neighbour link ----
+// A stored SU(3) link determines its third row: row 2 = conj(row 0 x row 1).
+// The FP32 staple reads rows 0 and 1 of every neighbour (3 of the 5 loads
+// of a link, 48 of 72 bytes) and rebuilds a third row only where a product
+// needs it, so that a term costs about what the full product costs.
+
+// Rows 0 and 1 of an FP32 link (entries 0..5: three float4 runs of the tile).
+__device__ __forceinline__ M3<float> load3_rows01(const float2 *__restrict__ L, int vh, int mu, int par, int s) {
+    const float2 *grp = L + (size_t)((mu * 2 + par) * ntile_of(vh)) * 576;
+    const uint32_t tile = (uint32_t)(s >> 6), lane = (uint32_t)(s & 63);
+    __builtin_assume(tile < (1u << 23));  // slots < 2^29 (at most 2^30 sites)
+    const float4 *q = reinterpret_cast<const float4 *>(grp) + (tile * 288u + lane);
+    M3<float> m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float4 t = q[k * 64];
+        m.v[4 * k] = t.x;
+        m.v[4 * k + 1] = t.y;
+        m.v[4 * k + 2] = t.z;
+        m.v[4 * k + 3] = t.w;
+    }
+#pragma unroll
+    for (int c = 12; c < 18; ++c) m.v[c] = 0.0f;
+    return m;
+}
+
+// Row 2 of M from rows 0 and 1: M(2, c) = conj(a_j b_k - a_k b_j) for
+// (c, j, k) cyclic, a = row 0, b = row 1; 8 operations per entry.
+template <typename Real>
+__device__ __forceinline__ void recon_row2(M3<Real> &M) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int j = (c + 1) % 3, k = (c + 2) % 3;
+        Real re = YM_RE(M, 0, j) * YM_RE(M, 1, k);
+        re = fmaR(-YM_IM(M, 0, j), YM_IM(M, 1, k), re);
+        re = fmaR(-YM_RE(M, 0, k), YM_RE(M, 1, j), re);
+        re = fmaR(YM_IM(M, 0, k), YM_IM(M, 1, j), re);
+        Real im = YM_RE(M, 0, j) * YM_IM(M, 1, k);
+        im = fmaR(YM_IM(M, 0, j), YM_RE(M, 1, k), im);
+        im = fmaR(-YM_RE(M, 0, k), YM_IM(M, 1, j), im);
+        im = fmaR(-YM_IM(M, 0, k), YM_RE(M, 1, j), im);
+        YM_RE(M, 2, c) = re;
+        YM_IM(M, 2, c) = -im;
+    }
+}
+
+// Rows 0 and 1 of T = X Y (Y complete).
+template <typename Real>
+__device__ __forceinline__ void mul_nn_rows01(const M3<Real> &X, const M3<Real> &Y, M3<Real> &T) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(X, i, 0) * YM_RE(Y, 0, j);
+            re = fmaR(-YM_IM(X, i, 0), YM_IM(Y, 0, j), re);
+            re = fmaR(YM_RE(X, i, 1), YM_RE(Y, 1, j), re);
+            re = fmaR(-YM_IM(X, i, 1), YM_IM(Y, 1, j), re);
+            re = fmaR(YM_RE(X, i, 2), YM_RE(Y, 2, j), re);
+            re = fmaR(-YM_IM(X, i, 2), YM_IM(Y, 2, j), re);
+            Real im = YM_RE(X, i, 0) * YM_IM(Y, 0, j);
+            im = fmaR(YM_IM(X, i, 0), YM_RE(Y, 0, j), im);
+            im = fmaR(YM_RE(X, i, 1), YM_IM(Y, 1, j), im);
+            im = fmaR(YM_IM(X, i, 1), YM_RE(Y, 1, j), im);
+            im = fmaR(YM_RE(X, i, 2), YM_IM(Y, 2, j), im);
+            im = fmaR(YM_IM(X, i, 2), YM_RE(Y, 2, j), im);
+            YM_RE(T, i, j) = re;
+            YM_IM(T, i, j) = im;
+        }
+}
+
+// Rows 0 and 1 of T = X Y^dagger (Y complete): mul_nd's chain for i < 2.
+template <typename Real>
+__device__ __forceinline__ void mul_nd_rows01(const M3<Real> &X, const M3<Real> &Y, M3<Real> &T) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(X, i, 0) * YM_RE(Y, j, 0);
+            re = fmaR(YM_IM(X, i, 0), YM_IM(Y, j, 0), re);
+            re = fmaR(YM_RE(X, i, 1), YM_RE(Y, j, 1), re);
+            re = fmaR(YM_IM(X, i, 1), YM_IM(Y, j, 1), re);
+            re = fmaR(YM_RE(X, i, 2), YM_RE(Y, j, 2), re);
+            re = fmaR(YM_IM(X, i, 2), YM_IM(Y, j, 2), re);
+            Real im = YM_IM(X, i, 0) * YM_RE(Y, j, 0);
+            im = fmaR(-YM_RE(X, i, 0), YM_IM(Y, j, 0), im);
+            im = fmaR(YM_IM(X, i, 1), YM_RE(Y, j, 1), im);
+            im = fmaR(-YM_RE(X, i, 1), YM_IM(Y, j, 1), im);
+            im = fmaR(YM_IM(X, i, 2), YM_RE(Y, j, 2), im);
+            im = fmaR(-YM_RE(X, i, 2), YM_IM(Y, j, 2), im);
+            YM_RE(T, i, j) = re;
+            YM_IM(T, i, j) = im;
+        }
+}
+
+// A += W^dagger Z (W and Z complete), chained into A's entries.
+template <typename Real>
+__device__ __forceinline__ void acc_dn(M3<Real> &A, const M3<Real> &W, const M3<Real> &Z) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(A, i, j), im = YM_IM(A, i, j);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                re = fmaR(YM_RE(W, k, i), YM_RE(Z, k, j), re);
+                re = fmaR(YM_IM(W, k, i), YM_IM(Z, k, j), re);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                im = fmaR(YM_RE(W, k, i), YM_IM(Z, k, j), im);
+                im = fmaR(-YM_IM(W, k, i), YM_RE(Z, k, j), im);
+            }
+            YM_RE(A, i, j) = re;
+            YM_IM(A, i, j) = im;
+        }
+}
+
 // Plain (not FMA-chained) product used by observables: C = X Y.
 template <typename Real>
 __device__ __forceinline__ M3<Real> mul_nn(const M3<Real> &X, const M3<Real> &Y) {

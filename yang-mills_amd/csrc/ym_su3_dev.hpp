@@ -140,27 +140,42 @@ __device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *_
             if ((k & 1) == 0) acc_nd(A, T, Z);
             else acc_nn(A, T, Z);
         }
-        return;
-    }
-    M3<Real> X[2], Y[2], Z[2];
-    X[0] = load3<Real>(L, g.vh, dir[0], px[0], sx[0]);
-    Y[0] = load3<Real>(L, g.vh, MU, q, sy[0]);
-    Z[0] = load3<Real>(L, g.vh, dir[0], pz[0], sz[0]);
+    } else {
+        // FP32 storage: rows 0 and 1 of each neighbour (load3_rows01), the third
+        // rows rebuilt where a product needs them.  Forward term
+        // X Y^+ Z^+ = X (Z Y)^+: Y completed, V = Z Y (rows 0, 1, then row 2
+        // rebuilt), P = X V^+ (rows 0, 1, then row 2 rebuilt), A += P.  Backward
+        // term X^+ Y^+ Z = (Y X)^+ Z: X completed, W = Y X (rows 0, 1, then row
+        // 2 rebuilt), Z completed, A += W^+ Z.
+        M3<Real> X[2], Y[2], Z[2];
+        X[0] = load3_rows01(L, g.vh, dir[0], px[0], sx[0]);
+        Y[0] = load3_rows01(L, g.vh, MU, q, sy[0]);
+        Z[0] = load3_rows01(L, g.vh, dir[0], pz[0], sz[0]);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const int b = k & 1;
-        if (k + 1 < 6) {
-            X[b ^ 1] = load3<Real>(L, g.vh, dir[k + 1], px[k + 1], sx[k + 1]);
-            Y[b ^ 1] = load3<Real>(L, g.vh, MU, q, sy[k + 1]);
-            Z[b ^ 1] = load3<Real>(L, g.vh, dir[k + 1], pz[k + 1], sz[k + 1]);
-        }
-        M3<Real> T;
-        if (b == 0) {
-            mul_nd(X[b], Y[b], T);
-            acc_nd(A, T, Z[b]);
-        } else {
-            mul_dd(X[b], Y[b], T);
-            acc_nn(A, T, Z[b]);
+        for (int k = 0; k < 6; ++k) {
+            const int b = k & 1;
+            if (k + 1 < 6) {
+                X[b ^ 1] = load3_rows01(L, g.vh, dir[k + 1], px[k + 1], sx[k + 1]);
+                Y[b ^ 1] = load3_rows01(L, g.vh, MU, q, sy[k + 1]);
+                Z[b ^ 1] = load3_rows01(L, g.vh, dir[k + 1], pz[k + 1], sz[k + 1]);
+            }
+            if (b == 0) {
+                M3<Real> V, P;
+                recon_row2(Y[b]);
+                mul_nn_rows01(Z[b], Y[b], V);
+                recon_row2(V);
+                mul_nd_rows01(X[b], V, P);
+                recon_row2(P);
+#pragma unroll
+                for (int c = 0; c < 18; ++c) A.v[c] = A.v[c] + P.v[c];
+            } else {
+                M3<Real> W;
+                recon_row2(X[b]);
+                mul_nn_rows01(Y[b], X[b], W);
+                recon_row2(W);
+                recon_row2(Z[b]);
+                acc_dn(A, W, Z[b]);
+            }
         }
     }
 }
