@@ -98,6 +98,50 @@ __device__ __forceinline__ double plaquette_site(const typename G::Store *__rest
     return acc;
 }
 
+// FP32 SU(3): every link read as rows 0 and 1 (48 of 72 bytes).  A plane's
+// a = U_mu(x) U_nu(x+mu) needs rows 0 and 1 of U_mu(x) and the whole
+// neighbour, whose third row is rebuilt as conj(row 0 x row 1); a's own
+// third row is rebuilt the same way, and likewise c = U_nu(x) U_mu(x+nu).
+template <>
+__device__ __forceinline__ double plaquette_site<G3<float>>(const float2 *__restrict__ L, const Geom &g, int p,
+                                                            int s) {
+    double acc = 0.0;
+    int x[4];
+    if (!site_of(g, p, s, x)) return acc;
+    constexpr int PM[6] = {0, 0, 0, 1, 1, 2}, PN[6] = {1, 2, 3, 2, 3, 3};
+    M3<float> U[4];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) U[mu] = load3_rows01(L, g.vh, mu, p, s);
+    M3<float> A[2], B[2];
+    {
+        int xm[4], xn[4];
+        shifted(g, x, PM[0], +1, xm);
+        shifted(g, x, PN[0], +1, xn);
+        A[0] = load3_rows01(L, g.vh, PN[0], parity_of(xm), slot_of(g, xm));
+        B[0] = load3_rows01(L, g.vh, PM[0], parity_of(xn), slot_of(g, xn));
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int b = k & 1;
+        if (k + 1 < 6) {
+            int xm[4], xn[4];
+            shifted(g, x, PM[k + 1], +1, xm);
+            shifted(g, x, PN[k + 1], +1, xn);
+            A[b ^ 1] = load3_rows01(L, g.vh, PN[k + 1], parity_of(xm), slot_of(g, xm));
+            B[b ^ 1] = load3_rows01(L, g.vh, PM[k + 1], parity_of(xn), slot_of(g, xn));
+        }
+        M3<float> a, c;
+        recon_row2(A[b]);
+        mul_nn_rows01(U[PM[k]], A[b], a);
+        recon_row2(a);
+        recon_row2(B[b]);
+        mul_nn_rows01(U[PN[k]], B[b], c);
+        recon_row2(c);
+        acc += G3<float>::retr_nd(a, c);
+    }
+    return acc;
+}
+
 // (1/3) Re Tr of the SU(3) rectangle from x: R links along M, T along N,
 // R back along M (daggered), T back along N (daggered), multiplied left to
 // right with mul_nn's FMA chain in the storage precision
