@@ -724,6 +724,30 @@ __device__ __forceinline__ void mul_nd_rows01(const M3<Real> &X, const M3<Real> 
         }
 }
 
+// Rows 0 and 1 of T = X^dagger W (X and W complete).
+template <typename Real>
+__device__ __forceinline__ void mul_dn_rows01(const M3<Real> &X, const M3<Real> &W, M3<Real> &T) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Real re = YM_RE(X, 0, i) * YM_RE(W, 0, j);
+            re = fmaR(YM_IM(X, 0, i), YM_IM(W, 0, j), re);
+            re = fmaR(YM_RE(X, 1, i), YM_RE(W, 1, j), re);
+            re = fmaR(YM_IM(X, 1, i), YM_IM(W, 1, j), re);
+            re = fmaR(YM_RE(X, 2, i), YM_RE(W, 2, j), re);
+            re = fmaR(YM_IM(X, 2, i), YM_IM(W, 2, j), re);
+            Real im = YM_RE(X, 0, i) * YM_IM(W, 0, j);
+            im = fmaR(-YM_IM(X, 0, i), YM_RE(W, 0, j), im);
+            im = fmaR(YM_RE(X, 1, i), YM_IM(W, 1, j), im);
+            im = fmaR(-YM_IM(X, 1, i), YM_RE(W, 1, j), im);
+            im = fmaR(YM_RE(X, 2, i), YM_IM(W, 2, j), im);
+            im = fmaR(-YM_IM(X, 2, i), YM_RE(W, 2, j), im);
+            YM_RE(T, i, j) = re;
+            YM_IM(T, i, j) = im;
+        }
+}
+
 // A += W^dagger Z (W and Z complete), chained into A's entries.
 template <typename Real>
 __device__ __forceinline__ void acc_dn(M3<Real> &A, const M3<Real> &W, const M3<Real> &Z) {

@@ -370,8 +370,35 @@ __device__ __forceinline__ void stout_link_dir(const typename Vec2<Real>::type *
         shifted(g, x, nu, +1, xpn);
         shifted(g, x, nu, -1, xmn);
         shifted(g, xmn, mu, +1, xmnpm);
-        const typename G::Mat f = G::mul(G::mul(G::load(in, g, x, nu), G::load(in, g, xpn, mu)), G::dag(G::load(in, g, xpm, nu)));
-        const typename G::Mat b = G::mul(G::mul(G::dag(G::load(in, g, xmn, nu)), G::load(in, g, xmn, mu)), G::load(in, g, xmnpm, nu));
+        typename G::Mat f, b;
+        if constexpr (sizeof(Real) == 4) {
+            // FP32: neighbours read as rows 0 and 1, third rows rebuilt as
+            // conj(row 0 x row 1) where a product needs them.
+            // f = U_nu(x) U_mu(x+nu) U_nu(x+mu)^+ = (X Y) Z^+
+            M3<float> X = load3_rows01(in, g.vh, nu, parity_of(x), slot_of(g, x));
+            M3<float> Y = load3_rows01(in, g.vh, mu, parity_of(xpn), slot_of(g, xpn));
+            M3<float> Z = load3_rows01(in, g.vh, nu, parity_of(xpm), slot_of(g, xpm));
+            M3<float> T;
+            recon_row2(Y);
+            mul_nn_rows01(X, Y, T);
+            recon_row2(T);
+            recon_row2(Z);
+            mul_nd_rows01(T, Z, f);
+            recon_row2(f);
+            // b = U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) = X^+ (Y Z)
+            X = load3_rows01(in, g.vh, nu, parity_of(xmn), slot_of(g, xmn));
+            Y = load3_rows01(in, g.vh, mu, parity_of(xmn), slot_of(g, xmn));
+            Z = load3_rows01(in, g.vh, nu, parity_of(xmnpm), slot_of(g, xmnpm));
+            recon_row2(Z);
+            mul_nn_rows01(Y, Z, T);
+            recon_row2(T);
+            recon_row2(X);
+            mul_dn_rows01(X, T, b);
+            recon_row2(b);
+        } else {
+            f = G::mul(G::mul(G::load(in, g, x, nu), G::load(in, g, xpn, mu)), G::dag(G::load(in, g, xpm, nu)));
+            b = G::mul(G::mul(G::dag(G::load(in, g, xmn, nu)), G::load(in, g, xmn, mu)), G::load(in, g, xmnpm, nu));
+        }
 #pragma unroll
         for (int c = 0; c < 18; ++c) C.v[c] += (double)f.v[c] + (double)b.v[c];
     }
