@@ -157,6 +157,65 @@ __device__ __forceinline__ double rect_re3(const typename Vec2<Real>::type *__re
         shifted(g, c, d, s, y);
         c[0] = y[0]; c[1] = y[1]; c[2] = y[2]; c[3] = y[3];
     };
+    if constexpr (sizeof(Real) == 4) {
+        // FP32: links read as rows 0 and 1.  Rows 0 and 1 of the running
+        // product need only rows 0 and 1 of the product so far and the whole
+        // next link (third row rebuilt as conj(row 0 x row 1)); the loop's
+        // last product P is formed on its upper 2 x 2 block and
+        // Re P(2,2) = Re(P00 P11 - P01 P10), the cofactor of an SU(3) matrix.
+        M3<float> h = load3_rows01(L, g.vh, M, parity_of(c), slot_of(g, c)), t, Lk;
+        step(M, +1);
+        auto fwd = [&](int d) {
+            Lk = load3_rows01(L, g.vh, d, parity_of(c), slot_of(g, c));
+            recon_row2(Lk);
+            mul_nn_rows01(h, Lk, t);
+            h = t;
+            step(d, +1);
+        };
+        auto bwd = [&](int d) {
+            step(d, -1);
+            Lk = load3_rows01(L, g.vh, d, parity_of(c), slot_of(g, c));
+            recon_row2(Lk);
+            mul_nd_rows01(h, Lk, t);
+            h = t;
+        };
+#pragma unroll
+        for (int k = 1; k < R; ++k) fwd(M);
+#pragma unroll
+        for (int k = 0; k < T; ++k) fwd(N);
+#pragma unroll
+        for (int k = 0; k < R; ++k) bwd(M);
+#pragma unroll
+        for (int k = 0; k < T - 1; ++k) bwd(N);
+        step(N, -1);
+        Lk = load3_rows01(L, g.vh, N, parity_of(c), slot_of(g, c));
+        recon_row2(Lk);
+        float pr[2][2], pim[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                float re = YM_RE(h, i, 0) * YM_RE(Lk, j, 0);
+                re = fmaR(YM_IM(h, i, 0), YM_IM(Lk, j, 0), re);
+                re = fmaR(YM_RE(h, i, 1), YM_RE(Lk, j, 1), re);
+                re = fmaR(YM_IM(h, i, 1), YM_IM(Lk, j, 1), re);
+                re = fmaR(YM_RE(h, i, 2), YM_RE(Lk, j, 2), re);
+                re = fmaR(YM_IM(h, i, 2), YM_IM(Lk, j, 2), re);
+                float im = YM_IM(h, i, 0) * YM_RE(Lk, j, 0);
+                im = fmaR(-YM_RE(h, i, 0), YM_IM(Lk, j, 0), im);
+                im = fmaR(YM_IM(h, i, 1), YM_RE(Lk, j, 1), im);
+                im = fmaR(-YM_RE(h, i, 1), YM_IM(Lk, j, 1), im);
+                im = fmaR(YM_IM(h, i, 2), YM_RE(Lk, j, 2), im);
+                im = fmaR(-YM_RE(h, i, 2), YM_IM(Lk, j, 2), im);
+                pr[i][j] = re;
+                pim[i][j] = im;
+            }
+        float d2 = pr[0][0] * pr[1][1];
+        d2 = fmaR(-pim[0][0], pim[1][1], d2);
+        d2 = fmaR(-pr[0][1], pr[1][0], d2);
+        d2 = fmaR(pim[0][1], pim[1][0], d2);
+        return ((double)pr[0][0] + (double)pr[1][1] + (double)d2) / 3.0;
+    }
     M3<Real> h = G::load(L, g, c, M);
     step(M, +1);
 #pragma unroll
