@@ -139,6 +139,59 @@ def test_staple_formula_reproduces_reference_local_action():
             assert abs(-(beta / 3) * np.trace((cand - U) @ A).real - ref.value) <= 1e-12
 
 
+def _staple_fp64_full(aos, n, x, mu):
+    """The staple from whole stored links, in FP64 (numpy)."""
+    def link(y, d):
+        o = O.link_offset(n, tuple(v % n[k] for k, v in enumerate(y)), d, 18)
+        return O.mat_from18(aos[o:o + 18])
+
+    def sh(y, d, s):
+        z = list(y)
+        z[d] += s
+        return z
+
+    A = np.zeros((3, 3), dtype=complex)
+    for nu in range(4):
+        if nu == mu:
+            continue
+        xpm = sh(x, mu, 1)
+        A += link(xpm, nu) @ link(sh(x, nu, 1), mu).conj().T @ link(x, nu).conj().T
+        A += link(sh(xpm, nu, -1), nu).conj().T @ link(sh(x, nu, -1), mu).conj().T @ link(sh(x, nu, -1), nu)
+    return A
+
+
+def test_fp32_staple_from_two_rows_matches_full_products():
+    """The FP32 staple reads rows 0 and 1 of each neighbour and rebuilds third
+    rows as conj(row 0 x row 1) (forward X (Z Y)^+, backward (Y X)^+ Z).  On
+    FP32 hot-start links it equals the staple of the whole stored links,
+    formed in FP64, to FP32 rounding; and it does not change when every
+    stored third row is overwritten, so no neighbour's third row is read.  A
+    wrong conjugation, sign or operand order in the rebuilt rows moves the
+    result by O(1)."""
+    n = (4, 4, 4, 6)
+    buf = dev_hot(n, 32, 9)
+    aos = dev_aos(n, buf, 32)
+    sites = [((0, 0, 0, 0), 0), ((1, 2, 3, 5), 3), ((3, 3, 0, 1), 2), ((2, 1, 3, 4), 1)]
+    got = {}
+    for x, mu in sites:
+        A = np.zeros(18)
+        assert L.ymd_su3_staple(O.ext(n), buf.ctypes.data_as(C.c_void_p), 32, (C.c_int64 * 4)(*x), mu, O.dptr(A)) == 0
+        got[(x, mu)] = A.copy()
+        ref = _staple_fp64_full(aos, n, x, mu)
+        assert np.abs(O.mat_from18(A) - ref).max() <= 2e-6 * np.abs(ref).max(), (x, mu)
+    # overwrite the third row of every stored link and repeat
+    bad = aos.reshape(-1, 18).copy()
+    bad[:, 12:18] = 7.0
+    buf2 = np.zeros_like(buf)
+    assert L.ymd_su3_to_device(O.ext(n), O.dptr(bad.reshape(-1)), buf2.ctypes.data_as(C.c_void_p), 32) == 0
+    for x, mu in sites:
+        A = np.zeros(18)
+        assert L.ymd_su3_staple(O.ext(n), buf2.ctypes.data_as(C.c_void_p), 32, (C.c_int64 * 4)(*x), mu, O.dptr(A)) == 0
+        assert np.array_equal(A, got[(x, mu)]), (x, mu)
+        # the FP64 full-product staple of the overwritten links is far off
+        assert np.abs(O.mat_from18(A) - _staple_fp64_full(bad.reshape(-1), n, x, mu)).max() > 0.1
+
+
 def test_beta0_replay_accepts_everything_and_stays_unitary():
     n = (4, 4, 2, 4)
     buf = dev_hot(n, 64, 5)

@@ -201,6 +201,36 @@ def test_su3_staple_matches_replay_bitwise(ym, prec):
         assert np.array_equal(g, c)
 
 
+def test_su3_fp32_staple_from_two_rows_vs_fp64_full_products(ym):
+    """The device's FP32 staple (rows 0 and 1 of each neighbour, third rows
+    rebuilt) against the staple of the whole stored links formed in FP64,
+    on the same FP32 hot-start links: equal to FP32 rounding."""
+    n = (4, 4, 4, 6)
+    lat = ym.Su3Lattice4(n, 32)
+    lat.hot_start(9)
+    aos = lat.download()
+
+    def link(y, d):
+        o = O.link_offset(n, tuple(v % n[k] for k, v in enumerate(y)), d, 18)
+        return O.mat_from18(aos[o:o + 18])
+
+    def sh(y, d, s):
+        z = list(y)
+        z[d] += s
+        return z
+
+    for x, mu in [((0, 0, 0, 0), 0), ((1, 2, 3, 5), 3), ((3, 3, 0, 1), 2), ((2, 1, 3, 4), 1)]:
+        ref = np.zeros((3, 3), dtype=complex)
+        for nu in range(4):
+            if nu == mu:
+                continue
+            xpm = sh(x, mu, 1)
+            ref += link(xpm, nu) @ link(sh(x, nu, 1), mu).conj().T @ link(x, nu).conj().T
+            ref += link(sh(xpm, nu, -1), nu).conj().T @ link(sh(x, nu, -1), mu).conj().T @ link(sh(x, nu, -1), nu)
+        got = O.mat_from18(lat.staple(x, mu))
+        assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max(), (x, mu)
+
+
 def test_determinism_same_key_same_links(ym):
     n = (4, 4, 4, 4)
     a, b = ym.Su3Lattice4(n, 32), ym.Su3Lattice4(n, 32)
