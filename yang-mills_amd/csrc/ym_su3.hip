@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_su3_metropolis(typename Vec2<Real>::typ
 // larger than the 256 MB cache: the same direction every time evicts each
 // line just before it is needed again).  Tiles are independent, so the
 // direction changes timing only.
-template <typename Real, int MU>
+template <typename Real, int MU, int HITS>
 __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::type *__restrict__ L, Geom g, int p,
                                                            double nbt, double angle, int hits, uint64_t key,
                                                            uint64_t sweep, int reunit, int ntiles,
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::
                 U = ubuf[b].get(lane);
             }
             wave_handoff_post(&freed[b][pair], i + 1, leader);
-            if (has) nacc += su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, key, sweep, reunit);
+            if (has) nacc += su3_update_from<Real, MU, HITS>(L, g, p, s, x, U, A, nbt, angle, hits, key, sweep, reunit);
         }
     }
     const unsigned long long tot = block_sum_u64<512>(nacc, red);
@@ -131,14 +131,26 @@ static void launch_metro_t(void *links, const Geom &g, int mu, int p, double bet
     const int nb = (g.vh + 255) / 256;
     V2 *L = (V2 *)links;
     if (nb >= 256 && g.kper == 0) {
-        // one 8-wave block per CU
+        // one 8-wave block per CU; 10 hits per link (generators 0-7, 0, 1) has
+        // its own fully unrolled instance
         const int grid = 256;
-        switch (mu) {
-        case 0: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 0>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status); break;
-        case 1: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 1>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status); break;
-        case 2: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 2>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status); break;
-        default: hipLaunchKernelGGL((k_su3_metropolis_ws<Real, 3>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status); break;
+#define YM_WS(M, H) hipLaunchKernelGGL((k_su3_metropolis_ws<Real, M, H>), dim3(grid), dim3(512), 0, st, L, g, p, nbt, angle, hits, key, sweep, reunit, nb, tally, status)
+        if (hits == 10) {
+            switch (mu) {
+            case 0: YM_WS(0, 10); break;
+            case 1: YM_WS(1, 10); break;
+            case 2: YM_WS(2, 10); break;
+            default: YM_WS(3, 10); break;
+            }
+        } else {
+            switch (mu) {
+            case 0: YM_WS(0, 0); break;
+            case 1: YM_WS(1, 0); break;
+            case 2: YM_WS(2, 0); break;
+            default: YM_WS(3, 0); break;
+            }
         }
+#undef YM_WS
         return;
     }
     switch (mu) {

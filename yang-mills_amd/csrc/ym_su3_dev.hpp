@@ -96,6 +96,20 @@ __device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t k
     }
 }
 
+// Hits H .. HITS-1 (generator h mod 8) as straight-line code, for a hit
+// count known at compile time: the compiler then schedules consecutive hits
+// together (one hit's Philox draw beside the previous hit's FP64 work),
+// which a loop over a runtime count with a generator switch prevents.
+template <int H, int HITS, int MU>
+__device__ __forceinline__ void su3_hits_unrolled(M3<double> &u, M3<double> &M, uint64_t key, uint32_t site,
+                                                  uint64_t sweep, double angle, double nbt,
+                                                  unsigned long long &nacc) {
+    if constexpr (H < HITS) {
+        su3_hit<H % 8, MU>(u, M, key, site, H, sweep, angle, nbt, nacc);
+        su3_hits_unrolled<H + 1, HITS, MU>(u, M, key, site, sweep, angle, nbt, nacc);
+    }
+}
+
 // Staple A(x) of the link U_MU(x), x of parity p at slot s: the six staples
 // in the storage precision with the fixed FMA chain (mul_nd/acc_nd forward,
 // mul_dd/acc_nn backward, nu ascending).  The three links of term k + 1 are
@@ -183,7 +197,7 @@ __device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *_
 // Update of the link U_MU(x) with staple A: M = U A in FP64, `hits`
 // proposals, optional FP64 reunitarisation, one rounding to storage.
 // Returns the number of accepted proposals.
-template <typename Real, int MU>
+template <typename Real, int MU, int HITS = 0>
 __device__ __forceinline__ unsigned long long su3_update_from(typename Vec2<Real>::type *__restrict__ L,
                                                               const Geom &g, int p, int s, const int x[4],
                                                               const M3<Real> &Uf, const M3<Real> &A, double nbt,
@@ -199,8 +213,10 @@ __device__ __forceinline__ unsigned long long su3_update_at(typename Vec2<Real>:
     return su3_update_from<Real, MU>(L, g, p, s, x, Uf, A, nbt, angle, hits, key, sweep, reunit);
 }
 
-// As su3_update_at with the stored link Uf already loaded.
-template <typename Real, int MU>
+// As su3_update_at with the stored link Uf already loaded.  HITS > 0: the
+// hit count is HITS (the caller passes hits == HITS) and the hits are
+// unrolled; 0: the runtime count `hits`.  The arithmetic is the same.
+template <typename Real, int MU, int HITS>
 __device__ __forceinline__ unsigned long long su3_update_from(typename Vec2<Real>::type *__restrict__ L,
                                                               const Geom &g, int p, int s, const int x[4],
                                                               const M3<Real> &Uf, const M3<Real> &A, double nbt,
@@ -231,6 +247,10 @@ __device__ __forceinline__ unsigned long long su3_update_from(typename Vec2<Real
         }
     const uint32_t site = (uint32_t)lex_of(g, x);
     int h = 0;
+    if constexpr (HITS > 0) {
+        su3_hits_unrolled<0, HITS, MU>(u, M, key, site, sweep, angle, nbt, nacc);
+        h = hits;
+    }
     for (; h + 8 <= hits; h += 8) {
         su3_hit<0, MU>(u, M, key, site, h + 0, sweep, angle, nbt, nacc);
         su3_hit<1, MU>(u, M, key, site, h + 1, sweep, angle, nbt, nacc);
