@@ -142,6 +142,13 @@ static void launch_metro_t(void *links, const Geom &g, int mu, int p, double bet
             case 2: YM_WS(2, 10); break;
             default: YM_WS(3, 10); break;
             }
+        } else if (hits == 8) {
+            switch (mu) {
+            case 0: YM_WS(0, 8); break;
+            case 1: YM_WS(1, 8); break;
+            case 2: YM_WS(2, 8); break;
+            default: YM_WS(3, 8); break;
+            }
         } else {
             switch (mu) {
             case 0: YM_WS(0, 0); break;
