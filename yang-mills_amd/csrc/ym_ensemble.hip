@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_ens_metropolis(typename Vec2<Real>::typ
 // tile list, parity-1 sub-passes walk them backwards.  Per link the
 // arithmetic is k_ens_metropolis's; accepted proposals are added per chain
 // by each consumer wave.
-template <typename Real, int MU>
+template <typename Real, int MU, int HITS>
 __global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::type *__restrict__ links,
                                                            const ChainDesc *__restrict__ cd,
                                                            const int *__restrict__ begin,
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::
         int x[4];
         if (s < g.vh && site_of(g, p, s, x)) {
             const M3<Real> A = abuf[i % NB].get(lane), U = ubuf[i % NB].get(lane);
-            nacc = su3_update_from<Real, MU>(L, g, p, s, x, U, A, nbt, angle, hits, cd[c].key, sweep, reunit);
+            nacc = su3_update_from<Real, MU, HITS>(L, g, p, s, x, U, A, nbt, angle, hits, cd[c].key, sweep, reunit);
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) nacc += __shfl_down(nacc, off, 64);
@@ -398,12 +398,24 @@ void enqueue_sweeps(ym_ensemble *E, double beta, double angle, int hits, int nsw
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p) {
                 if (ntiles >= 512) {
-                    switch (mu) {
-                    case 0: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 0>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
-                    case 1: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 1>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
-                    case 2: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 2>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
-                    default: hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, 3>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc); break;
+                    // 8 and 10 hits per link have fully unrolled instances
+#define YM_EWS(M, H) hipLaunchKernelGGL((ym::k_ens_metropolis_ws<Real, M, H>), dim3(256), dim3(512), 0, E->stream, L, E->d_cd, E->d_metro, E->d_tile_chain, ntiles, p, nbt, angle, hits, E->d_ctr, k, reunit, E->d_acc)
+#define YM_EWS_MU(H)                    \
+    switch (mu) {                       \
+    case 0: YM_EWS(0, H); break;        \
+    case 1: YM_EWS(1, H); break;        \
+    case 2: YM_EWS(2, H); break;        \
+    default: YM_EWS(3, H); break;       \
+    }
+                    if (hits == 8) {
+                        YM_EWS_MU(8)
+                    } else if (hits == 10) {
+                        YM_EWS_MU(10)
+                    } else {
+                        YM_EWS_MU(0)
                     }
+#undef YM_EWS_MU
+#undef YM_EWS
                     continue;
                 }
                 switch (mu) {
