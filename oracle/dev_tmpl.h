@@ -365,6 +365,8 @@ static void F(su3_update_link)(REAL *L, const geom *g, const int64_t x[4], int m
     REAL A[18], Uf[18];
     F(su3_staple)(L, g, x, mu, A);
     F(load_link)(L, g, x, mu, 18, Uf);
+    /* FP32 storage: the link's third row rebuilt from rows 0 and 1 */
+    if (sizeof(REAL) == 4) F(recon_row2)(Uf);
     double u[18], a[18], M[18];
     for (int c = 0; c < 18; ++c) { u[c] = (double)Uf[c]; a[c] = (double)A[c]; }
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {

@@ -1022,4 +1022,28 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
     return t;
 }
 
+// The link a sub-pass updates.  FP32: rows 0 and 1 (non-temporal, as
+// load3_nt) and the third row rebuilt as conj(row 0 x row 1) in FP32, so a
+// sweep never reads a stored third row.  FP64: the whole link.
+__device__ __forceinline__ M3<float> load3_own(const float2 *__restrict__ L, int vh, int mu, int par, int s) {
+    const float2 *grp = L + (size_t)((mu * 2 + par) * ntile_of(vh)) * 576;
+    const uint32_t tile = (uint32_t)(s >> 6), lane = (uint32_t)(s & 63);
+    __builtin_assume(tile < (1u << 23));
+    const float4 *q = reinterpret_cast<const float4 *>(grp) + (tile * 288u + lane);
+    M3<float> m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const nt_f4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(q + k * 64));
+        m.v[4 * k] = t.x;
+        m.v[4 * k + 1] = t.y;
+        m.v[4 * k + 2] = t.z;
+        m.v[4 * k + 3] = t.w;
+    }
+    recon_row2(m);
+    return m;
+}
+__device__ __forceinline__ M3<double> load3_own(const double2 *__restrict__ L, int vh, int mu, int par, int s) {
+    return load3_nt(L, vh, mu, par, s);
+}
+
 }  // namespace ym

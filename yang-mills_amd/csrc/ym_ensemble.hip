@@ -118,7 +118,7 @@ __global__ __launch_bounds__(512) void k_ens_metropolis_ws(typename Vec2<Real>::
         if (s < g.vh && site_of(g, p, s, x)) {
             M3<Real> A;
             su3_staple_at<Real, MU>(L, g, p, s, x, A);
-            const M3<Real> U = load3<Real>(L, g.vh, MU, p, s);
+            const M3<Real> U = load3_own(L, g.vh, MU, p, s);
             abuf[i % NB].put(lane, A);
             ubuf[i % NB].put(lane, U);
         }

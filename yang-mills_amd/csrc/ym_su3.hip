@@ -96,7 +96,7 @@ __global__ __launch_bounds__(512) void k_su3_metropolis_ws(typename Vec2<Real>::
             if (s < g.vh && site_of(g, p, s, x)) {
                 M3<Real> A;
                 su3_staple_at<Real, MU>(L, g, p, s, x, A);
-                const M3<Real> U = load3_nt(L, g.vh, MU, p, s);
+                const M3<Real> U = load3_own(L, g.vh, MU, p, s);
                 abuf[b].put(lane, A);
                 ubuf[b].put(lane, U);
             }

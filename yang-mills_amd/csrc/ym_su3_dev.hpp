@@ -209,7 +209,7 @@ __device__ __forceinline__ unsigned long long su3_update_at(typename Vec2<Real>:
                                                             int p, int s, const int x[4], const M3<Real> &A,
                                                             double nbt, double angle, int hits, uint64_t key,
                                                             uint64_t sweep, int reunit) {
-    const M3<Real> Uf = load3<Real>(L, g.vh, MU, p, s);
+    const M3<Real> Uf = load3_own(L, g.vh, MU, p, s);
     return su3_update_from<Real, MU>(L, g, p, s, x, Uf, A, nbt, angle, hits, key, sweep, reunit);
 }
 
