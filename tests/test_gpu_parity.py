@@ -231,6 +231,29 @@ def test_su3_fp32_staple_from_two_rows_vs_fp64_full_products(ym):
         assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max(), (x, mu)
 
 
+def test_fp32_sweep_never_reads_a_stored_third_row(ym):
+    """FP32 sweeps read rows 0 and 1 of every link (staple neighbours and the
+    updated link) and rebuild third rows: a field whose stored third rows are
+    overwritten sweeps to the same links, bit for bit, once every link has
+    been updated (one full sweep), and so do the accept counts."""
+    n = (4, 4, 4, 6)
+    clean = ym.Su3Lattice4(n, 32)
+    clean.hot_start(21)
+    aos = clean.download()
+    bad = aos.reshape(-1, 18).copy()
+    bad[:, 12:18] = 0.5
+    dirty = ym.Su3Lattice4(n, 32)
+    dirty.upload(bad.reshape(-1), checked=False)
+    for lat in (clean, dirty):
+        lat.metropolis_sweeps(6.0, 0.35, 5, 0, 1, 10)
+    a, b = clean.download(), dirty.download()
+    assert np.array_equal(a, b)
+    pa, ca = clean.metropolis_sweeps(6.0, 0.35, 5, 1, 2, 8)
+    pb, cb = dirty.metropolis_sweeps(6.0, 0.35, 5, 1, 2, 8)
+    assert (pa, ca) == (pb, cb)
+    assert np.array_equal(clean.download(), dirty.download())
+
+
 def test_determinism_same_key_same_links(ym):
     n = (4, 4, 4, 4)
     a, b = ym.Su3Lattice4(n, 32), ym.Su3Lattice4(n, 32)
