@@ -13,9 +13,10 @@
  *  - even/odd checkerboard: for mu = 0..3, parity 0 then 1;
  *  - device layout: per (mu, parity) NC planes of Vh slots, slot of a site =
  *    (row * N3p + x3) >> 1 with N3p = N3 rounded up to even;
- *  - SU(3): FP32 or FP64 staple with a fixed FMA chain, M = U*A in FP64,
- *    closed-form single-generator rotations, Delta S = -(beta/3) Re Tr((R-I)M),
- *    accept iff Delta S <= 0 or u < exp(-Delta S), U <- R U and M <- R M
+ *  - SU(3): FP32 or FP64 staple A with a fixed FMA chain, held in FP64;
+ *    closed-form single-generator rotations, Delta S = -(beta/3) Re Tr((R-I)M)
+ *    with the entries of M = U*A it needs formed from the current U as one
+ *    FMA chain each, accept iff Delta S <= 0 or u < exp(-Delta S), U <- R U
  *    with R the identity rotation (c, s) = (1, 0) when rejected, FP64
  *    Gram-Schmidt reunitarisation before the store;
  *  - SU(2): the reference heat-bath algorithm (truncated-exponential inverse

@@ -292,6 +292,28 @@ static inline void rot_asym(double *m, int i, int j, double c, double s) {
         IM(m, j, k) = fma(c, yi, -(s * xi));
     }
 }
+/* Parts of M = u a used by a proposal (a the staple in FP64), as single
+   FMA chains over k: re_chain continues acc with +-Re (u a)_{rb}, im_chain
+   with +-Im (u a)_{rb}; `first` starts the chain with a product, `neg`
+   subtracts.  Each proposal reads the rows of the current u. */
+static inline double re_chain(const double *u, const double *a, int r, int b, double acc, int first, int neg) {
+    for (int k = 0; k < 3; ++k) {
+        double ur = neg ? -RE(u, r, k) : RE(u, r, k);
+        double ui = neg ? IM(u, r, k) : -IM(u, r, k);
+        acc = (first && k == 0) ? ur * RE(a, k, b) : fma(ur, RE(a, k, b), acc);
+        acc = fma(ui, IM(a, k, b), acc);
+    }
+    return acc;
+}
+static inline double im_chain(const double *u, const double *a, int r, int b, double acc, int first, int neg) {
+    for (int k = 0; k < 3; ++k) {
+        double ur = neg ? -RE(u, r, k) : RE(u, r, k);
+        double ui = neg ? -IM(u, r, k) : IM(u, r, k);
+        acc = (first && k == 0) ? ur * IM(a, k, b) : fma(ur, IM(a, k, b), acc);
+        acc = fma(ui, RE(a, k, b), acc);
+    }
+    return acc;
+}
 /* row i <- (c + i s) row i */
 static inline void rot_phase(double *m, int i, double c, double s) {
     for (int k = 0; k < 3; ++k) {
@@ -367,24 +389,8 @@ static void F(su3_update_link)(REAL *L, const geom *g, const int64_t x[4], int m
     F(load_link)(L, g, x, mu, 18, Uf);
     /* FP32 storage: the link's third row rebuilt from rows 0 and 1 */
     if (sizeof(REAL) == 4) F(recon_row2)(Uf);
-    double u[18], a[18], M[18];
+    double u[18], a[18];
     for (int c = 0; c < 18; ++c) { u[c] = (double)Uf[c]; a[c] = (double)A[c]; }
-    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
-        double re = RE(u, i, 0) * RE(a, 0, j);
-        re = fma(-IM(u, i, 0), IM(a, 0, j), re);
-        re = fma(RE(u, i, 1), RE(a, 1, j), re);
-        re = fma(-IM(u, i, 1), IM(a, 1, j), re);
-        re = fma(RE(u, i, 2), RE(a, 2, j), re);
-        re = fma(-IM(u, i, 2), IM(a, 2, j), re);
-        double im = RE(u, i, 0) * IM(a, 0, j);
-        im = fma(IM(u, i, 0), RE(a, 0, j), im);
-        im = fma(RE(u, i, 1), IM(a, 1, j), im);
-        im = fma(IM(u, i, 1), RE(a, 1, j), im);
-        im = fma(RE(u, i, 2), IM(a, 2, j), im);
-        im = fma(IM(u, i, 2), RE(a, 2, j), im);
-        RE(M, i, j) = re;
-        IM(M, i, j) = im;
-    }
     double nbt = -(P->beta / 3.0);
     uint32_t site = lex_of(g, x);
     static const int blk[8][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 2}, {0, 2}, {1, 2}, {1, 2}, {0, 1}};
@@ -396,29 +402,29 @@ static void F(su3_update_link)(REAL *L, const geom *g, const int64_t x[4], int m
         double uacc = unit53(o[2], o[3]);
         int gen = h & 7;
         double s, cm1, s2 = 0.0, cm2 = 0.0, dre;
+        int bi = blk[gen][0], bj = blk[gen][1];
+        /* t1 = Re (u a)_ii + Re (u a)_jj as one FMA chain */
+        double t1 = re_chain(u, a, bj, bj, re_chain(u, a, bi, bi, 0.0, 1, 0), 0, 0);
         if (gen == 7) {
             double psi = theta * 0x1.279a74590331cp-2;
             sincos_hit_angle(P->angle, psi, &s, &cm1);
             sincos_hit_angle(P->angle, -2.0 * psi, &s2, &cm2);
-            double t1 = RE(M, 0, 0) + RE(M, 1, 1), t2 = IM(M, 0, 0) + IM(M, 1, 1);
+            double t2 = im_chain(u, a, 1, 1, im_chain(u, a, 0, 0, 0.0, 1, 0), 0, 0);
+            double r22 = re_chain(u, a, 2, 2, 0.0, 1, 0), i22 = im_chain(u, a, 2, 2, 0.0, 1, 0);
             double d1 = fma(cm1, t1, -(s * t2));
-            double d2 = fma(cm2, RE(M, 2, 2), -(s2 * IM(M, 2, 2)));
+            double d2 = fma(cm2, r22, -(s2 * i22));
             dre = d1 + d2;
         } else {
             sincos_hit_angle(P->angle, theta * 0.5, &s, &cm1);
             if (gen == 2) {
-                double t1 = RE(M, 0, 0) + RE(M, 1, 1), t2 = IM(M, 0, 0) - IM(M, 1, 1);
+                double t2 = im_chain(u, a, 1, 1, im_chain(u, a, 0, 0, 0.0, 1, 0), 0, 1);
+                dre = fma(cm1, t1, -(s * t2));
+            } else if (gen == 0 || gen == 3 || gen == 5) {
+                double t2 = im_chain(u, a, bj, bi, im_chain(u, a, bi, bj, 0.0, 1, 0), 0, 0);
                 dre = fma(cm1, t1, -(s * t2));
             } else {
-                int i = blk[gen][0], j = blk[gen][1];
-                double t1 = RE(M, i, i) + RE(M, j, j);
-                if (gen == 0 || gen == 3 || gen == 5) {
-                    double t2 = IM(M, i, j) + IM(M, j, i);
-                    dre = fma(cm1, t1, -(s * t2));
-                } else {
-                    double t2 = RE(M, j, i) - RE(M, i, j);
-                    dre = fma(cm1, t1, s * t2);
-                }
+                double t2 = re_chain(u, a, bi, bj, re_chain(u, a, bj, bi, 0.0, 1, 0), 0, 1);
+                dre = fma(cm1, t1, s * t2);
             }
         }
         double dS = nbt * dre;
@@ -431,14 +437,12 @@ static void F(su3_update_link)(REAL *L, const geom *g, const int64_t x[4], int m
             double c2 = accept ? 1.0 + cm2 : 1.0;
             double s2a = accept ? s2 : 0.0;
             rot_phase(u, 0, c, sa); rot_phase(u, 1, c, sa); rot_phase(u, 2, c2, s2a);
-            rot_phase(M, 0, c, sa); rot_phase(M, 1, c, sa); rot_phase(M, 2, c2, s2a);
         } else if (gen == 2) {
             rot_phase(u, 0, c, sa); rot_phase(u, 1, c, -sa);
-            rot_phase(M, 0, c, sa); rot_phase(M, 1, c, -sa);
+        } else if (gen == 0 || gen == 3 || gen == 5) {
+            rot_sym(u, bi, bj, c, sa);
         } else {
-            int i = blk[gen][0], j = blk[gen][1];
-            if (gen == 0 || gen == 3 || gen == 5) { rot_sym(u, i, j, c, sa); rot_sym(M, i, j, c, sa); }
-            else { rot_asym(u, i, j, c, sa); rot_asym(M, i, j, c, sa); }
+            rot_asym(u, bi, bj, c, sa);
         }
     }
     if (P->reunitarize) reunitarize3(u);

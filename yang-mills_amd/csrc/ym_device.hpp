@@ -73,10 +73,22 @@ __device__ __forceinline__ double angle_from_raw(uint32_t lo, uint32_t hi, doubl
     const uint64_t bits = ((uint64_t)((hi & 0xFFFFFu) | 0x3FF00000u) << 32) | lo;
     const double unit = (__builtin_bit_cast(double, bits) - 1.0) + 0x1p-53;
     const double magnitude = proposal_angle * unit;
-    return (hi >> 31) ? magnitude : -magnitude;
+    // (hi >> 31) ? magnitude : -magnitude, the negation as a flip of the
+    // sign bit (one three-input bit operation instead of compare and select)
+    const uint64_t flip = (uint64_t)(~hi & 0x80000000u) << 32;
+    return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, magnitude) ^ flip);
 }
 
 // ------------------------------------------------------------ FP64 math
+// a * b + c, one rounding, as the three-address v_fma_f64.  The compiler
+// turns __builtin_fma with a loop-invariant addend (a series coefficient
+// held in registers) into a register copy plus the two-address
+// v_fmac_f64; this form needs no copy.  Same operation, same bits.
+__device__ __forceinline__ double fma_v3(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 __device__ __forceinline__ double exp_neg(double d) {
     if (!(d <= 745.0)) return 0.0;
     double x = -d;
@@ -137,17 +149,17 @@ __device__ __forceinline__ void sincos_small(double x, double &s, double &cm1) {
 __device__ __forceinline__ void sincos_short(double x, double &s, double &cm1) {
     const double y = -(x * x);
     double ps = 0x1.6124613a86d09p-33;  // 1/13!
-    ps = __builtin_fma(ps, y, 0x1.ae64567f544e4p-26);
-    ps = __builtin_fma(ps, y, 0x1.71de3a556c734p-19);
-    ps = __builtin_fma(ps, y, 0x1.a01a01a01a01ap-13);
-    ps = __builtin_fma(ps, y, 0x1.1111111111111p-7);
-    ps = __builtin_fma(ps, y, 0x1.5555555555555p-3);
+    ps = fma_v3(ps, y, 0x1.ae64567f544e4p-26);
+    ps = fma_v3(ps, y, 0x1.71de3a556c734p-19);
+    ps = fma_v3(ps, y, 0x1.a01a01a01a01ap-13);
+    ps = fma_v3(ps, y, 0x1.1111111111111p-7);
+    ps = fma_v3(ps, y, 0x1.5555555555555p-3);
     s = __builtin_fma(x * y, ps, x);
     double pc = 0x1.1eed8eff8d898p-29;  // 1/12!
-    pc = __builtin_fma(pc, y, 0x1.27e4fb7789f5cp-22);
-    pc = __builtin_fma(pc, y, 0x1.a01a01a01a01ap-16);
-    pc = __builtin_fma(pc, y, 0x1.6c16c16c16c17p-10);
-    pc = __builtin_fma(pc, y, 0x1.5555555555555p-5);
+    pc = fma_v3(pc, y, 0x1.27e4fb7789f5cp-22);
+    pc = fma_v3(pc, y, 0x1.a01a01a01a01ap-16);
+    pc = fma_v3(pc, y, 0x1.6c16c16c16c17p-10);
+    pc = fma_v3(pc, y, 0x1.5555555555555p-5);
     pc = __builtin_fma(pc, y, 0x1p-1);
     cm1 = y * pc;
 }

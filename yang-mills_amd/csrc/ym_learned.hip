@@ -476,18 +476,18 @@ __global__ __launch_bounds__(256) void k_su3_learned(float2 *__restrict__ L, Geo
             }
         }
         if (metro_hits > 0) {
-            M3<double> M = mul_nn(u, from_c3(A));
+            const M3<double> a = from_c3(A);
             const double nbt = -(beta / 3.0);
             for (int h = 0; h < metro_hits; ++h) {
                 switch (h & 7) {
-                case 0: su3_hit<0, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
-                case 1: su3_hit<1, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
-                case 2: su3_hit<2, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
-                case 3: su3_hit<3, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
-                case 4: su3_hit<4, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
-                case 5: su3_hit<5, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
-                case 6: su3_hit<6, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
-                default: su3_hit<7, MU>(u, M, key, site, h, sweep, angle, nbt, macc); break;
+                case 0: su3_hit<0, MU>(u, a, key, site, h, sweep, angle, nbt, macc); break;
+                case 1: su3_hit<1, MU>(u, a, key, site, h, sweep, angle, nbt, macc); break;
+                case 2: su3_hit<2, MU>(u, a, key, site, h, sweep, angle, nbt, macc); break;
+                case 3: su3_hit<3, MU>(u, a, key, site, h, sweep, angle, nbt, macc); break;
+                case 4: su3_hit<4, MU>(u, a, key, site, h, sweep, angle, nbt, macc); break;
+                case 5: su3_hit<5, MU>(u, a, key, site, h, sweep, angle, nbt, macc); break;
+                case 6: su3_hit<6, MU>(u, a, key, site, h, sweep, angle, nbt, macc); break;
+                default: su3_hit<7, MU>(u, a, key, site, h, sweep, angle, nbt, macc); break;
                 }
             }
         }

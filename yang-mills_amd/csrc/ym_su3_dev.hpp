@@ -15,15 +15,18 @@ namespace ym {
 // 2 ulp of FP64, and below 2^-126 only the absolute term matters.  So
 // u < a - tol decides accept and u >= a + tol decides reject exactly as
 // exp_neg would; only u inside that band (probability < 2^-10 per
-// proposal) takes the FP64 path.  Decisions, counts and links are
-// therefore bit for bit those of the plain rule.
+// proposal) takes the FP64 path.  Each test is one FMA and a compare:
+// a (1 - 2^-12) - u > 2^-126 and u - a (1 + 2^-12) >= 2^-126, the FMA
+// rounding its exact difference by at most 2^-53 relative (the sign and
+// all but 2^-179 of the 2^-126 margin survive), far inside the gap between
+// the 2^-17 error and the 2^-12 tolerance.  Decisions, counts and links
+// are therefore bit for bit those of the plain rule.
 __device__ __forceinline__ bool metropolis_accept(double dS, double u) {
     if (!(dS > 0.0)) return true;
     const float t = (float)(dS * -0x1.71547652b82fep+0);
     const double a = (double)__builtin_amdgcn_exp2f(t);
-    const double tol = __builtin_fma(a, 0x1p-12, 0x1p-126);
-    if (u < a - tol) return true;
-    if (u >= a + tol) return false;
+    if (__builtin_fma(a, 0x1.ffep-1, -u) > 0x1p-126) return true;
+    if (__builtin_fma(-a, 0x1.001p+0, u) >= 0x1p-126) return false;
     return u < exp_neg(dS);
 }
 
@@ -37,43 +40,77 @@ __device__ __forceinline__ void sincos_hit(double x, bool short_series, double &
     else sincos_small(x, s, cm1);
 }
 
-// One single-generator proposal on the FP64 link u with M = u A (staple A):
-// generator GEN = h mod 8, angle from the reference's paired grid, accept iff
-// dS <= 0 or u < exp(-dS), then u <- R u and M <- R M, where a rejected
-// proposal applies the identity rotation (c, s) = (1, 0) so the update is
-// branch-free and leaves every entry unchanged.
+// Parts of M = u a (a the staple in FP64) that a proposal needs, each a
+// single FMA chain over k: re_chain continues acc with +-Re (u a)_{rb},
+// im_chain with +-Im (u a)_{rb}; FIRST starts the chain with a product,
+// NEG subtracts.  A proposal reads rows of the current u, so no copy of
+// M is carried or rotated between hits.
+template <bool FIRST, bool NEG>
+__device__ __forceinline__ double re_chain(const M3<double> &u, const M3<double> &a, int r, int b, double acc) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double ur = NEG ? -YM_RE(u, r, k) : YM_RE(u, r, k);
+        const double ui = NEG ? YM_IM(u, r, k) : -YM_IM(u, r, k);
+        acc = (FIRST && k == 0) ? ur * YM_RE(a, k, b) : __builtin_fma(ur, YM_RE(a, k, b), acc);
+        acc = __builtin_fma(ui, YM_IM(a, k, b), acc);
+    }
+    return acc;
+}
+template <bool FIRST, bool NEG>
+__device__ __forceinline__ double im_chain(const M3<double> &u, const M3<double> &a, int r, int b, double acc) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double ur = NEG ? -YM_RE(u, r, k) : YM_RE(u, r, k);
+        const double ui = NEG ? -YM_IM(u, r, k) : YM_IM(u, r, k);
+        acc = (FIRST && k == 0) ? ur * YM_IM(a, k, b) : __builtin_fma(ur, YM_IM(a, k, b), acc);
+        acc = __builtin_fma(ui, YM_RE(a, k, b), acc);
+    }
+    return acc;
+}
+
+// One single-generator proposal on the FP64 link u with staple a (FP64):
+// generator GEN = h mod 8, angle from the reference's paired grid, accept
+// iff dS <= 0 or u < exp(-dS), then u <- R u, where a rejected proposal
+// applies the identity rotation (c, s) = (1, 0) so the update is
+// branch-free and leaves every entry unchanged.  dS = -(beta/3) Re tr((R -
+// 1) u a) from the 2 x 2 block (i, j) of u a (all three diagonal entries
+// for generator 7), with t1 = Re M_ii + Re M_jj and t2 as one chain each.
 template <int GEN, int MU>
-__device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t key, uint32_t site, int h,
+__device__ __forceinline__ void su3_hit(M3<double> &u, const M3<double> &a, uint64_t key, uint32_t site, int h,
                                         uint64_t sweep, double angle, double nbt, unsigned long long &nacc) {
     constexpr int bi = (GEN <= 2 || GEN == 7) ? 0 : (GEN <= 4 ? 0 : 1);
     constexpr int bj = (GEN <= 2 || GEN == 7) ? 1 : 2;
     uint32_t o[4];
     draw(key, site, MU, KIND_SU3_HIT, (uint32_t)h, sweep, o);
-    const double theta = angle_from_raw(o[0], o[1], angle);
+    // theta for generator 7; theta / 2 for generators 0-6, drawn directly:
+    // (angle / 2) * unit is exactly (angle * unit) / 2, since scaling by 2^-1
+    // commutes with the rounding.
+    const double ang = angle_from_raw(o[0], o[1], GEN == 7 ? angle : 0.5 * angle);
     const double uacc = unit53(o[2], o[3]);
     const bool short_series = angle <= kShortSeriesMaxAngle;  // uniform over the grid
     double sn, cm1, s2 = 0.0, cm2 = 0.0, dre;
+    const double t1 = re_chain<false, false>(u, a, bj, bj, re_chain<true, false>(u, a, bi, bi, 0.0));
     if constexpr (GEN == 7) {
-        const double psi = theta * 0x1.279a74590331cp-2;
+        const double psi = ang * 0x1.279a74590331cp-2;
         sincos_hit(psi, short_series, sn, cm1);
         sincos_hit(-2.0 * psi, short_series, s2, cm2);
-        const double t1 = YM_RE(M, 0, 0) + YM_RE(M, 1, 1), t2 = YM_IM(M, 0, 0) + YM_IM(M, 1, 1);
+        const double t2 = im_chain<false, false>(u, a, 1, 1, im_chain<true, false>(u, a, 0, 0, 0.0));
+        const double r22 = re_chain<true, false>(u, a, 2, 2, 0.0);
+        const double i22 = im_chain<true, false>(u, a, 2, 2, 0.0);
         const double d1 = __builtin_fma(cm1, t1, -(sn * t2));
-        const double d2 = __builtin_fma(cm2, YM_RE(M, 2, 2), -(s2 * YM_IM(M, 2, 2)));
+        const double d2 = __builtin_fma(cm2, r22, -(s2 * i22));
         dre = d1 + d2;
     } else if constexpr (GEN == 2) {
-        sincos_hit(theta * 0.5, short_series, sn, cm1);
-        const double t1 = YM_RE(M, 0, 0) + YM_RE(M, 1, 1), t2 = YM_IM(M, 0, 0) - YM_IM(M, 1, 1);
+        sincos_hit(ang, short_series, sn, cm1);
+        const double t2 = im_chain<false, true>(u, a, 1, 1, im_chain<true, false>(u, a, 0, 0, 0.0));
         dre = __builtin_fma(cm1, t1, -(sn * t2));
     } else if constexpr (GEN == 0 || GEN == 3 || GEN == 5) {
-        sincos_hit(theta * 0.5, short_series, sn, cm1);
-        const double t1 = YM_RE(M, bi, bi) + YM_RE(M, bj, bj);
-        const double t2 = YM_IM(M, bi, bj) + YM_IM(M, bj, bi);
+        sincos_hit(ang, short_series, sn, cm1);
+        const double t2 = im_chain<false, false>(u, a, bj, bi, im_chain<true, false>(u, a, bi, bj, 0.0));
         dre = __builtin_fma(cm1, t1, -(sn * t2));
     } else {
-        sincos_hit(theta * 0.5, short_series, sn, cm1);
-        const double t1 = YM_RE(M, bi, bi) + YM_RE(M, bj, bj);
-        const double t2 = YM_RE(M, bj, bi) - YM_RE(M, bi, bj);
+        sincos_hit(ang, short_series, sn, cm1);
+        const double t2 = re_chain<false, true>(u, a, bi, bj, re_chain<true, false>(u, a, bj, bi, 0.0));
         dre = __builtin_fma(cm1, t1, sn * t2);
     }
     const double dS = nbt * dre;
@@ -85,14 +122,12 @@ __device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t k
         const double c2 = accept ? 1.0 + cm2 : 1.0;
         const double s2a = accept ? s2 : 0.0;
         rot_phase(u, 0, c, sa); rot_phase(u, 1, c, sa); rot_phase(u, 2, c2, s2a);
-        rot_phase(M, 0, c, sa); rot_phase(M, 1, c, sa); rot_phase(M, 2, c2, s2a);
     } else if constexpr (GEN == 2) {
         rot_phase(u, 0, c, sa); rot_phase(u, 1, c, -sa);
-        rot_phase(M, 0, c, sa); rot_phase(M, 1, c, -sa);
     } else if constexpr (GEN == 0 || GEN == 3 || GEN == 5) {
-        rot_sym(u, bi, bj, c, sa); rot_sym(M, bi, bj, c, sa);
+        rot_sym(u, bi, bj, c, sa);
     } else {
-        rot_asym(u, bi, bj, c, sa); rot_asym(M, bi, bj, c, sa);
+        rot_asym(u, bi, bj, c, sa);
     }
 }
 
@@ -101,12 +136,12 @@ __device__ __forceinline__ void su3_hit(M3<double> &u, M3<double> &M, uint64_t k
 // together (one hit's Philox draw beside the previous hit's FP64 work),
 // which a loop over a runtime count with a generator switch prevents.
 template <int H, int HITS, int MU>
-__device__ __forceinline__ void su3_hits_unrolled(M3<double> &u, M3<double> &M, uint64_t key, uint32_t site,
+__device__ __forceinline__ void su3_hits_unrolled(M3<double> &u, const M3<double> &a, uint64_t key, uint32_t site,
                                                   uint64_t sweep, double angle, double nbt,
                                                   unsigned long long &nacc) {
     if constexpr (H < HITS) {
-        su3_hit<H % 8, MU>(u, M, key, site, H, sweep, angle, nbt, nacc);
-        su3_hits_unrolled<H + 1, HITS, MU>(u, M, key, site, sweep, angle, nbt, nacc);
+        su3_hit<H % 8, MU>(u, a, key, site, H, sweep, angle, nbt, nacc);
+        su3_hits_unrolled<H + 1, HITS, MU>(u, a, key, site, sweep, angle, nbt, nacc);
     }
 }
 
@@ -194,7 +229,7 @@ __device__ __forceinline__ void su3_staple_at(const typename Vec2<Real>::type *_
     }
 }
 
-// Update of the link U_MU(x) with staple A: M = U A in FP64, `hits`
+// Update of the link U_MU(x) with staple A (held in FP64), `hits`
 // proposals, optional FP64 reunitarisation, one rounding to storage.
 // Returns the number of accepted proposals.
 template <typename Real, int MU, int HITS = 0>
@@ -223,54 +258,38 @@ __device__ __forceinline__ unsigned long long su3_update_from(typename Vec2<Real
                                                               double angle, int hits, uint64_t key, uint64_t sweep,
                                                               int reunit) {
     unsigned long long nacc = 0;
-    M3<double> u, M;
+    M3<double> u, a;
 #pragma unroll
-    for (int c = 0; c < 18; ++c) u.v[c] = (double)Uf.v[c];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            double re = YM_RE(u, i, 0) * (double)YM_RE(A, 0, j);
-            re = __builtin_fma(-YM_IM(u, i, 0), (double)YM_IM(A, 0, j), re);
-            re = __builtin_fma(YM_RE(u, i, 1), (double)YM_RE(A, 1, j), re);
-            re = __builtin_fma(-YM_IM(u, i, 1), (double)YM_IM(A, 1, j), re);
-            re = __builtin_fma(YM_RE(u, i, 2), (double)YM_RE(A, 2, j), re);
-            re = __builtin_fma(-YM_IM(u, i, 2), (double)YM_IM(A, 2, j), re);
-            double im = YM_RE(u, i, 0) * (double)YM_IM(A, 0, j);
-            im = __builtin_fma(YM_IM(u, i, 0), (double)YM_RE(A, 0, j), im);
-            im = __builtin_fma(YM_RE(u, i, 1), (double)YM_IM(A, 1, j), im);
-            im = __builtin_fma(YM_IM(u, i, 1), (double)YM_RE(A, 1, j), im);
-            im = __builtin_fma(YM_RE(u, i, 2), (double)YM_IM(A, 2, j), im);
-            im = __builtin_fma(YM_IM(u, i, 2), (double)YM_RE(A, 2, j), im);
-            YM_RE(M, i, j) = re;
-            YM_IM(M, i, j) = im;
-        }
+    for (int c = 0; c < 18; ++c) {
+        u.v[c] = (double)Uf.v[c];
+        a.v[c] = (double)A.v[c];
+    }
     const uint32_t site = (uint32_t)lex_of(g, x);
     int h = 0;
     if constexpr (HITS > 0) {
-        su3_hits_unrolled<0, HITS, MU>(u, M, key, site, sweep, angle, nbt, nacc);
+        su3_hits_unrolled<0, HITS, MU>(u, a, key, site, sweep, angle, nbt, nacc);
         h = hits;
     }
     for (; h + 8 <= hits; h += 8) {
-        su3_hit<0, MU>(u, M, key, site, h + 0, sweep, angle, nbt, nacc);
-        su3_hit<1, MU>(u, M, key, site, h + 1, sweep, angle, nbt, nacc);
-        su3_hit<2, MU>(u, M, key, site, h + 2, sweep, angle, nbt, nacc);
-        su3_hit<3, MU>(u, M, key, site, h + 3, sweep, angle, nbt, nacc);
-        su3_hit<4, MU>(u, M, key, site, h + 4, sweep, angle, nbt, nacc);
-        su3_hit<5, MU>(u, M, key, site, h + 5, sweep, angle, nbt, nacc);
-        su3_hit<6, MU>(u, M, key, site, h + 6, sweep, angle, nbt, nacc);
-        su3_hit<7, MU>(u, M, key, site, h + 7, sweep, angle, nbt, nacc);
+        su3_hit<0, MU>(u, a, key, site, h + 0, sweep, angle, nbt, nacc);
+        su3_hit<1, MU>(u, a, key, site, h + 1, sweep, angle, nbt, nacc);
+        su3_hit<2, MU>(u, a, key, site, h + 2, sweep, angle, nbt, nacc);
+        su3_hit<3, MU>(u, a, key, site, h + 3, sweep, angle, nbt, nacc);
+        su3_hit<4, MU>(u, a, key, site, h + 4, sweep, angle, nbt, nacc);
+        su3_hit<5, MU>(u, a, key, site, h + 5, sweep, angle, nbt, nacc);
+        su3_hit<6, MU>(u, a, key, site, h + 6, sweep, angle, nbt, nacc);
+        su3_hit<7, MU>(u, a, key, site, h + 7, sweep, angle, nbt, nacc);
     }
     for (; h < hits; ++h) {
         switch (h & 7) {
-        case 0: su3_hit<0, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
-        case 1: su3_hit<1, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
-        case 2: su3_hit<2, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
-        case 3: su3_hit<3, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
-        case 4: su3_hit<4, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
-        case 5: su3_hit<5, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
-        case 6: su3_hit<6, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
-        default: su3_hit<7, MU>(u, M, key, site, h, sweep, angle, nbt, nacc); break;
+        case 0: su3_hit<0, MU>(u, a, key, site, h, sweep, angle, nbt, nacc); break;
+        case 1: su3_hit<1, MU>(u, a, key, site, h, sweep, angle, nbt, nacc); break;
+        case 2: su3_hit<2, MU>(u, a, key, site, h, sweep, angle, nbt, nacc); break;
+        case 3: su3_hit<3, MU>(u, a, key, site, h, sweep, angle, nbt, nacc); break;
+        case 4: su3_hit<4, MU>(u, a, key, site, h, sweep, angle, nbt, nacc); break;
+        case 5: su3_hit<5, MU>(u, a, key, site, h, sweep, angle, nbt, nacc); break;
+        case 6: su3_hit<6, MU>(u, a, key, site, h, sweep, angle, nbt, nacc); break;
+        default: su3_hit<7, MU>(u, a, key, site, h, sweep, angle, nbt, nacc); break;
         }
     }
     if (reunit) reunitarize3(u);
