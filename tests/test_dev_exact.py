@@ -192,6 +192,79 @@ def test_fp32_staple_from_two_rows_matches_full_products():
         assert np.abs(O.mat_from18(A) - _staple_fp64_full(bad.reshape(-1), n, x, mu)).max() > 0.1
 
 
+# Gell-Mann matrices lambda_1 .. lambda_8; generator h mod 8 of a hit is
+# T = lambda_{h mod 8 + 1} / 2 (the reference's generator(), su3.rs:325-352).
+_GELL_MANN = [np.array(m, dtype=complex) for m in (
+    [[0, 1, 0], [1, 0, 0], [0, 0, 0]],
+    [[0, -1j, 0], [1j, 0, 0], [0, 0, 0]],
+    [[1, 0, 0], [0, -1, 0], [0, 0, 0]],
+    [[0, 0, 1], [0, 0, 0], [1, 0, 0]],
+    [[0, 0, -1j], [0, 0, 0], [1j, 0, 0]],
+    [[0, 0, 0], [0, 0, 1], [0, 1, 0]],
+    [[0, 0, 0], [0, 0, -1j], [0, 1j, 0]],
+    np.diag([1, 1, -2]) / math.sqrt(3.0),
+)]
+
+
+def _numpy_hit_sweep(n, aos, beta, angle, hits, key, sweep):
+    """One checkerboard sweep written from the definitions: for each (mu,
+    parity) sub-pass and link, hit h draws Philox at (site, mu | h << 5,
+    sweep) under the key, takes theta on the reference's grid from words 0-1
+    and the uniform from words 2-3, proposes U' = expm(i theta T_{h mod 8}) U,
+    and accepts iff dS = -(beta/3) Re Tr((U' - U) A) <= 0 or u < exp(-dS),
+    A the FP64 staple of the current field.  Returns the links and the
+    accept count."""
+    aos = aos.copy()
+    acc = 0
+    for mu in range(4):
+        for par in range(2):
+            for x in np.ndindex(*n):
+                if sum(x) % 2 != par:
+                    continue
+                A = _staple_fp64_full(aos, n, x, mu)
+                off = O.link_offset(n, x, mu, 18)
+                u = O.mat_from18(aos[off:off + 18])
+                site = ((x[0] * n[1] + x[1]) * n[2] + x[2]) * n[3] + x[3]
+                for h in range(hits):
+                    o = philox((site, mu | (h << 5), sweep & 0xFFFFFFFF, sweep >> 32),
+                               (key & 0xFFFFFFFF, key >> 32))
+                    raw = (o[1] << 32) | o[0]
+                    theta = angle * float(2 * (raw & ((1 << 52) - 1)) + 1) * 2.0 ** -53
+                    theta = theta if raw >> 63 else -theta
+                    uacc = float(((o[3] << 32) | o[2]) >> 11) * 2.0 ** -53
+                    cand = expm(1j * theta * _GELL_MANN[h % 8] / 2) @ u
+                    dS = -(beta / 3) * np.trace((cand - u) @ A).real
+                    if dS <= 0 or uacc < math.exp(-dS):
+                        u = cand
+                        acc += 1
+                aos[off:off + 18] = O.mat_to18(u)
+    return aos, acc
+
+
+def test_replay_hits_follow_generator_exponentials():
+    """The replay's closed-form hits (rotations of rows by cos/sin of theta/2,
+    phases for lambda_3 and lambda_8, Delta S from the 2 x 2 block of U A)
+    against the definitions in numpy: one FP64 sweep of 10 hits per link
+    (every generator, 0 and 1 twice) on a hot 4x4x2x4 lattice at beta = 6,
+    angle 0.8, without reunitarisation.  The accept count must agree exactly
+    and every link to 1e-12; the same comparison with theta of the opposite
+    sign or a generator off by one fails by far."""
+    n = (4, 4, 2, 4)
+    beta, angle, hits, key, sweep = 6.0, 0.8, 10, 0x1234ABCD5678EF01, 3
+    buf = dev_hot(n, 64, 11)
+    start = dev_aos(n, buf, 64)
+    p = O.DevSu3Params(beta=beta, angle=angle, hits=hits, reunitarize=0, key=key, sweep=sweep)
+    pr, ac = C.c_uint64(), C.c_uint64()
+    assert L.ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), 64, C.byref(p),
+                                      C.byref(pr), C.byref(ac)) == 0
+    got = dev_aos(n, buf, 64)
+    want, acc = _numpy_hit_sweep(n, start, beta, angle, hits, key, sweep)
+    assert pr.value == hits * 4 * int(np.prod(n))
+    assert 0 < ac.value < pr.value
+    assert ac.value == acc, (ac.value, acc)
+    assert np.max(np.abs(got - want)) <= 1e-12, np.max(np.abs(got - want))
+
+
 def test_beta0_replay_accepts_everything_and_stays_unitary():
     n = (4, 4, 2, 4)
     buf = dev_hot(n, 64, 5)
