@@ -96,16 +96,16 @@ __device__ __forceinline__ double exp_neg(double d) {
     double r = __builtin_fma(-k, 0x1.62e42fefa39efp-1, x);
     r = __builtin_fma(-k, 0x1.abc9e3b39803fp-56, r);
     double p = 0x1.6124613a86d09p-33;
-    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);
-    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
-    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
-    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
-    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
-    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
-    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
-    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = fma_v3(p, r, 0x1.1eed8eff8d898p-29);
+    p = fma_v3(p, r, 0x1.ae64567f544e4p-26);
+    p = fma_v3(p, r, 0x1.27e4fb7789f5cp-22);
+    p = fma_v3(p, r, 0x1.71de3a556c734p-19);
+    p = fma_v3(p, r, 0x1.a01a01a01a01ap-16);
+    p = fma_v3(p, r, 0x1.a01a01a01a01ap-13);
+    p = fma_v3(p, r, 0x1.6c16c16c16c17p-10);
+    p = fma_v3(p, r, 0x1.1111111111111p-7);
+    p = fma_v3(p, r, 0x1.5555555555555p-5);
+    p = fma_v3(p, r, 0x1.5555555555555p-3);
     p = __builtin_fma(p, r, 0x1.0000000000000p-1);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
@@ -120,24 +120,24 @@ __device__ __forceinline__ double exp_neg(double d) {
 __device__ __forceinline__ void sincos_small(double x, double &s, double &cm1) {
     const double y = -(x * x);
     double ps = 0x1.2f49b46814157p-57;  // 1/19!
-    ps = __builtin_fma(ps, y, 0x1.952c77030ad4ap-49);
-    ps = __builtin_fma(ps, y, 0x1.ae7f3e733b81fp-41);
-    ps = __builtin_fma(ps, y, 0x1.6124613a86d09p-33);
-    ps = __builtin_fma(ps, y, 0x1.ae64567f544e4p-26);
-    ps = __builtin_fma(ps, y, 0x1.71de3a556c734p-19);
-    ps = __builtin_fma(ps, y, 0x1.a01a01a01a01ap-13);
-    ps = __builtin_fma(ps, y, 0x1.1111111111111p-7);
-    ps = __builtin_fma(ps, y, 0x1.5555555555555p-3);
+    ps = fma_v3(ps, y, 0x1.952c77030ad4ap-49);
+    ps = fma_v3(ps, y, 0x1.ae7f3e733b81fp-41);
+    ps = fma_v3(ps, y, 0x1.6124613a86d09p-33);
+    ps = fma_v3(ps, y, 0x1.ae64567f544e4p-26);
+    ps = fma_v3(ps, y, 0x1.71de3a556c734p-19);
+    ps = fma_v3(ps, y, 0x1.a01a01a01a01ap-13);
+    ps = fma_v3(ps, y, 0x1.1111111111111p-7);
+    ps = fma_v3(ps, y, 0x1.5555555555555p-3);
     s = __builtin_fma(x * y, ps, x);
     double pc = 0x1.e542ba4020225p-62;  // 1/20!
-    pc = __builtin_fma(pc, y, 0x1.6827863b97d97p-53);
-    pc = __builtin_fma(pc, y, 0x1.ae7f3e733b81fp-45);
-    pc = __builtin_fma(pc, y, 0x1.93974a8c07c9dp-37);
-    pc = __builtin_fma(pc, y, 0x1.1eed8eff8d898p-29);
-    pc = __builtin_fma(pc, y, 0x1.27e4fb7789f5cp-22);
-    pc = __builtin_fma(pc, y, 0x1.a01a01a01a01ap-16);
-    pc = __builtin_fma(pc, y, 0x1.6c16c16c16c17p-10);
-    pc = __builtin_fma(pc, y, 0x1.5555555555555p-5);
+    pc = fma_v3(pc, y, 0x1.6827863b97d97p-53);
+    pc = fma_v3(pc, y, 0x1.ae7f3e733b81fp-45);
+    pc = fma_v3(pc, y, 0x1.93974a8c07c9dp-37);
+    pc = fma_v3(pc, y, 0x1.1eed8eff8d898p-29);
+    pc = fma_v3(pc, y, 0x1.27e4fb7789f5cp-22);
+    pc = fma_v3(pc, y, 0x1.a01a01a01a01ap-16);
+    pc = fma_v3(pc, y, 0x1.6c16c16c16c17p-10);
+    pc = fma_v3(pc, y, 0x1.5555555555555p-5);
     pc = __builtin_fma(pc, y, 0x1p-1);
     cm1 = y * pc;
 }
@@ -186,16 +186,16 @@ __device__ __forceinline__ double log_pos(double w) {
     double f = (m - 1.0) / (m + 1.0);
     double f2 = f * f;
     double q = 0x1.642c8590b2164p-5;
-    q = __builtin_fma(q, f2, 0x1.8618618618618p-5);
-    q = __builtin_fma(q, f2, 0x1.af286bca1af28p-5);
-    q = __builtin_fma(q, f2, 0x1.e1e1e1e1e1e1ep-5);
-    q = __builtin_fma(q, f2, 0x1.1111111111111p-4);
-    q = __builtin_fma(q, f2, 0x1.3b13b13b13b14p-4);
-    q = __builtin_fma(q, f2, 0x1.745d1745d1746p-4);
-    q = __builtin_fma(q, f2, 0x1.c71c71c71c71cp-4);
-    q = __builtin_fma(q, f2, 0x1.2492492492492p-3);
-    q = __builtin_fma(q, f2, 0x1.999999999999ap-3);
-    q = __builtin_fma(q, f2, 0x1.5555555555555p-2);
+    q = fma_v3(q, f2, 0x1.8618618618618p-5);
+    q = fma_v3(q, f2, 0x1.af286bca1af28p-5);
+    q = fma_v3(q, f2, 0x1.e1e1e1e1e1e1ep-5);
+    q = fma_v3(q, f2, 0x1.1111111111111p-4);
+    q = fma_v3(q, f2, 0x1.3b13b13b13b14p-4);
+    q = fma_v3(q, f2, 0x1.745d1745d1746p-4);
+    q = fma_v3(q, f2, 0x1.c71c71c71c71cp-4);
+    q = fma_v3(q, f2, 0x1.2492492492492p-3);
+    q = fma_v3(q, f2, 0x1.999999999999ap-3);
+    q = fma_v3(q, f2, 0x1.5555555555555p-2);
     double tf = 2.0 * f;
     double lm = __builtin_fma(tf * f2, q, tf);
     double de = (double)e;
@@ -211,17 +211,17 @@ __device__ __forceinline__ double log1p_dev(double y) {
 __device__ __forceinline__ double expm1_nonpos(double x) {
     if (x > -0x1.62e42fefa39efp-2) {
         double p = 0x1.93974a8c07c9dp-37;
-        p = __builtin_fma(p, x, 0x1.6124613a86d09p-33);
-        p = __builtin_fma(p, x, 0x1.1eed8eff8d898p-29);
-        p = __builtin_fma(p, x, 0x1.ae64567f544e4p-26);
-        p = __builtin_fma(p, x, 0x1.27e4fb7789f5cp-22);
-        p = __builtin_fma(p, x, 0x1.71de3a556c734p-19);
-        p = __builtin_fma(p, x, 0x1.a01a01a01a01ap-16);
-        p = __builtin_fma(p, x, 0x1.a01a01a01a01ap-13);
-        p = __builtin_fma(p, x, 0x1.6c16c16c16c17p-10);
-        p = __builtin_fma(p, x, 0x1.1111111111111p-7);
-        p = __builtin_fma(p, x, 0x1.5555555555555p-5);
-        p = __builtin_fma(p, x, 0x1.5555555555555p-3);
+        p = fma_v3(p, x, 0x1.6124613a86d09p-33);
+        p = fma_v3(p, x, 0x1.1eed8eff8d898p-29);
+        p = fma_v3(p, x, 0x1.ae64567f544e4p-26);
+        p = fma_v3(p, x, 0x1.27e4fb7789f5cp-22);
+        p = fma_v3(p, x, 0x1.71de3a556c734p-19);
+        p = fma_v3(p, x, 0x1.a01a01a01a01ap-16);
+        p = fma_v3(p, x, 0x1.a01a01a01a01ap-13);
+        p = fma_v3(p, x, 0x1.6c16c16c16c17p-10);
+        p = fma_v3(p, x, 0x1.1111111111111p-7);
+        p = fma_v3(p, x, 0x1.5555555555555p-5);
+        p = fma_v3(p, x, 0x1.5555555555555p-3);
         p = __builtin_fma(p, x, 0x1p-1);
         return __builtin_fma(p, x * x, x);
     }
