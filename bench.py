@@ -195,7 +195,10 @@ def sweep_record(ym_amd, L, hits, beta, steps, warmup, hot_start, repeats=4):
                      "bytes_per_launch": bytes_per_launch, "mean_launch_us": avg_launch_s * 1e6,
                      "note": "achieved = 648 B per update (9 link transfers of 18 FP32 reals) / launch time; "
                              "the staple reads rows 0 and 1 of each neighbour link (48 of 72 B) and rebuilds "
-                             "the third, so the measured traffic is below the algorithmic bytes"},
+                             "the third, so the measured traffic is below the algorithmic bytes; what limits "
+                             "the kernel is the VALU issue of its FP32 staple and FP64 hits (rocprofv3 VALUBusy "
+                             "64% at 32^4, 10 hits; with its loads served from the caches it is only 2-3.5 us "
+                             "faster)"},
     }
 
 
