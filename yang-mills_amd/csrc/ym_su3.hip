@@ -4,14 +4,15 @@
 // single-generator Metropolis.  One thread owns one link of the sub-pass; no
 // link it reads is written in the same sub-pass, so all threads run
 // independently.  Per link:
-//   1. staple A (6 terms) in the storage precision with a fixed FMA chain;
-//   2. M = U A in FP64, so that Re Tr(U' A) - Re Tr(U A) = Re Tr((R - I) M)
-//      for U' = R U costs a few flops per hit;
-//   3. `hits` proposals, generator h mod 8, angle from the reference's paired
-//      grid driven by Philox(site, mu, hit, sweep); accept iff dS <= 0 or
-//      u < exp(-dS) in FP64; U <- R U and M <- R M with R replaced by the
+//   1. staple A (6 terms) in the storage precision with a fixed FMA chain,
+//      then held in FP64 with the link U;
+//   2. `hits` proposals, generator h mod 8, angle from the reference's paired
+//      grid driven by Philox(site, mu, hit, sweep); Re Tr(U' A) - Re Tr(U A)
+//      = Re Tr((R - I) U A) for U' = R U from the 2 x 2 block of U A that R
+//      touches, each entry part one FMA chain over the current U; accept iff
+//      dS <= 0 or u < exp(-dS) in FP64; U <- R U with R replaced by the
 //      identity rotation (c, s) = (1, 0) when rejected (branch-free);
-//   4. FP64 Gram-Schmidt reunitarisation, then one rounding to storage.
+//   3. FP64 Gram-Schmidt reunitarisation, then one rounding to storage.
 // The CPU replay in the oracle performs the same operations in the same order.
 #include "ym_device.hpp"
 #include "ym_internal.hpp"
