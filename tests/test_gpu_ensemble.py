@@ -43,19 +43,20 @@ def test_chains_match_single_lattices_bitwise(ym, prec):
             assert np.array_equal(ops[lev, :, 0], pl) and np.array_equal(ops[lev, :, 1], rc)
 
 
-@pytest.mark.parametrize("prec", [32, 64])
-def test_wave_specialised_batch_matches_single_lattices(ym, prec):
+@pytest.mark.parametrize("prec,hits", [(32, 8), (64, 8), (32, 10), (64, 10), (32, 3)])
+def test_wave_specialised_batch_matches_single_lattices(ym, prec, hits):
     """From 512 tiles of 256 links per sub-pass the batched sweep runs the
     wave-specialised kernel (staples staged in LDS, tiles of all chains in one
-    list); every chain still equals a single lattice swept alone."""
+    list; 8 and 10 hits have unrolled instances, other counts loop); every
+    chain still equals a single lattice swept alone."""
     geoms = BIG
     ks = keys(ym, len(geoms))
     ens = ym.Ensemble(geoms, ks, prec)
-    res = ens.run_cycles(6.0, 0.35, hits=8, sweeps_per_cycle=2, n_cycles=1, stout_levels=0, rho=0.1,
+    res = ens.run_cycles(6.0, 0.35, hits=hits, sweeps_per_cycle=2, n_cycles=1, stout_levels=0, rho=0.1,
                          use_graph=False)
     for c, (g, k) in enumerate(zip(geoms, ks)):
         lat = ym.Su3Lattice4(g, prec)
-        _, acc = lat.metropolis_sweeps(6.0, 0.35, k, 0, 2, 8)
+        _, acc = lat.metropolis_sweeps(6.0, 0.35, k, 0, 2, hits)
         assert int(res.accepts[c]) == acc, c
         assert np.array_equal(ens.download_chain(c), lat.download()), c
         assert res.plaquette[0, c] == lat.average_plaquette()
