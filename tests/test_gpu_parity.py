@@ -116,19 +116,21 @@ def test_su3_metropolis_matches_replay_bitwise(ym, prec, n, beta, hits, angle):
         assert ga == gp  # every proposal accepted at beta = 0
 
 
-@pytest.mark.parametrize("prec", [32, 64])
-def test_su3_metropolis_wave_specialised_path_bitwise(ym, prec):
+@pytest.mark.parametrize("prec,hits,angle", [(32, 8, 0.35), (64, 8, 0.35), (32, 10, 0.35), (64, 10, 0.35),
+                                              (32, 3, 0.8)])
+def test_su3_metropolis_wave_specialised_path_bitwise(ym, prec, hits, angle):
     """Lattices of >= 256 tiles take the wave-specialised sub-pass kernel
-    (staples of the next tile overlapping the hits of the current one); one
-    sweep of 16^3x32 against the device-order replay."""
+    (staples of the next tile overlapping the hits of the current one; 8 and
+    10 hits have unrolled instances, other counts loop); one sweep of
+    16^3x32 against the device-order replay."""
     n = (16, 16, 16, 32)
     key = ym.derive_su3_chain_seed(0x2c02198105010002, 2)
     lat = ym.Su3Lattice4(n, prec)
     lat.hot_start(key ^ 0x77)
     buf = dev_buffer(n, 18, prec)
     assert O.lib().ymd_su3_hot_start(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, key ^ 0x77) == 0
-    gp, ga = lat.metropolis_sweeps(6.0, 0.35, key, 5, 1, 8)
-    params = O.DevSu3Params(beta=6.0, angle=0.35, hits=8, reunitarize=1, key=key, sweep=5)
+    gp, ga = lat.metropolis_sweeps(6.0, angle, key, 5, 1, hits)
+    params = O.DevSu3Params(beta=6.0, angle=angle, hits=hits, reunitarize=1, key=key, sweep=5)
     cp, ca = C.c_uint64(), C.c_uint64()
     assert O.lib().ymd_su3_metropolis_sweep(O.ext(n), buf.ctypes.data_as(C.c_void_p), prec, C.byref(params),
                                            C.byref(cp), C.byref(ca)) == 0
